@@ -1,0 +1,216 @@
+"""Headline benchmark: denoised Mpixel/s of the statistics-and-denoise path.
+
+Workload (one step = one frame through the hot path, inputs resident in HBM):
+1920x1080 RGB, 256 samples per pixel, the mt5 lambda list
+[.01 .25 .5 .75 1 3] (5 Box-Cox transforms + identity), M4C statistics;
+per-pixel prepass (Jarque-Bera, Bonett, LEV), variance filter and
+multi-transform Welch filter, radius 10, alpha 0.005, z 2.80703, spatial
+sd 10, albedo/normal/depth range weights.  Synthetic seeded samples and
+features (seed 2) are generated on the device before the timed region.
+A step = reset + accumulate + denoise_vars + denoise_film.
+
+Prints ONE JSON line.  Per-kernel durations come from HIP events recorded
+around every launch on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_LANE_OPS = 39.3   # T lane-ops/s: 256 CU x 64 lanes x 2.4 GHz, one FP32 op per lane per clock
+PEAK_HBM = 8000.0      # GB/s
+# Algorithmic work constants (lane-ops; a transcendental counts 4).
+WELCH_OPS_PER_TEST = 16
+VAR_OPS_PER_TEST = 8
+WEIGHT_OPS_PER_PAIR = 20
+ACC_OPS_PER_SET_CHANNEL = 22
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def window_pairs(W: int, H: int, r: int) -> int:
+    """In-image neighbour pairs (p, q != p) of a (2r+1)^2 window."""
+    sx = sum(W - abs(d) for d in range(-r, r + 1) if abs(d) < W)
+    sy = sum(H - abs(d) for d in range(-r, r + 1) if abs(d) < H)
+    return sx * sy - W * H
+
+
+def kernel_work(W, H, spp, S, r):
+    P = W * H
+    pairs = window_pairs(W, H, r)
+    tests = pairs * S * 3
+    T = S - 1
+    acc_ops = P * spp * (S * 3 * ACC_OPS_PER_SET_CHANNEL + 4 * (3 + 3 * T + 3 * S))
+    return {
+        "welch_filter": {"ops": tests * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
+        "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
+        "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) * 2, "bound": "valu"},
+        "prepass": {"bytes": P * (4 + S * 5 * 3 * 4 + S * (9 * 4 + 4)), "bound": "hbm"},
+    }
+
+
+def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
+    import numpy as np
+
+    import oracle
+
+    x0, y0, w, h = region
+    t0 = time.perf_counter()
+    f = oracle.features(sc, region)
+    smp = oracle.samples(sc, f["albedo"], 0, spp, nthreads=nthreads, region=region)
+    log(f"cpu baseline inputs generated in {time.perf_counter() - t0:.1f}s")
+    feats = [(f["albedo"], 0.02), (f["normal"], 0.1), (f["depth"], 0.5)]
+    t0 = time.perf_counter()
+    n, st = oracle.accumulate(smp, lambdas, 5, nthreads=nthreads)
+    t1 = time.perf_counter()
+    jb, lnS2, se2, m2C = oracle.prepass(n, st, p.var_z)
+    vD, vCD = oracle.denoise_vars(n, st[:, 1], lnS2.astype(np.float32), se2.astype(np.float32),
+                                  m2C.astype(np.float32), r, p.ds_factor, p.var_z, feats, nthreads=nthreads)
+    oracle.denoise_film(n, st[:, 0], vCD.astype(np.float32), st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
+                        jb=jb.astype(np.float32), feats=feats, nthreads=nthreads)
+    t2 = time.perf_counter()
+    total = t2 - t0
+    return {
+        "value": round(w * h / total / 1e6, 4),
+        "unit": "Mpixel/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{w}x{h} crop at ({x0},{y0}) of the same 1080p frame, {spp} spp mt5; oracle accumulate "
+                  f"(reference FP32 recurrence, 16x16 tiles) {t1 - t0:.2f}s + FP64 prepass/variance/Welch "
+                  f"filters {t2 - t1:.2f}s on {nthreads} threads",
+        "accumulate_gsample_s": round(w * h * spp / (t1 - t0) / 1e9, 4),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--radius", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-region", default="800,400,480,270")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by image "
+              f"and by row bands, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import stater_amd as sa
+    from stater_amd.workload import MT5_LAMBDAS, Features, SampleChunk, make_scene, synth_features, synth_samples
+
+    W, H, spp, r = args.width, args.height, args.spp, args.radius
+    P = W * H
+    lambdas = MT5_LAMBDAS
+    ctx = sa.Context(0, timing=True)
+    sc = make_scene(2, 2, W, H)
+    feats = Features(W, H)
+    synth_features(ctx, sc, feats)
+    chunk = SampleChunk(W, H, spp)
+    log(f"generating {P * spp / 1e6:.0f}M samples ({chunk.buf.nbytes / 2**30:.1f} GiB) on the device")
+    synth_samples(ctx, sc, chunk, 0, feats)
+    ctx.synchronize()
+    bs = sa.BufferSet(ctx, W, H, lambdas, 5)
+    S = len(set(float(x) for x in lambdas) | {1.0})
+    p = sa.default_params(radius=r)
+    g = feats.gbuffers()
+    desc = chunk.desc()
+
+    def step():
+        bs.reset()
+        bs.accumulate(desc)
+        bs.denoise_vars(p, g)
+        bs.denoise_film(p, g)
+
+    for i in range(args.warmup):
+        step()
+        ctx.synchronize()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    log(f"timed {args.steps} steps in {t1 - t0:.3f}s")
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = P * args.steps / (t1 - t0) / 1e6
+
+    tim = ctx.timing()
+    work = kernel_work(W, H, spp, S, r)
+    kernels = {}
+    for name, (cnt, ms) in tim.items():
+        avg = ms / max(cnt, 1)
+        k = {"launches": cnt, "avg_ms": round(avg, 4)}
+        w = work.get(name, {})
+        if "ops" in w:
+            k["tlane_op_s"] = round(w["ops"] / (avg * 1e-3) / 1e12, 3)
+            k["valu_frac"] = round(k["tlane_op_s"] / PEAK_LANE_OPS, 4)
+        if "bytes" in w:
+            k["gb_s"] = round(w["bytes"] / (avg * 1e-3) / 1e9, 1)
+            k["hbm_frac"] = round(k["gb_s"] / PEAK_HBM, 4)
+        kernels[name] = k
+    dom = max(kernels, key=lambda n: kernels[n]["avg_ms"] * kernels[n]["launches"])
+    dk = kernels[dom]
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    if "tlane_op_s" in dk:
+        roofline = {"bound": "valu", "achieved": dk["tlane_op_s"], "peak": PEAK_LANE_OPS, "unit": "Tlane-op/s",
+                    "frac": dk["valu_frac"], "traffic": traffic, "kernel": dom}
+    else:
+        roofline = {"bound": "hbm", "achieved": dk.get("gb_s"), "peak": PEAK_HBM, "unit": "GB/s",
+                    "frac": dk.get("hbm_frac"), "traffic": traffic, "kernel": dom}
+    acc = kernels.get("accumulate")
+    gsample = round(P * spp / (acc["avg_ms"] * 1e-3) / 1e9, 3) if acc else None
+
+    cpu = None
+    if not args.no_cpu:
+        region = tuple(int(v) for v in args.cpu_region.split(","))
+        nthreads = max(1, min(os.cpu_count() or 1, 16))
+        log(f"cpu baseline on a {region[2]}x{region[3]} crop, {nthreads} threads")
+        cpu = cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads)
+
+    out = {
+        "metric": "denoised Mpixel/s (1080p, mt5, r=10) + p50/p99 job latency at fixed error",
+        "value": round(value, 3),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox sample model, device-generated before timing)",
+        "config": {"workload": f"{W}x{H} RGB, {spp} spp, mt5 (6 lambda slots, 5 Box-Cox), M4C, "
+                               f"prepass + variance filter + Welch filter, r={r}, alpha=0.005, "
+                               f"albedo/normal/depth weights", "width": W, "height": H, "spp": spp,
+                   "radius": r, "sets": S},
+        "gsample_s": gsample,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    bs.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

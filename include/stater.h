@@ -1,0 +1,269 @@
+/*
+ * stater.h — C-ABI of the MI355X (gfx950) statistics-and-denoise library.
+ *
+ * This is the drop-in boundary for StatER's statistics-buffer and denoise path.
+ * Every entry point is plain C: device pointers, sizes and an int status.
+ * No exception crosses it.  Each call enqueues its kernels on the HIP stream it
+ * is given (NULL = the context's own stream) and returns without waiting,
+ * like the reference's calls on a cv::cuda::Stream.
+ *
+ * Reference interfaces replaced (paths relative to the StatER checkout):
+ *   stater_create / stater_destroy          cv::cuda::stat_denoiser::setup()
+ *                                           (src/statistics/estimator.h:383)
+ *   stater_print_macro_configuration        stat_denoiser::printMacroConfiguration()
+ *                                           (src/statistics/estimator.cpp:519-521)
+ *   stater_bufset_create / _view            Estimator::AllocateBuffers + BUF_LIST /
+ *                                           FILM_BUF_LIST names
+ *                                           (estimator.cpp:83-245, estimator.h:459-479,
+ *                                           name scheme estimator.cpp:21-24)
+ *   stater_accumulate                       StatTile<T>::Add[Transform]SampleM1..M4C
+ *                                           (estimator.h:201-324) fed by
+ *                                           statpath.cpp:313-349, followed by
+ *                                           Estimator::Merge[Transform]Tiles
+ *                                           (estimator.cpp:267-343) and Upload
+ *                                           (estimator.cpp:346-353): the moments
+ *                                           are built directly in device memory
+ *   stater_denoise_vars                     Estimator::DenoiseVars ->
+ *                                           stat_denoiser::denoiseVars<float3>
+ *                                           (estimator.cpp:365-416)
+ *   stater_denoise_film                     Estimator::DenoiseMeans ->
+ *                                           stat_denoiser::denoiseFilm
+ *                                           (estimator.cpp:419-459)
+ *   stater_calc_target_spp                  Estimator::CalcTargetSPPs ->
+ *                                           stat_denoiser::calcTargetSPP<float3>
+ *                                           (estimator.cpp:498-513)
+ *   stater_calc_mean_vars                   Estimator::CalcMeanVars / calcMatMeanVar
+ *                                           (estimator.cpp:462-496)
+ *   stater_synchronize                      stat_denoiser::synchronize(stream)
+ *                                           (estimator.cpp:515-517)
+ *
+ * Error convention: 0 = ok, negative = STATER_E_*; stater_last_error() gives a
+ * thread-local message for the last failing call on the calling thread.
+ * Threading: calls on distinct buffer sets are thread-safe; calls on one
+ * buffer set must be ordered by the caller (one stream per set).
+ */
+#ifndef STATER_H
+#define STATER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STATER_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define STATER_OK             0
+#define STATER_E_ARG         -1  /* invalid argument (size, pointer, name) */
+#define STATER_E_HIP         -2  /* a HIP runtime call failed */
+#define STATER_E_NOMEM       -3  /* device allocation failed */
+#define STATER_E_STATE       -4  /* call not valid in the object's state */
+#define STATER_E_UNSUPPORTED -5  /* configuration not supported */
+#define STATER_E_NOTFOUND    -6  /* unknown buffer name */
+
+/* ---- limits ------------------------------------------------------------ */
+#define STATER_MAX_LAMBDAS   8   /* Box-Cox lambda slots per buffer set */
+#define STATER_MAX_RADIUS   20   /* filter radius (reference default is 20) */
+#define STATER_MAX_GBUFFERS  4   /* feature buffers per filter */
+#define STATER_MAX_EDGES    16   /* synthetic-scene step edges */
+
+typedef struct stater_ctx stater_ctx;
+typedef struct stater_bufset stater_bufset;
+
+typedef enum { STATER_F32 = 0, STATER_I32 = 1 } stater_dtype;
+typedef enum { STATER_PLANAR = 0, STATER_INTERLEAVED = 1 } stater_layout;
+
+/* A 2-D image plane in device memory.  PLANAR: channel c starts at
+ * ptr + c*channel_stride_bytes.  INTERLEAVED: channels are adjacent
+ * (the reference's CV_32FC3 GpuMat, a PtrStepSzb with step = pitch_bytes). */
+typedef struct {
+    void   *ptr;
+    int64_t pitch_bytes;
+    int64_t channel_stride_bytes;
+    int32_t width;
+    int32_t height;
+    int32_t channels;
+    int32_t dtype;   /* stater_dtype */
+    int32_t layout;  /* stater_layout */
+    int32_t reserved;
+} stater_plane;
+
+/* ---- context ----------------------------------------------------------- */
+#define STATER_CTX_TIMING 1u  /* record HIP events around every kernel launch */
+
+int  stater_create(int device, unsigned flags, stater_ctx **out);
+int  stater_destroy(stater_ctx *ctx);
+/* The context's own HIP stream (hipStream_t), created at default priority. */
+int  stater_ctx_stream(stater_ctx *ctx, void **stream_out);
+const char *stater_last_error(void);
+int  stater_abi_version(void);
+/* Prints the denoiser-side configuration (the reference's macro dump). */
+void stater_print_macro_configuration(void);
+
+/* Per-kernel timing (contexts created with STATER_CTX_TIMING).  Reading the
+ * statistics synchronises the recorded events. */
+int  stater_timing_reset(stater_ctx *ctx);
+int  stater_timing_count(stater_ctx *ctx);
+int  stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len,
+                       uint64_t *launches, double *total_ms);
+
+/* ---- buffer sets (the Estimator's statistics buffers, device resident) ---
+ * One set holds, for a W x H RGB film: the sample count n (i32) and, for the
+ * untransformed samples (set 0) and every distinct Box-Cox lambda != 1
+ * (sets 1..T), the statistics mean, m2, m3, m4, c2X, c2Y, m1 plus the derived
+ * jb, lnS2, se2, m2C, varD, varCD, discr; film-level meanVar, meanD, filmD and
+ * targetspp.  All planes are PLANAR f32 with a 256-byte aligned row pitch.
+ *
+ * max_moment follows the reference's maxMoment (statpath.cpp:1154):
+ * 1 = mean, 2 = +m2, 3 = +m3, 4 = +m4, 5 = +m4 with c2X, c2Y and m1 ("M4C").
+ * lambdas are the reference's "lambdas" list (scenes/render-denoise-mt5.pbrt:26);
+ * n_lambdas = 0 means no transform (the reference's transform = false). */
+#define STATER_BUFSET_NO_DERIVED 1u  /* accumulate-only set: skip denoise planes */
+
+int stater_bufset_create(stater_ctx *ctx, int width, int height,
+                         int n_lambdas, const float *lambdas, int max_moment,
+                         unsigned flags, stater_bufset **out);
+int stater_bufset_destroy(stater_bufset *bs);
+/* Zero the accumulated statistics (n and every moment) on the stream. */
+int stater_bufset_reset(stater_bufset *bs, void *stream);
+/* Look up a buffer by the reference's name: "n", "film", "filmD", "targetspp",
+ * "t0-b<j>-film-<stat>" (untransformed statistics of lambda slot j) or
+ * "t0-b<j>-<stat>" (Box-Cox statistics of slot j), stat in
+ * mean m2 m3 m4 c2X c2Y m1 jb lnS2 se2 m2C varD varCD discr meanVar meanD n. */
+int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out);
+/* Bytes of device memory the set holds (the admission unit for serving). */
+int64_t stater_bufset_bytes(const stater_bufset *bs);
+/* Bytes a set of this shape would hold, without allocating. */
+int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas,
+                                const float *lambdas, int max_moment, unsigned flags);
+
+/* ---- accumulation -----------------------------------------------------
+ * A sample chunk: for sample index j (0..spp-1) and component k (R,G,B,u,v)
+ * the plane starts at data + j*sample_stride_bytes + k*comp_stride_bytes, with
+ * rows pitch_bytes apart.  (u,v) is the in-pixel position in [0,1)^2
+ * (statpath.cpp:339).  counts (optional, i32 plane) limits pixel p to its
+ * first counts[p] samples of the chunk. */
+typedef struct {
+    const float   *data;
+    int64_t        pitch_bytes;
+    int64_t        comp_stride_bytes;
+    int64_t        sample_stride_bytes;
+    int32_t        spp;
+    int32_t        sanitize;   /* NaN / negative / inf guard of statpath.cpp:313-332 */
+    const int32_t *counts;
+    int64_t        counts_pitch_bytes;
+} stater_samples;
+
+/* Sequential update in the reference's exact FP32 operation order for the
+ * untransformed statistics (bit-identical to estimator.h:278-305). */
+#define STATER_ACC_SEQUENTIAL 0
+/* Per-call partial moments merged into the resident ones in FP64
+ * (Chan-Golub-LeVeque / Pebay); c2X, c2Y, m1 continue sequentially. */
+#define STATER_ACC_MERGE64    1
+
+int stater_accumulate(stater_bufset *bs, const stater_samples *samples,
+                      int mode, void *stream);
+
+/* ---- denoising ---------------------------------------------------------- */
+typedef struct {
+    int32_t radius;       /* window radius (filterRadius), <= STATER_MAX_RADIUS */
+    float   ds_factor;    /* spatial factor -0.5/filterSD^2; 0 = uniform (estimator.h:353) */
+    float   var_z;        /* variance-test z (varZValue, default 2.80703) */
+    float   alpha;        /* two-sided Welch level for the mean filter (0.005) */
+    int32_t mt_mode;      /* 2: accept iff every lambda set accepts;
+                             3: test only in the set with the smallest jb */
+    int32_t cascade;      /* 1: Welch uses the denoised variance (varD/varCD) */
+    int32_t use_levc;     /* 1: cascade uses the LEV-corrected varCD */
+    int32_t reserved;
+} stater_denoise_params;
+
+/* Feature (G-) buffers for the range weights exp(dr * |g_p - g_q|^2),
+ * dr = -0.5/sd^2 (estimator.cpp:10-18). */
+typedef struct {
+    int32_t      count;
+    int32_t      reserved;
+    stater_plane planes[STATER_MAX_GBUFFERS];
+    float        dr_factors[STATER_MAX_GBUFFERS];
+} stater_gbuffers;
+
+void stater_default_params(stater_denoise_params *p);
+
+/* Per-pixel Jarque-Bera, Bonett lnS2/se2, LEV m2C, then the variance filter
+ * varD/varCD for every set.  gbufs may be NULL. */
+int stater_denoise_vars(stater_bufset *bs, const stater_denoise_params *p,
+                        const stater_gbuffers *gbufs, void *stream);
+/* Welch-test mean filter.  film: the mean to filter and to test set 0 with
+ * (NULL = the set-0 sample mean).  film_out: NULL = the set's filmD plane.
+ * Writes discr (per set) and filmD. */
+int stater_denoise_film(stater_bufset *bs, const stater_denoise_params *p,
+                        const stater_gbuffers *gbufs, const stater_plane *film,
+                        const stater_plane *film_out, void *stream);
+/* The plain method: Welch test on the raw sample variances, uniform weights,
+ * no features; writes meanD (the mean over accepted neighbours) and varD of
+ * set 0 (the mean of their sample variances). */
+int stater_denoise_plain(stater_bufset *bs, int radius, float alpha, void *stream);
+
+/* Adaptive-sampling map: distributes budget_spp * W * H samples over the
+ * pixels in proportion to the relative standard error of the mean,
+ * sqrt(varD/n)/max(|mean|,1e-3) averaged over RGB, with n + target <= cap.
+ * The targets sum to the budget exactly unless every pixel is capped.
+ * out: NULL = the set's targetspp plane. */
+int stater_calc_target_spp(stater_bufset *bs, uint64_t budget_spp, uint32_t cap,
+                           const stater_plane *out, void *stream);
+/* meanVar = m2 / ((n-1) n) of set 0 (estimator.cpp:462-474). */
+int stater_calc_mean_vars(stater_bufset *bs, void *stream);
+int stater_synchronize(void *stream);
+
+/* ---- synthetic scenes (workload generator, not the hot path) -------------
+ * Seeded, counter-based (Philox4x32-10) sample model: every draw is keyed by
+ * (seed, image_id) and addressed by (pixel, sample index), so any launch
+ * shape produces the same samples. */
+typedef struct {
+    int32_t  width, height;
+    uint32_t seed, image_id;
+    int32_t  model;          /* 1 = plain (128x128 case), 2 = features + edges */
+    int32_t  n_edges;
+    double   base;
+    double   amp[3][3];      /* [sinusoid][channel] */
+    double   fx[3], fy[3];
+    double   cos_phi[3][3], sin_phi[3][3];
+    int32_t  edge_vertical[STATER_MAX_EDGES];
+    double   edge_pos[STATER_MAX_EDGES];
+    double   edge_factor[STATER_MAX_EDGES];
+    double   s_min, s_max;
+    double   firefly_mult;
+    uint32_t firefly_thresh; /* firefly iff a 32-bit draw < thresh */
+    int32_t  albedo_modulation;
+    int32_t  cell_size;
+    int32_t  pad0;
+    double   normal_jitter;
+    double   depth_min, depth_max;
+} stater_scene;
+
+int stater_scene_init(stater_scene *s, int model, uint32_t seed, uint32_t image_id,
+                      int width, int height);
+/* Albedo (3 ch), normal (3 ch) and depth (1 ch) planes, PLANAR f32. */
+int stater_synth_features(stater_ctx *ctx, const stater_scene *s,
+                          const stater_plane *albedo, const stater_plane *normal,
+                          const stater_plane *depth, void *stream);
+/* Writes samples first_sample .. first_sample+spp-1 of every pixel into a
+ * chunk laid out as stater_samples describes.  albedo may be NULL unless the
+ * scene modulates by albedo. */
+int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
+                         const stater_plane *albedo, uint32_t first_sample,
+                         const stater_samples *out, void *stream);
+
+/* ---- testing ------------------------------------------------------------
+ * Runs the Welch mean filter (plain != 0: the plain method) and also writes
+ * every accept/reject decision to dec (device, uint8):
+ * dec[((y*W + x)*(2r+1)^2 + (dy+r)*(2r+1) + (dx+r))*3 + c]. */
+int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *p,
+                                const stater_gbuffers *gbufs, const stater_plane *film,
+                                int plain, uint8_t *dec, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STATER_H */

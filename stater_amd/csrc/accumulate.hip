@@ -1,0 +1,297 @@
+// accumulate.hip — per-sample moment accumulation on gfx950.
+//
+// Replaces the CPU hot loop of the reference: for every sample the renderer
+// calls StatTile<Vec3>::AddTransformSampleM4C once per lambda slot
+// (src/statistics/statpath.cpp:341-344 -> estimator.h:312-324 -> :278-305),
+// then merges the tile into host buffers (estimator.cpp:301-343) that are
+// uploaded to the GPU every iteration (estimator.cpp:346-353).  Here the
+// statistics never leave device memory.
+//
+// Mapping: one thread per (pixel, channel).  A 192-thread workgroup is three
+// wavefronts, one per RGB channel, over 64 consecutive pixels of a row, so
+// every sample-plane load of a wavefront is one coalesced 256-byte segment and
+// the three channel wavefronts share the (u, v) lines through L1.  Each thread
+// keeps its pixel-channel's statistics of every set in registers for the
+// whole chunk and walks the samples in order, so the update stays the
+// reference's sequential recurrence.
+//
+// Numerics:
+//  * set 0 (untransformed) runs the reference update in its exact FP32
+//    operation order (no contraction, IEEE division and square root: sqrtf,
+//    not __fsqrt_rn, which lowers to the approximate v_sqrt_f32), so its
+//    mean, m2, m3, m4, c2X, c2Y and m1 are bit-identical to StatTile<Vec3>
+//    fed the same samples;
+//  * the Box-Cox sets are accumulated relative to a per-pixel reference
+//    point: with L = ln x1 of the pixel's first sample, K = (e^{lL} - 1)/l and
+//    w = e^{lL}/l, every transformed value is taken as
+//        y - K = w * expm1(l (ln x - L)),
+//    which keeps full relative precision for every lambda (the reference's
+//    (pow(x,l) - 1)/l, estimator.h:56,72-74, cancels for small l, and for
+//    l = 3 on dark pixels FP32 y = -1/3 + tiny loses the variation entirely).
+//    Central moments, c2X, c2Y and m1 do not depend on the shift; the stored
+//    mean is K + mean(y - K).  One log per sample-channel is shared by all
+//    sets.  The update is the reference recurrence in factored form,
+//        m2 += d^2 (1 - 1/n)
+//        m3 += -3 (d/n) m2 + d^3 (1 - 1/n^2)
+//        m4 += -4 (d/n) m3 - 6 (d^2/n^2) m2 + d^4 (1 - 1/n^3)
+//        m1 += |d| sqrt(1 - 1/n),
+//    with the per-sample factors shared by all sets; it is exact algebra of
+//    estimator.h:288-304 and has no cancellation at n = 1.  The Box-Cox sets
+//    are checked against an FP64 evaluation of the same recurrence.
+
+#include "internal.hpp"
+
+namespace stater {
+
+struct AccArgs {
+    const float *data;
+    int64_t pitch, comp, sstride;
+    int spp;
+    int sanitize;
+    const int32_t *counts;
+    int64_t counts_pitch;
+    int32_t *n;
+    float *stats;
+    float *lref;  // [3][plane] ln of the reference sample of each pixel-channel
+    int ld;
+    size_t plane;
+    int W, H;
+    float lam[kMaxSets];
+    float inv_lam[kMaxSets];
+};
+
+// Reference update, estimator.h:226-305, evaluated exactly as written.
+template <int MM>
+__device__ __forceinline__ void meng_exact(float *st, float x, float nf, float xD, float yD) {
+#pragma clang fp contract(off)
+    float &mean = st[0];
+    if (MM == 1) {
+        const float d = x - mean;
+        const float dN = d / nf;
+        mean += dN;
+        return;
+    }
+    if (MM == 2) {
+        const float d = x - mean;
+        const float dN = d / nf;
+        mean += dN;
+        st[1] += d * (d - dN);
+        return;
+    }
+    float &m2 = st[1];
+    float &m3 = st[2];
+    const float d = x - mean;
+    const float d2 = d * d;
+    const float dN = d / nf;
+    const float dN2 = dN * dN;
+    mean += dN;
+    m2 += d * (d - dN);
+    m3 += -3.f * dN * m2 + d * (d2 - dN2);
+    if (MM == 3) return;
+    float &m4 = st[3];
+    m4 += -4.f * dN * m3 - 6.f * dN2 * m2 + d * (d * d2 - dN * dN2);
+    if (MM == 4) return;
+    st[4] += (x - mean) * xD;
+    st[5] += (x - mean) * yD;
+    st[6] += sqrtf(d * (d - dN));  // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
+}
+
+// Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
+struct NFac {
+    float b, a1, a2, a3, b4, b6, sa1;
+};
+
+__device__ __forceinline__ NFac nfac(float nf) {
+    NFac f;
+    f.b = __builtin_amdgcn_rcpf(nf);
+    const float b2 = f.b * f.b;
+    f.a1 = 1.f - f.b;
+    f.a2 = 1.f - b2;
+    f.a3 = 1.f - b2 * f.b;
+    f.b4 = -4.f * f.b;
+    f.b6 = -6.f * b2;
+    f.sa1 = __builtin_amdgcn_sqrtf(f.a1);
+    return f;
+}
+
+// The reference recurrence in factored form on a shifted value t = y - K
+// (st[0] holds the shifted mean).
+template <int MM>
+__device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f, float xD, float yD) {
+    float &mean = st[0];
+    const float d = t - mean;
+    mean = __builtin_fmaf(d, f.b, mean);
+    if (MM == 1) return;
+    const float d2 = d * d;
+    float &m2 = st[1];
+    m2 = __builtin_fmaf(d2, f.a1, m2);
+    if (MM == 2) return;
+    const float db = d * f.b;
+    float &m3 = st[2];
+    m3 = __builtin_fmaf(-3.f * db, m2, __builtin_fmaf(d * d2, f.a2, m3));
+    if (MM == 3) return;
+    float &m4 = st[3];
+    m4 = __builtin_fmaf(d * f.b4, m3, __builtin_fmaf(d2 * f.b6, m2, __builtin_fmaf(d2 * d2, f.a3, m4)));
+    if (MM == 4) return;
+    const float r = t - mean;
+    st[4] = __builtin_fmaf(r, xD, st[4]);
+    st[5] = __builtin_fmaf(r, yD, st[5]);
+    st[6] = __builtin_fmaf(fabsf(d), f.sa1, st[6]);
+}
+
+constexpr int nstats(int MM) { return MM >= 5 ? 7 : MM; }
+
+// statpath.cpp:313-332: NaN channel, luminance < -1e-5 or infinite -> black.
+__device__ __forceinline__ bool keep_sample(float r, float g, float b) {
+#pragma clang fp contract(off)
+    const bool nan = __builtin_isnan(r) || __builtin_isnan(g) || __builtin_isnan(b);
+    const float y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
+    return !(nan || (double)y < -1e-5 || __builtin_isinf(y));
+}
+
+__device__ __forceinline__ float load_sample(const char *base, int64_t off, int64_t comp, int c, bool sanitize) {
+    float v = __builtin_nontemporal_load((const float *)(base + off + c * comp));
+    if (sanitize) {
+        const float r = __builtin_nontemporal_load((const float *)(base + off));
+        const float g = __builtin_nontemporal_load((const float *)(base + off + comp));
+        const float b = __builtin_nontemporal_load((const float *)(base + off + 2 * comp));
+        if (!keep_sample(r, g, b)) v = 0.f;
+    }
+    return v;
+}
+
+template <int T, int MM>
+__global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
+    constexpr int NS = nstats(MM);
+    const int c = threadIdx.x >> 6;  // wavefront-uniform channel
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y;
+    if (x >= a.W) return;
+    const size_t pix = (size_t)y * a.ld + x;
+    int cnt = a.spp;
+    if (a.counts) cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4));
+    if (cnt <= 0) return;
+    uint32_t n = (uint32_t)a.n[pix];
+    const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
+
+    float st[T + 1][NS];
+#pragma unroll
+    for (int s = 0; s <= T; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
+
+    // Box-Cox reference point: the first sample the pixel-channel ever saw.
+    float L = 0.f, K[T + 1], Wt[T + 1];
+    if (T > 0) {
+        const size_t lp = (size_t)c * a.plane + pix;
+        if (n == 0) {
+            const float v0 = load_sample(base, 0, a.comp, c, a.sanitize);
+            L = (v0 > 0.f && __builtin_isfinite(v0)) ? logf(v0) : 0.f;
+            a.lref[lp] = L;
+        } else {
+            L = a.lref[lp];
+        }
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            K[s] = expm1f(a.lam[s] * L) * a.inv_lam[s];
+            Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
+            st[s][0] = n ? st[s][0] - K[s] : 0.f;
+        }
+    }
+
+    for (int j = 0; j < cnt; ++j) {
+        const int64_t off = (int64_t)j * a.sstride;
+        const float v = load_sample(base, off, a.comp, c, a.sanitize);
+        float xD = 0.f, yD = 0.f;
+        if (MM >= 5) {
+            xD = __builtin_nontemporal_load((const float *)(base + off + 3 * a.comp)) - 0.5f;
+            yD = __builtin_nontemporal_load((const float *)(base + off + 4 * a.comp)) - 0.5f;
+        }
+        ++n;
+        const float nf = (float)n;
+        meng_exact<MM>(st[0], v, nf, xD, yD);
+        if (T > 0) {
+            const NFac f = nfac(nf);
+            const float dl = logf(v) - L;
+#pragma unroll
+            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], Wt[s] * expm1f(a.lam[s] * dl), f, xD, yD);
+        }
+    }
+#pragma unroll
+    for (int s = 1; s <= T; ++s) st[s][0] += K[s];
+#pragma unroll
+    for (int s = 0; s <= T; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
+    if (c == 0) a.n[pix] = (int32_t)n;
+}
+
+__global__ void k_zero(uint32_t *p, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+int launch_zero(void *p, size_t bytes, hipStream_t st) {
+    const size_t words = bytes / 4;
+    const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, st, (uint32_t *)p, words);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+template <int T, int MM>
+static void launch_tm(const AccArgs &a, dim3 grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_accumulate<T, MM>), grid, dim3(192), 0, st, a);
+}
+
+template <int T>
+static void launch_t(int mm, const AccArgs &a, dim3 grid, hipStream_t st) {
+    switch (mm) {
+        case 1: launch_tm<T, 1>(a, grid, st); break;
+        case 2: launch_tm<T, 2>(a, grid, st); break;
+        case 3: launch_tm<T, 3>(a, grid, st); break;
+        case 4: launch_tm<T, 4>(a, grid, st); break;
+        default: launch_tm<T, 5>(a, grid, st); break;
+    }
+}
+
+int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st) {
+    if (mode != STATER_ACC_SEQUENTIAL) return fail(STATER_E_UNSUPPORTED, "stater_accumulate: MERGE64 not built yet");
+    if (s->pitch_bytes % 4 || s->comp_stride_bytes % 4 || s->sample_stride_bytes % 4 ||
+        s->pitch_bytes < (int64_t)bs->W * 4)
+        return fail(STATER_E_ARG, "stater_accumulate: strides must be float multiples covering a row");
+    AccArgs a{};
+    a.data = s->data;
+    a.pitch = s->pitch_bytes;
+    a.comp = s->comp_stride_bytes;
+    a.sstride = s->sample_stride_bytes;
+    a.spp = s->spp;
+    a.sanitize = s->sanitize;
+    a.counts = s->counts;
+    a.counts_pitch = s->counts_pitch_bytes;
+    a.n = bs->n;
+    a.stats = bs->stats;
+    a.lref = bs->lref;
+    a.ld = bs->ld;
+    a.plane = bs->plane;
+    a.W = bs->W;
+    a.H = bs->H;
+    for (int k = 0; k < kMaxSets; ++k) {
+        a.lam[k] = k < bs->S ? bs->set_lambda[k] : 1.f;
+        a.inv_lam[k] = 1.f / a.lam[k];
+    }
+    dim3 grid((bs->W + 63) / 64, bs->H);
+    LaunchTimer t(bs->ctx, st, "accumulate");
+    switch (bs->S - 1) {
+        case 0: launch_t<0>(bs->mm, a, grid, st); break;
+        case 1: launch_t<1>(bs->mm, a, grid, st); break;
+        case 2: launch_t<2>(bs->mm, a, grid, st); break;
+        case 3: launch_t<3>(bs->mm, a, grid, st); break;
+        case 4: launch_t<4>(bs->mm, a, grid, st); break;
+        default: launch_t<5>(bs->mm, a, grid, st); break;
+    }
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+}  // namespace stater

@@ -1,0 +1,534 @@
+// api.cpp — the C-ABI: contexts, buffer sets, named views, dispatch, timing.
+//
+// The buffer set replaces the reference's Estimator buffer family
+// (src/statistics/estimator.cpp:83-245): instead of one host cv::Mat plus one
+// device GpuMat per "t<type>-b<slot>-[film-]<name>" buffer, uploaded every
+// iteration (estimator.cpp:346-353), every statistic lives only in device
+// memory as FP32 planes with a 256-byte row pitch, and the untransformed
+// statistics that the reference duplicates in every lambda slot are kept once.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace stater {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return STATER_E_HIP;
+}
+
+static hipEvent_t take_event(stater_ctx *ctx) {
+    if (!ctx->free_events.empty()) {
+        hipEvent_t e = ctx->free_events.back();
+        ctx->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+LaunchTimer::LaunchTimer(stater_ctx *c, hipStream_t s, const char *n) : ctx(c), st(s), name(n) {
+    if (!(ctx->flags & STATER_CTX_TIMING)) return;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    a = take_event(ctx);
+    b = take_event(ctx);
+    if (a && b) (void)hipEventRecord(a, st);
+}
+
+LaunchTimer::~LaunchTimer() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, st);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->pending.push_back({name, a, b});
+}
+
+hipStream_t pick_stream(const stater_bufset *bs, void *stream) {
+    return stream ? (hipStream_t)stream : bs->ctx->stream;
+}
+
+// ------------------------------------------------ critical-value polynomial
+static double normal_t2_crit(double alpha) {
+    double lo = 0, hi = 200;
+    for (int i = 0; i < 200; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (std::erfc(std::sqrt(0.5 * mid)) > alpha) lo = mid; else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+double t2_crit_host(double nu, double alpha) {
+    if (!(nu < 1e300)) return normal_t2_crit(alpha);
+    double lo = 0.0, hi = 1.0;  // x = nu/(nu+t2); I_x(nu/2,1/2) rises with x
+    for (int i = 0; i < 200; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (betainc_reg(0.5 * nu, 0.5, mid) < alpha) lo = mid; else hi = mid;
+    }
+    const double x = 0.5 * (lo + hi);
+    return nu * (1.0 - x) / x;
+}
+
+TCritPoly make_tcrit_poly(double alpha) {
+    TCritPoly tp{};
+    const int D = kTCritDeg;
+    const double umax = 0.125;
+    // Chebyshev interpolation of g(u) = t2crit(1/u) on s = 2u/umax - 1.
+    double gnode[D + 1], cheb[D + 1];
+    for (int k = 0; k <= D; ++k) {
+        const double s = std::cos(kPi * (k + 0.5) / (D + 1));
+        const double u = 0.5 * (s + 1.0) * umax;
+        gnode[k] = u == 0.0 ? normal_t2_crit(alpha) : t2_crit_host(1.0 / u, alpha);
+    }
+    for (int j = 0; j <= D; ++j) {
+        double acc = 0;
+        for (int k = 0; k <= D; ++k) acc += gnode[k] * std::cos(kPi * j * (k + 0.5) / (D + 1));
+        cheb[j] = acc * (j == 0 ? 1.0 : 2.0) / (D + 1);
+    }
+    // Chebyshev -> monomial coefficients in s (T_{j+1} = 2 s T_j - T_{j-1}).
+    double Tm[D + 1][D + 1] = {};
+    Tm[0][0] = 1.0;
+    Tm[1][1] = 1.0;
+    for (int j = 2; j <= D; ++j)
+        for (int i = 0; i <= D; ++i)
+            Tm[j][i] = (i ? 2.0 * Tm[j - 1][i - 1] : 0.0) - Tm[j - 2][i];
+    for (int i = 0; i <= D; ++i) {
+        double a = 0;
+        for (int j = 0; j <= D; ++j) a += cheb[j] * Tm[j][i];
+        tp.c[i] = (float)a;
+    }
+    tp.scale = (float)(2.0 / umax);
+    tp.shift = -1.0f;
+    tp.umax = (float)umax;
+    tp.alpha = alpha;
+    // Measure the FP32 evaluation error against the FP64 tail on a grid and
+    // size the exact-tail band well above it.
+    double maxerr = 0;
+    for (int i = 0; i <= 400; ++i) {
+        const double u = umax * i / 400.0;
+        const float s = (float)u * tp.scale + tp.shift;
+        float acc = tp.c[D];
+        for (int k = D - 1; k >= 0; --k) acc = acc * s + tp.c[k];
+        const double ref = u == 0.0 ? normal_t2_crit(alpha) : t2_crit_host(1.0 / u, alpha);
+        maxerr = std::max(maxerr, std::fabs((double)acc / ref - 1.0));
+    }
+    tp.band = (float)std::max(2e-6, 8.0 * maxerr + 2e-6);
+    return tp;
+}
+
+const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tcrit.find(alpha);
+    if (it == ctx->tcrit.end()) it = ctx->tcrit.emplace(alpha, make_tcrit_poly(alpha)).first;
+    return it->second;
+}
+
+}  // namespace stater
+
+using namespace stater;
+
+// ================================================================ context
+extern "C" {
+
+const char *stater_last_error(void) { return g_err.c_str(); }
+int stater_abi_version(void) { return STATER_ABI_VERSION; }
+
+void stater_print_macro_configuration(void) {
+    std::printf("==== stater (gfx950) configuration ====\n");
+    std::printf("max sets = %d (untransformed + Box-Cox), max radius = %d\n", kMaxSets, STATER_MAX_RADIUS);
+    std::printf("FILM_MULTI_TRANSFORM_MODE = runtime (mt_mode 2 | 3)\n");
+    std::printf("CASCADE_MODE = runtime (cascade), USE_LEVC = runtime (use_levc)\n");
+    std::printf("accumulation: untransformed set bit-exact FP32 (Meng), Box-Cox sets expm1 form\n");
+}
+
+int stater_create(int device, unsigned flags, stater_ctx **out) {
+    if (!out) return fail(STATER_E_ARG, "stater_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    STATER_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(STATER_E_ARG, "stater_create: no such device");
+    STATER_HIP(hipSetDevice(device));
+    auto *ctx = new stater_ctx();
+    ctx->device = device;
+    ctx->flags = flags;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) { delete ctx; return hip_fail(e, "hipGetDeviceProperties"); }
+    ctx->num_cu = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete ctx; return hip_fail(e, "hipStreamCreate"); }
+    *out = ctx;
+    return STATER_OK;
+}
+
+int stater_destroy(stater_ctx *ctx) {
+    if (!ctx) return STATER_OK;
+    if (ctx->live_bufsets.load() > 0)
+        return fail(STATER_E_STATE, "stater_destroy: destroy the context's buffer sets first");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return STATER_OK;
+}
+
+int stater_ctx_stream(stater_ctx *ctx, void **stream_out) {
+    if (!ctx || !stream_out) return fail(STATER_E_ARG, "stater_ctx_stream: NULL argument");
+    *stream_out = (void *)ctx->stream;
+    return STATER_OK;
+}
+
+static int flush_timing(stater_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto &p : ctx->pending) {
+        STATER_HIP(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        STATER_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto &acc = ctx->stats[p.name];
+        acc.launches++;
+        acc.ms += ms;
+        ctx->free_events.push_back(p.a);
+        ctx->free_events.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return STATER_OK;
+}
+
+int stater_timing_reset(stater_ctx *ctx) {
+    if (!ctx) return fail(STATER_E_ARG, "stater_timing_reset: NULL context");
+    int rc = flush_timing(ctx);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->stats.clear();
+    return rc;
+}
+
+int stater_timing_count(stater_ctx *ctx) {
+    if (!ctx) return fail(STATER_E_ARG, "stater_timing_count: NULL context");
+    int rc = flush_timing(ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return (int)ctx->stats.size();
+}
+
+int stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len, uint64_t *launches,
+                      double *total_ms) {
+    if (!ctx) return fail(STATER_E_ARG, "stater_timing_get: NULL context");
+    int rc = flush_timing(ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (index < 0 || index >= (int)ctx->stats.size()) return fail(STATER_E_ARG, "stater_timing_get: index");
+    auto it = ctx->stats.begin();
+    std::advance(it, index);
+    if (name && name_len > 0) {
+        std::strncpy(name, it->first.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (launches) *launches = it->second.launches;
+    if (total_ms) *total_ms = it->second.ms;
+    return STATER_OK;
+}
+
+// ============================================================ buffer sets
+struct Layout {
+    int S = 1, nst = 7, ld = 0;
+    size_t plane = 0, n_planes = 0;
+    float set_lambda[kMaxSets] = {1.f};
+    int slot_set[STATER_MAX_LAMBDAS] = {};
+};
+
+static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm, unsigned flags, Layout &L) {
+    if (W <= 0 || H <= 0 || W > 16384 || H > 16384) return fail(STATER_E_ARG, "bufset: width/height out of range");
+    if (n_lambdas < 0 || n_lambdas > STATER_MAX_LAMBDAS || (n_lambdas && !lambdas))
+        return fail(STATER_E_ARG, "bufset: bad lambda list");
+    if (mm < 1 || mm > 5) return fail(STATER_E_ARG, "bufset: max_moment must be 1..5");
+    L.S = 1;
+    L.set_lambda[0] = 1.f;
+    for (int j = 0; j < n_lambdas; ++j) {
+        const float l = lambdas[j];
+        if (!(l > 0.f) || !std::isfinite(l)) return fail(STATER_E_ARG, "bufset: lambdas must be positive");
+        int s = -1;
+        for (int k = 0; k < L.S; ++k)
+            if (L.set_lambda[k] == l) s = k;
+        if (s < 0) {
+            if (L.S >= kMaxSets) return fail(STATER_E_UNSUPPORTED, "bufset: at most 5 distinct lambdas != 1");
+            s = L.S++;
+            L.set_lambda[s] = l;
+        }
+        L.slot_set[j] = s;
+    }
+    L.nst = mm >= 5 ? 7 : mm;
+    L.ld = (W + kRowAlign - 1) / kRowAlign * kRowAlign;
+    L.plane = (size_t)L.ld * H;
+    size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3;  // n + moments (all 7 slots) + Box-Cox reference point
+    if (!(flags & STATER_BUFSET_NO_DERIVED)) np += (size_t)L.S * (1 + 6 * 3) + 9 + 1;
+    L.n_planes = np;
+    return STATER_OK;
+}
+
+int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas, const float *lambdas, int max_moment,
+                                unsigned flags) {
+    Layout L;
+    if (make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L)) return -1;
+    return (int64_t)(L.n_planes * L.plane * 4);
+}
+
+int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, const float *lambdas,
+                         int max_moment, unsigned flags, stater_bufset **out) {
+    if (!ctx || !out) return fail(STATER_E_ARG, "stater_bufset_create: NULL argument");
+    *out = nullptr;
+    Layout L;
+    int rc = make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L);
+    if (rc) return rc;
+    STATER_HIP(hipSetDevice(ctx->device));
+    auto *bs = new stater_bufset();
+    bs->ctx = ctx;
+    bs->W = width; bs->H = height; bs->ld = L.ld; bs->plane = L.plane; bs->S = L.S;
+    bs->n_lambdas = n_lambdas;
+    for (int j = 0; j < n_lambdas; ++j) { bs->lambdas[j] = lambdas[j]; bs->slot_set[j] = L.slot_set[j]; }
+    for (int s = 0; s < L.S; ++s) bs->set_lambda[s] = L.set_lambda[s];
+    bs->mm = max_moment;
+    bs->flags = flags;
+    bs->bytes = L.n_planes * L.plane * 4;
+    hipError_t e = hipMalloc((void **)&bs->base, bs->bytes);
+    if (e != hipSuccess) {
+        delete bs;
+        (void)hipGetLastError();
+        return fail(STATER_E_NOMEM, "stater_bufset_create: device allocation failed");
+    }
+    e = hipMemset(bs->base, 0, bs->bytes);
+    if (e != hipSuccess) { (void)hipFree(bs->base); delete bs; return hip_fail(e, "hipMemset"); }
+    float *p = (float *)bs->base;
+    bs->n = (int32_t *)p; p += L.plane;
+    bs->stats = p; p += (size_t)L.S * N_STATS * 3 * L.plane;
+    bs->lref = p; p += 3 * L.plane;
+    if (!(flags & STATER_BUFSET_NO_DERIVED)) {
+        bs->jb = p; p += (size_t)L.S * L.plane;
+        float **d[6] = {&bs->lnS2, &bs->se2, &bs->m2C, &bs->varD, &bs->varCD, &bs->discr};
+        for (auto q : d) { *q = p; p += (size_t)L.S * 3 * L.plane; }
+        bs->meanVar = p; p += 3 * L.plane;
+        bs->meanD = p; p += 3 * L.plane;
+        bs->filmD = p; p += 3 * L.plane;
+        bs->targetspp = (int32_t *)p; p += L.plane;
+    }
+    ctx->live_bufsets.fetch_add(1);
+    *out = bs;
+    return STATER_OK;
+}
+
+int stater_bufset_destroy(stater_bufset *bs) {
+    if (!bs) return STATER_OK;
+    (void)hipSetDevice(bs->ctx->device);
+    (void)hipStreamSynchronize(bs->ctx->stream);
+    if (bs->as_scratch) (void)hipFree(bs->as_scratch);
+    (void)hipFree(bs->base);
+    bs->ctx->live_bufsets.fetch_sub(1);
+    delete bs;
+    return STATER_OK;
+}
+
+int64_t stater_bufset_bytes(const stater_bufset *bs) { return bs ? (int64_t)bs->bytes : -1; }
+
+int stater_bufset_reset(stater_bufset *bs, void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_bufset_reset: NULL set");
+    const size_t bytes = (1 + (size_t)bs->S * N_STATS * 3) * bs->plane * 4;
+    return launch_zero(bs->base, bytes, pick_stream(bs, stream));
+}
+
+static void fill_plane(const stater_bufset *bs, void *ptr, int channels, stater_dtype dt, stater_plane *out) {
+    out->ptr = ptr;
+    out->pitch_bytes = (int64_t)bs->ld * 4;
+    out->channel_stride_bytes = (int64_t)bs->plane * 4;
+    out->width = bs->W;
+    out->height = bs->H;
+    out->channels = channels;
+    out->dtype = dt;
+    out->layout = STATER_PLANAR;
+    out->reserved = 0;
+}
+
+int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out) {
+    if (!bs || !name || !out) return fail(STATER_E_ARG, "stater_bufset_view: NULL argument");
+    const std::string nm(name);
+    const bool derived = !(bs->flags & STATER_BUFSET_NO_DERIVED);
+    auto need_derived = [&]() { return fail(STATER_E_NOTFOUND, "buffer '" + nm + "' not allocated (NO_DERIVED)"); };
+    if (nm == "n") { fill_plane(bs, bs->n, 1, STATER_I32, out); return STATER_OK; }
+    if (nm == "film") { fill_plane(bs, bs->stat(0, ST_MEAN, 0), 3, STATER_F32, out); return STATER_OK; }
+    if (nm == "filmD") { if (!derived) return need_derived(); fill_plane(bs, bs->filmD, 3, STATER_F32, out); return STATER_OK; }
+    if (nm == "targetspp") { if (!derived) return need_derived(); fill_plane(bs, bs->targetspp, 1, STATER_I32, out); return STATER_OK; }
+    // t<type>-b<slot>-[film-]<stat>; the set holds type 0 (Radiance).
+    int slot = -1, consumed = 0;
+    if (std::sscanf(name, "t0-b%d-%n", &slot, &consumed) != 1 || consumed == 0)
+        return fail(STATER_E_NOTFOUND, "unknown buffer '" + nm + "'");
+    const int nslots = std::max(1, bs->n_lambdas);
+    if (slot < 0 || slot >= nslots) return fail(STATER_E_NOTFOUND, "no slot in '" + nm + "'");
+    std::string rest = nm.substr(consumed);
+    bool film = false;
+    if (rest.rfind("film-", 0) == 0) { film = true; rest = rest.substr(5); }
+    const int s = film ? 0 : (bs->n_lambdas ? bs->slot_set[slot] : 0);
+    static const char *stat_names[N_STATS] = {"mean", "m2", "m3", "m4", "c2X", "c2Y", "m1"};
+    for (int k = 0; k < N_STATS; ++k)
+        if (rest == stat_names[k]) { fill_plane(bs, bs->stat(s, k, 0), 3, STATER_F32, out); return STATER_OK; }
+    if (film && rest == "n") { fill_plane(bs, bs->n, 1, STATER_I32, out); return STATER_OK; }
+    if (!derived) return need_derived();
+    if (rest == "jb") { fill_plane(bs, bs->jb + (size_t)s * bs->plane, 1, STATER_F32, out); return STATER_OK; }
+    struct { const char *n; float *b; } der[] = {{"lnS2", bs->lnS2}, {"se2", bs->se2}, {"m2C", bs->m2C},
+                                                 {"varD", bs->varD}, {"varCD", bs->varCD}, {"discr", bs->discr}};
+    for (auto &d : der)
+        if (rest == d.n) { fill_plane(bs, bs->per_set(d.b, s, 0), 3, STATER_F32, out); return STATER_OK; }
+    if (film && rest == "meanVar") { fill_plane(bs, bs->meanVar, 3, STATER_F32, out); return STATER_OK; }
+    if (film && rest == "meanD") { fill_plane(bs, bs->meanD, 3, STATER_F32, out); return STATER_OK; }
+    return fail(STATER_E_NOTFOUND, "unknown buffer '" + nm + "'");
+}
+
+// ============================================================== dispatch
+int stater_accumulate(stater_bufset *bs, const stater_samples *samples, int mode, void *stream) {
+    if (!bs || !samples || !samples->data) return fail(STATER_E_ARG, "stater_accumulate: NULL argument");
+    if (samples->spp < 0) return fail(STATER_E_ARG, "stater_accumulate: spp < 0");
+    if (mode != STATER_ACC_SEQUENTIAL && mode != STATER_ACC_MERGE64)
+        return fail(STATER_E_ARG, "stater_accumulate: unknown mode");
+    if (samples->spp == 0) return STATER_OK;
+    return launch_accumulate(bs, samples, mode, pick_stream(bs, stream));
+}
+
+void stater_default_params(stater_denoise_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->radius = 10;
+    p->ds_factor = -0.5f / (10.f * 10.f);
+    p->var_z = 2.80703f;
+    p->alpha = 0.005f;
+    p->mt_mode = 2;
+    p->cascade = 1;
+    p->use_levc = 1;
+}
+
+static int check_params(const stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g) {
+    if (!p) return fail(STATER_E_ARG, "denoise: NULL params");
+    if (p->radius < 0 || p->radius > STATER_MAX_RADIUS) return fail(STATER_E_ARG, "denoise: radius out of range");
+    if (!(p->alpha > 0.f && p->alpha < 1.f)) return fail(STATER_E_ARG, "denoise: alpha must be in (0,1)");
+    if (!(p->var_z > 0.f)) return fail(STATER_E_ARG, "denoise: var_z must be positive");
+    if (p->mt_mode != 2 && p->mt_mode != 3) return fail(STATER_E_ARG, "denoise: mt_mode must be 2 or 3");
+    if (!(p->ds_factor <= 0.f)) return fail(STATER_E_ARG, "denoise: ds_factor must be <= 0");
+    if (bs->flags & STATER_BUFSET_NO_DERIVED) return fail(STATER_E_STATE, "denoise: set was created NO_DERIVED");
+    if (g) {
+        if (g->count < 0 || g->count > STATER_MAX_GBUFFERS) return fail(STATER_E_ARG, "denoise: gbuffer count");
+        int total = 0;
+        for (int k = 0; k < g->count; ++k) {
+            const stater_plane &pl = g->planes[k];
+            if (!pl.ptr || pl.width != bs->W || pl.height != bs->H || pl.channels < 1 || pl.channels > 3 ||
+                pl.dtype != STATER_F32 || pl.layout != STATER_PLANAR || pl.pitch_bytes % 4 ||
+                pl.channel_stride_bytes % 4)
+                return fail(STATER_E_ARG, "denoise: gbuffer plane must be planar f32 of the film's size");
+            if (!(g->dr_factors[k] <= 0.f)) return fail(STATER_E_ARG, "denoise: gbuffer factor must be <= 0");
+            total += pl.channels;
+        }
+        if (total > 7) return fail(STATER_E_UNSUPPORTED, "denoise: at most 7 feature channels");
+    }
+    return STATER_OK;
+}
+
+int stater_denoise_vars(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *gbufs,
+                        void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_denoise_vars: NULL set");
+    int rc = check_params(bs, p, gbufs);
+    if (rc) return rc;
+    if (bs->mm < 4) return fail(STATER_E_STATE, "stater_denoise_vars: needs max_moment >= 4");
+    hipStream_t st = pick_stream(bs, stream);
+    rc = launch_prepass(bs, p->var_z, st);
+    if (rc) return rc;
+    return launch_var_filter(bs, p, gbufs, st);
+}
+
+static int check_film_plane(const stater_bufset *bs, const stater_plane *f, const char *what) {
+    if (!f) return STATER_OK;
+    if (!f->ptr || f->width != bs->W || f->height != bs->H || f->channels != 3 || f->dtype != STATER_F32 ||
+        f->pitch_bytes % 4 || f->channel_stride_bytes % 4)
+        return fail(STATER_E_ARG, std::string("stater_denoise_film: bad ") + what + " plane");
+    return STATER_OK;
+}
+
+int stater_denoise_film(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *gbufs,
+                        const stater_plane *film, const stater_plane *film_out, void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_denoise_film: NULL set");
+    int rc = check_params(bs, p, gbufs);
+    if (rc) return rc;
+    if (bs->mm < 2) return fail(STATER_E_STATE, "stater_denoise_film: needs max_moment >= 2");
+    if ((rc = check_film_plane(bs, film, "film")) || (rc = check_film_plane(bs, film_out, "film_out"))) return rc;
+    return launch_welch_filter(bs, p, gbufs, film, film_out, false, pick_stream(bs, stream));
+}
+
+int stater_denoise_plain(stater_bufset *bs, int radius, float alpha, void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_denoise_plain: NULL set");
+    stater_denoise_params p;
+    stater_default_params(&p);
+    p.radius = radius;
+    p.alpha = alpha;
+    p.ds_factor = 0.f;
+    p.cascade = 0;
+    p.use_levc = 0;
+    int rc = check_params(bs, &p, nullptr);
+    if (rc) return rc;
+    if (bs->mm < 2) return fail(STATER_E_STATE, "stater_denoise_plain: needs max_moment >= 2");
+    return launch_welch_filter(bs, &p, nullptr, nullptr, nullptr, true, pick_stream(bs, stream));
+}
+
+int stater_calc_target_spp(stater_bufset *bs, uint64_t budget_spp, uint32_t cap, const stater_plane *out,
+                           void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_calc_target_spp: NULL set");
+    if (bs->flags & STATER_BUFSET_NO_DERIVED) return fail(STATER_E_STATE, "stater_calc_target_spp: NO_DERIVED set");
+    if (out && (!out->ptr || out->width != bs->W || out->height != bs->H || out->dtype != STATER_I32 ||
+                out->pitch_bytes % 4))
+        return fail(STATER_E_ARG, "stater_calc_target_spp: bad output plane");
+    if (budget_spp > (uint64_t)1 << 32) return fail(STATER_E_ARG, "stater_calc_target_spp: budget too large");
+    return launch_target_spp(bs, budget_spp, cap, out, pick_stream(bs, stream));
+}
+
+int stater_calc_mean_vars(stater_bufset *bs, void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_calc_mean_vars: NULL set");
+    if (bs->flags & STATER_BUFSET_NO_DERIVED) return fail(STATER_E_STATE, "stater_calc_mean_vars: NO_DERIVED set");
+    if (bs->mm < 2) return fail(STATER_E_STATE, "stater_calc_mean_vars: needs max_moment >= 2");
+    return launch_mean_vars(bs, pick_stream(bs, stream));
+}
+
+int stater_synchronize(void *stream) {
+    STATER_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return STATER_OK;
+}
+
+// ============================================================== synthesis
+int stater_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo,
+                          const stater_plane *normal, const stater_plane *depth, void *stream) {
+    if (!ctx || !s || !albedo || !normal || !depth) return fail(STATER_E_ARG, "stater_synth_features: NULL argument");
+    const stater_plane *pl[3] = {albedo, normal, depth};
+    const int ch[3] = {3, 3, 1};
+    for (int k = 0; k < 3; ++k)
+        if (!pl[k]->ptr || pl[k]->width != s->width || pl[k]->height != s->height || pl[k]->channels != ch[k] ||
+            pl[k]->dtype != STATER_F32 || pl[k]->layout != STATER_PLANAR)
+            return fail(STATER_E_ARG, "stater_synth_features: planes must be planar f32 of the scene's size");
+    return launch_synth_features(ctx, s, albedo, normal, depth, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stater_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo, uint32_t first,
+                         const stater_samples *out, void *stream) {
+    if (!ctx || !s || !out || !out->data) return fail(STATER_E_ARG, "stater_synth_samples: NULL argument");
+    if (s->albedo_modulation && (!albedo || !albedo->ptr))
+        return fail(STATER_E_ARG, "stater_synth_samples: scene modulates by albedo; pass the albedo plane");
+    if (out->spp <= 0) return fail(STATER_E_ARG, "stater_synth_samples: spp must be positive");
+    return launch_synth_samples(ctx, s, albedo, first, out, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stater_scene_init(stater_scene *s, int model, uint32_t seed, uint32_t image_id, int width, int height);
+
+}  // extern "C"

@@ -1,0 +1,919 @@
+// denoise.hip — the denoiser of the statistics path on gfx950.
+//
+// Replaces cv::cuda::stat_denoiser::{denoiseVars, denoiseFilm, calcTargetSPP}
+// called at src/statistics/estimator.cpp:374-414, 421-457 and 502-511.  The
+// reference's kernel source (stat_denoiser.cu, an external submodule) is not
+// part of the reference checkout, so the arithmetic follows the published
+// methods the call sites and buffer names point to:
+//   prepass   Jarque-Bera (1980) per channel, mean over RGB -> jb;
+//             Bonett (2006) log-variance: c = n/(n-z), lnS2 = ln(c s^2),
+//             se2 = c^2 (g4 - (n-3)/n)/(n-1), g4 = n m4/m2^2;
+//             LEV: m2C = max(0, m2 - 12 (c2X^2 + c2Y^2)/n), the variance not
+//             explained by a linear fit on the in-pixel position (var 1/12).
+//   varD      weighted mean of s^2_q over neighbours q whose Bonett intervals
+//             agree, (lnS2_p - lnS2_q)^2 <= z^2 (se2_p + se2_q); varCD the same
+//             mean of m2C_q/(n_q-1).
+//   filmD     weighted mean of the film over neighbours whose means pass a
+//             two-sided Welch t-test at level alpha in every lambda set
+//             (mode 2) or in the set with the smallest jb (mode 3).
+//   weights   exp(ds_factor |dq|^2 + sum_k dr_k |g_p - g_q|^2) (estimator.h:353,
+//             estimator.cpp:10-18); uniform for the plain method.
+// A test whose operands are not finite abstains (accepts).
+//
+// Both filters stream a ring of TY halo rows through LDS: a 64x4 output tile
+// walks the 2r+1 neighbour rows, and after each row the oldest halo row is
+// replaced, so every neighbour pair sees all its operands in LDS while the
+// per-workgroup footprint stays a few rows wide.  Weights are evaluated once
+// per pair for all sets and channels.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace stater {
+
+constexpr int TX = 64, TY = 4;
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr int kMaxFeat = 7;
+
+struct FeatArgs {
+    const float *ptr[kMaxFeat];
+    int64_t pitch[kMaxFeat];
+    float scale[kMaxFeat];  // sqrt(-dr log2 e): w = exp2(sp - sum (g'_p - g'_q)^2)
+    int nf;
+};
+
+static FeatArgs make_feats(const stater_gbuffers *g) {
+    FeatArgs f{};
+    if (!g) return f;
+    for (int k = 0; k < g->count; ++k)
+        for (int c = 0; c < g->planes[k].channels; ++c) {
+            f.ptr[f.nf] = (const float *)((const char *)g->planes[k].ptr + c * g->planes[k].channel_stride_bytes);
+            f.pitch[f.nf] = g->planes[k].pitch_bytes;
+            f.scale[f.nf] = std::sqrt(-g->dr_factors[k] * kLog2e);
+            f.nf++;
+        }
+    return f;
+}
+
+__device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y) {
+    return *(const float *)((const char *)f.ptr[k] + (int64_t)y * f.pitch[k] + (int64_t)x * 4) * f.scale[k];
+}
+
+// ================================================================ prepass
+struct PreArgs {
+    const int32_t *n;
+    const float *stats;
+    float *jb, *lnS2, *se2, *m2C;
+    int S, W, H, ld, lev;
+    size_t plane;
+    float z;
+};
+
+__global__ void __launch_bounds__(256) k_prepass(PreArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= a.W) return;
+    const size_t pix = (size_t)y * a.ld + x;
+    const int n = a.n[pix];
+    const float nf = (float)n;
+    for (int s = 0; s < a.S; ++s) {
+        float jbs = 0.f;
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = ((size_t)s * 3 + c) * a.plane + pix;
+            auto at = [&](int st) { return a.stats[(((size_t)s * N_STATS + st) * 3 + c) * a.plane + pix]; };
+            if (n < 4) { a.lnS2[o] = 0.f; a.se2[o] = 0.f; a.m2C[o] = 0.f; continue; }
+            const float m2 = at(ST_M2), m3 = at(ST_M3), m4 = at(ST_M4);
+            if (!(__builtin_isfinite(m2) && __builtin_isfinite(m4))) {
+                a.lnS2[o] = INFINITY; a.se2[o] = INFINITY; a.m2C[o] = m2; jbs = INFINITY;
+                continue;
+            }
+            if (m2 > 0.f) {
+                const float sm = sqrtf(m2);
+                const float S3 = sqrtf(nf) * (m3 / m2) / sm;       // (m3/n)/(m2/n)^1.5
+                const float K = nf * (m4 / m2) / m2;               // (m4/n)/(m2/n)^2
+                jbs += nf / 6.f * (S3 * S3 + 0.25f * (K - 3.f) * (K - 3.f));
+                const float cb = nf / (nf - a.z);
+                const float s2 = m2 / (nf - 1.f);
+                a.lnS2[o] = logf(fmaxf(cb * s2, 1e-30f));
+                a.se2[o] = cb * cb * (K - (nf - 3.f) / nf) / (nf - 1.f);
+                float m2c = m2;
+                if (a.lev) {
+                    const float cx = at(ST_C2X) / sm, cy = at(ST_C2Y) / sm;
+                    m2c = m2 * fmaxf(0.f, 1.f - 12.f * (cx * cx + cy * cy) / nf);
+                }
+                a.m2C[o] = m2c;
+            } else {
+                a.lnS2[o] = logf(1e-30f);
+                a.se2[o] = 0.f;
+                a.m2C[o] = 0.f;
+            }
+        }
+        a.jb[(size_t)s * a.plane + pix] = n < 4 ? 0.f : jbs / 3.f;
+    }
+}
+
+int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
+    PreArgs a{bs->n, bs->stats, bs->jb, bs->lnS2, bs->se2, bs->m2C, bs->S, bs->W, bs->H, bs->ld,
+              bs->mm >= 5 ? 1 : 0, bs->plane, var_z};
+    LaunchTimer t(bs->ctx, st, "prepass");
+    hipLaunchKernelGGL(k_prepass, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, a);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+// ======================================================= variance filter
+struct VarArgs {
+    const int32_t *n;
+    const float *m2;  // set-0 base of the m2 statistic: stats + ST_M2 offset
+    const float *lnS2, *se2, *m2C;
+    float *varD, *varCD;
+    FeatArgs f;
+    int W, H, ld, r, g0;  // g0: first set of this group
+    size_t plane, set_stride;  // distance between sets of the statistics block
+    float z2, sp_scale;        // sp_scale = ds_factor * log2 e
+};
+
+// Quantities per staged halo pixel: ok, features, then per set/channel
+// L = lnS2, Z = z^2 se2, V = s^2, C = m2C/(n-1).
+template <int G>
+__device__ __forceinline__ void var_stage_row(const VarArgs &a, float *ring, int Q, int HW, int slot, int yq, int x0) {
+    const int items = HW * (1 + 3 * G);
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        const int kind = i / HW, col = i - kind * HW;
+        const int xq = x0 - a.r + col;
+        const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+        float *row = ring + (size_t)slot * Q * HW + col;
+        if (kind == 0) {
+            float ok = 0.f;
+            if (in) ok = a.n[(size_t)yq * a.ld + xq] >= 4 ? 1.f : 0.f;
+            row[0] = ok;
+            for (int k = 0; k < a.f.nf; ++k) row[(1 + k) * HW] = in ? feat_at(a.f, k, xq, yq) : 0.f;
+        } else {
+            const int sc = kind - 1;  // g*3 + c within the group
+            const int g = sc / 3, c = sc - g * 3;
+            float L = 0.f, Z = 0.f, V = 0.f, C = 0.f;
+            if (in) {
+                const size_t pix = (size_t)yq * a.ld + xq;
+                const int n = a.n[pix];
+                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+                if (n >= 4) { L = a.lnS2[o]; Z = a.z2 * a.se2[o]; }
+                if (n >= 2) {
+                    const float inv = 1.f / (float)(n - 1);
+                    V = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
+                    C = a.m2C[o] * inv;
+                }
+            }
+            const int base = 1 + a.f.nf + sc * 4;
+            row[base * HW] = L;
+            row[(base + 1) * HW] = Z;
+            row[(base + 2) * HW] = V;
+            row[(base + 3) * HW] = C;
+        }
+    }
+}
+
+template <int G, bool HASW>
+__global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
+    extern __shared__ float ring[];
+    const int HW = TX + 2 * a.r;
+    const int Q = 1 + a.f.nf + 12 * G;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < a.W && y < a.H;
+
+    // own pixel
+    float Lp[G][3], Zp[G][3], den[G][3], num[G][3], numC[G][3];
+    float gp[kMaxFeat];
+    float okp = 0.f;
+    if (active) {
+        const size_t pix = (size_t)y * a.ld + x;
+        const int n = a.n[pix];
+        okp = n >= 4 ? 1.f : 0.f;
+        for (int k = 0; k < a.f.nf; ++k) gp[k] = feat_at(a.f, k, x, y);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+                Lp[g][c] = n >= 4 ? a.lnS2[o] : 0.f;
+                Zp[g][c] = n >= 4 ? a.z2 * a.se2[o] : 0.f;
+                float V = 0.f, C = 0.f;
+                if (n >= 2) {
+                    const float inv = 1.f / (float)(n - 1);
+                    V = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
+                    C = a.m2C[o] * inv;
+                }
+                den[g][c] = 1.f;  // the pixel itself, weight 1
+                num[g][c] = V;
+                numC[g][c] = C;
+            }
+    }
+
+    for (int t = 0; t < TY; ++t) var_stage_row<G>(a, ring, Q, HW, t, y0 - a.r + t, x0);
+    __syncthreads();
+    for (int k = 0; k <= 2 * a.r; ++k) {
+        const int dy = k - a.r;
+        const float *row = ring + (size_t)((ty + k) % TY) * Q * HW;
+        if (active && okp > 0.f) {
+            for (int dx = -a.r; dx <= a.r; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int col = tx + a.r + dx;
+                float w = row[col];
+                if (HASW) {
+                    float lw = a.sp_scale * (float)(dx * dx + dy * dy);
+                    for (int f = 0; f < a.f.nf; ++f) {
+                        const float d = gp[f] - row[(1 + f) * HW + col];
+                        lw = __builtin_fmaf(-d, d, lw);
+                    }
+                    w *= exp2f(lw);
+                }
+                const float *q = row + (1 + a.f.nf) * HW + col;
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float *qq = q + (g * 3 + c) * 4 * HW;
+                        const float dl = Lp[g][c] - qq[0];
+                        const bool acc = !(dl * dl > Zp[g][c] + qq[HW]);
+                        den[g][c] += acc ? w : 0.f;
+                        num[g][c] += acc ? w * qq[2 * HW] : 0.f;
+                        numC[g][c] += acc ? w * qq[3 * HW] : 0.f;
+                    }
+            }
+        }
+        if (k == 2 * a.r) break;
+        __syncthreads();
+        var_stage_row<G>(a, ring, Q, HW, k % TY, y0 - a.r + k + TY, x0);
+        __syncthreads();
+    }
+    if (!active) return;
+    const size_t pix = (size_t)y * a.ld + x;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+            a.varD[o] = num[g][c] / den[g][c];
+            a.varCD[o] = numC[g][c] / den[g][c];
+        }
+}
+
+template <int G>
+static int var_launch(VarArgs &a, bool hasw, dim3 grid, hipStream_t st) {
+    const int Q = 1 + a.f.nf + 12 * G;
+    const size_t lds = (size_t)TY * Q * (TX + 2 * a.r) * 4;
+    if (hasw) {
+        STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_var_filter<G, true>), grid, dim3(256), lds, st, a);
+    } else {
+        STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_var_filter<G, false>), grid, dim3(256), lds, st, a);
+    }
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                      hipStream_t st) {
+    VarArgs a{};
+    a.n = bs->n;
+    a.m2 = bs->stat(0, ST_M2, 0);
+    a.lnS2 = bs->lnS2; a.se2 = bs->se2; a.m2C = bs->m2C;
+    a.varD = bs->varD; a.varCD = bs->varCD;
+    a.f = make_feats(g);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
+    a.plane = bs->plane;
+    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    a.z2 = p->var_z * p->var_z;
+    a.sp_scale = p->ds_factor * kLog2e;
+    const bool hasw = a.f.nf > 0 || p->ds_factor != 0.f;
+    dim3 grid((bs->W + TX - 1) / TX, (bs->H + TY - 1) / TY);
+    LaunchTimer t(bs->ctx, st, "var_filter");
+    for (int g0 = 0; g0 < bs->S;) {
+        const int left = bs->S - g0;
+        const int G = left >= 3 ? 3 : left;
+        a.g0 = g0;
+        int rc = G == 3 ? var_launch<3>(a, hasw, grid, st) : G == 2 ? var_launch<2>(a, hasw, grid, st)
+                                                                     : var_launch<1>(a, hasw, grid, st);
+        if (rc) return rc;
+        g0 += G;
+    }
+    return STATER_OK;
+}
+
+// ========================================================== Welch filter
+struct WelchArgs {
+    const int32_t *n;
+    const float *mean;     // statistics block base (set s mean = mean + s*set_stride)
+    const float *var;      // variance fed to the test, [S][3][plane] (or m2 for the raw test)
+    int var_is_m2;         // 1: var = m2/(n-1) computed on the fly from the statistics
+    const float *film;     // filtered values and set-0 test mean (planar, 3 ch)
+    int64_t film_pitch, film_cs;
+    const float *jb;
+    float *out;            // filmD
+    int64_t out_pitch, out_cs;
+    float *discr;          // [S][3][plane]
+    float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
+    uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
+    FeatArgs f;
+    int W, H, ld, r, mode3;
+    size_t plane, set_stride;
+    float sp_scale;
+    double alpha;
+    TCritPoly tp;
+};
+
+// FP64 tail for pairs near the critical value (rare; kept out of line).
+__device__ __noinline__ bool welch_exact(float mp, float vp, float nm1p, float mq, float vq, float nm1q, double alpha) {
+    const double V = (double)vp + (double)vq, dm = (double)mp - (double)mq;
+    if (!(V > 0.0)) return dm == 0.0;
+    const double t2 = dm * dm / V;
+    const double Wd = (double)vp * vp / (double)nm1p + (double)vq * vq / (double)nm1q;
+    const double nu = Wd > 0.0 ? V * V / Wd : INFINITY;
+    return t2_tail(t2, nu) > alpha;
+}
+
+__device__ __forceinline__ float tcrit_eval(const TCritPoly &tp, float u) {
+    const float s = __builtin_fmaf(u, tp.scale, tp.shift);
+    float acc = tp.c[kTCritDeg];
+#pragma unroll
+    for (int k = kTCritDeg - 1; k >= 0; --k) acc = __builtin_fmaf(acc, s, tp.c[k]);
+    return acc;
+}
+
+// Staged quantities per halo pixel: ok (n >= 2), nm1 = n-1, features,
+// film[3], then per set/channel M (mean; NaN = abstain) and V (v = var/n),
+// then, for the plain method, s^2 [3].
+template <int S, bool PLAIN>
+__device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, int s, int c, float &M, float &V) {
+    const size_t pix = (size_t)yq * a.ld + xq;
+    const int n = a.n[pix];
+    M = s == 0 ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 + c * a.film_cs)
+               : a.mean[(size_t)s * a.set_stride + c * a.plane + pix];
+    float var;
+    if (a.var_is_m2) var = n >= 2 ? a.var[(size_t)s * a.set_stride + c * a.plane + pix] / (float)(n - 1) : 0.f;
+    else var = a.var[((size_t)s * 3 + c) * a.plane + pix];
+    V = n >= 1 ? var / (float)n : 0.f;
+    if (!(__builtin_isfinite(M) && __builtin_isfinite(V))) { M = NAN; V = 0.f; }
+}
+
+template <int S, bool PLAIN>
+__device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *ring, int Q, int HW, int slot, int yq, int x0) {
+    const int items = HW * (1 + 3 * S);
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        const int kind = i / HW, col = i - kind * HW;
+        const int xq = x0 - a.r + col;
+        const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+        float *row = ring + (size_t)slot * Q * HW + col;
+        if (kind == 0) {
+            float ok = 0.f, nm1 = 1.f;
+            if (in) {
+                const int n = a.n[(size_t)yq * a.ld + xq];
+                ok = n >= 2 ? 1.f : 0.f;
+                nm1 = n >= 2 ? (float)(n - 1) : 1.f;
+            }
+            row[0] = ok;
+            row[HW] = nm1;
+            int q = 2;
+            for (int k = 0; k < a.f.nf; ++k, ++q) row[q * HW] = in ? feat_at(a.f, k, xq, yq) : 0.f;
+            for (int c = 0; c < 3; ++c, ++q)
+                row[q * HW] = in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch +
+                                                    (int64_t)xq * 4 + c * a.film_cs)
+                                 : 0.f;
+            if (PLAIN) {
+                const int base = 2 + a.f.nf + 3 + 6 * S;
+                for (int c = 0; c < 3; ++c) {
+                    float s2 = 0.f;
+                    if (in) {
+                        const size_t pix = (size_t)yq * a.ld + xq;
+                        const int n = a.n[pix];
+                        if (n >= 2) s2 = a.var[c * a.plane + pix] / (float)(n - 1);
+                    }
+                    row[(base + c) * HW] = s2;
+                }
+            }
+        } else {
+            const int sc = kind - 1, s = sc / 3, c = sc - s * 3;
+            float M = 0.f, V = 0.f;
+            if (in) welch_load<S, PLAIN>(a, xq, yq, s, c, M, V);
+            const int base = 2 + a.f.nf + 3 + sc * 2;
+            row[base * HW] = M;
+            row[(base + 1) * HW] = V;
+        }
+    }
+}
+
+template <int S, bool HASW, bool PLAIN>
+__global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
+    extern __shared__ float ring[];
+    const int HW = TX + 2 * a.r;
+    const int Q = 2 + a.f.nf + 3 + 6 * S + (PLAIN ? 3 : 0);
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < a.W && y < a.H;
+    const int win = 2 * a.r + 1;
+
+    float Mp[S][3], Vp[S][3];
+    float gp[kMaxFeat];
+    float okp = 0.f, nm1p = 1.f, rp = 0.f;
+    float den[3], num[3], numV[3];
+    bool use[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) use[s] = true;
+    if (active) {
+        const size_t pix = (size_t)y * a.ld + x;
+        const int n = a.n[pix];
+        okp = n >= 2 ? 1.f : 0.f;
+        nm1p = n >= 2 ? (float)(n - 1) : 1.f;
+        rp = __builtin_amdgcn_rcpf(nm1p);
+        for (int k = 0; k < a.f.nf; ++k) gp[k] = feat_at(a.f, k, x, y);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                welch_load<S, PLAIN>(a, x, y, s, c, Mp[s][c], Vp[s][c]);
+                a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
+            }
+        if (a.mode3) {
+            int best = 0;
+            float bj = a.jb[pix];
+            for (int s = 1; s < S; ++s) {
+                const float j = a.jb[(size_t)s * a.plane + pix];
+                if (j < bj) { bj = j; best = s; }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) use[s] = s == best;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            den[c] = 1.f;
+            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+            numV[c] = 0.f;
+            if (PLAIN) numV[c] = n >= 2 ? a.var[c * a.plane + pix] / (float)(n - 1) : 0.f;
+        }
+        if (a.dec) {
+            const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
+            a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
+        }
+    }
+
+    for (int t = 0; t < TY; ++t) welch_stage_row<S, PLAIN>(a, ring, Q, HW, t, y0 - a.r + t, x0);
+    __syncthreads();
+    for (int k = 0; k <= 2 * a.r; ++k) {
+        const int dy = k - a.r;
+        const float *row = ring + (size_t)((ty + k) % TY) * Q * HW;
+        if (active) {
+            for (int dx = -a.r; dx <= a.r; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int col = tx + a.r + dx;
+                const float okq = row[col];
+                const float nm1q = row[HW + col];
+                const float rq = __builtin_amdgcn_rcpf(nm1q);
+                float w = okq * okp;
+                if (HASW) {
+                    float lw = a.sp_scale * (float)(dx * dx + dy * dy);
+                    for (int f = 0; f < a.f.nf; ++f) {
+                        const float d = gp[f] - row[(2 + f) * HW + col];
+                        lw = __builtin_fmaf(-d, d, lw);
+                    }
+                    w *= exp2f(lw);
+                }
+                const float *fq = row + (2 + a.f.nf) * HW + col;
+                const float *mv = fq + 3 * HW;
+                bool acc[3] = {okq * okp > 0.f, okq * okp > 0.f, okq * okp > 0.f};
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float mq = mv[(s * 3 + c) * 2 * HW];
+                        const float vq = mv[((s * 3 + c) * 2 + 1) * HW];
+                        const float vp = Vp[s][c];
+                        const float V = vp + vq;
+                        const float dm = Mp[s][c] - mq;
+                        const float iV = __builtin_amdgcn_rcpf(V);
+                        const float ap = vp * iV, aq = vq * iV;
+                        const float u = __builtin_fmaf(ap * ap, rp, aq * aq * rq);
+                        const float t2 = (dm * iV) * dm;
+                        const float crit = tcrit_eval(a.tp, u);
+                        bool pass = !(t2 > crit || t2 == INFINITY);
+                        const bool near = fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax;
+                        if (near && acc[c] && use[s]) pass = welch_exact(Mp[s][c], vp, nm1p, mq, vq, nm1q, a.alpha);
+                        acc[c] = acc[c] && (pass || !use[s]);
+                    }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    den[c] += acc[c] ? w : 0.f;
+                    num[c] += acc[c] ? w * fq[c * HW] : 0.f;
+                    if (PLAIN) numV[c] += acc[c] ? w * fq[(3 + 6 * S + c) * HW] : 0.f;
+                }
+                if (a.dec) {
+                    const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
+                    const int xq = x + dx, yq = y + dy;
+                    const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+                    for (int c = 0; c < 3; ++c) a.dec[kk + c] = (in && acc[c]) ? 1 : 0;
+                }
+            }
+        }
+        if (k == 2 * a.r) break;
+        __syncthreads();
+        welch_stage_row<S, PLAIN>(a, ring, Q, HW, k % TY, y0 - a.r + k + TY, x0);
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = num[c] / den[c];
+        if (PLAIN) a.varW[c * a.plane + (size_t)y * a.ld + x] = numV[c] / den[c];
+    }
+}
+
+template <int S, bool HASW, bool PLAIN>
+static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
+    const int Q = 2 + a.f.nf + 3 + 6 * S + (PLAIN ? 3 : 0);
+    const size_t lds = (size_t)TY * Q * (TX + 2 * a.r) * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN>), grid, dim3(256), lds, st, a);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+template <int S>
+static int welch_launch2(const WelchArgs &a, bool hasw, bool plain, dim3 grid, hipStream_t st) {
+    if (plain) return welch_launch3<S, false, true>(a, grid, st);
+    return hasw ? welch_launch3<S, true, false>(a, grid, st) : welch_launch3<S, false, false>(a, grid, st);
+}
+
+static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                        const stater_plane *film, const stater_plane *film_out, bool plain, uint8_t *dec,
+                        hipStream_t st) {
+    WelchArgs a{};
+    a.n = bs->n;
+    a.mean = bs->stat(0, ST_MEAN, 0);
+    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    if (plain || !p->cascade) {
+        a.var = bs->stat(0, ST_M2, 0);
+        a.var_is_m2 = 1;
+    } else {
+        a.var = p->use_levc ? bs->varCD : bs->varD;
+        a.var_is_m2 = 0;
+    }
+    if (film) {
+        a.film = (const float *)film->ptr;
+        a.film_pitch = film->pitch_bytes;
+        a.film_cs = film->layout == STATER_PLANAR ? film->channel_stride_bytes : 4;
+        if (film->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film must be planar");
+    } else {
+        a.film = bs->stat(0, ST_MEAN, 0);
+        a.film_pitch = (int64_t)bs->ld * 4;
+        a.film_cs = (int64_t)bs->plane * 4;
+    }
+    float *outp = plain ? bs->meanD : bs->filmD;
+    a.out = outp;
+    a.out_pitch = (int64_t)bs->ld * 4;
+    a.out_cs = (int64_t)bs->plane * 4;
+    if (film_out) {
+        if (film_out->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film_out must be planar");
+        a.out = (float *)film_out->ptr;
+        a.out_pitch = film_out->pitch_bytes;
+        a.out_cs = film_out->channel_stride_bytes;
+    }
+    a.jb = bs->jb;
+    a.discr = bs->discr;
+    a.varW = bs->varD;
+    a.dec = dec;
+    a.f = plain ? FeatArgs{} : make_feats(g);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
+    a.mode3 = (!plain && p->mt_mode == 3 && bs->S > 1) ? 1 : 0;
+    a.plane = bs->plane;
+    a.sp_scale = plain ? 0.f : p->ds_factor * kLog2e;
+    a.alpha = (double)p->alpha;
+    a.tp = tcrit_for(bs->ctx, (double)p->alpha);
+    const int S = plain ? 1 : bs->S;
+    const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
+    dim3 grid((bs->W + TX - 1) / TX, (bs->H + TY - 1) / TY);
+    LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
+    switch (S) {
+        case 1: return welch_launch2<1>(a, hasw, plain, grid, st);
+        case 2: return welch_launch2<2>(a, hasw, plain, grid, st);
+        case 3: return welch_launch2<3>(a, hasw, plain, grid, st);
+        case 4: return welch_launch2<4>(a, hasw, plain, grid, st);
+        case 5: return welch_launch2<5>(a, hasw, plain, grid, st);
+        default: return welch_launch2<6>(a, hasw, plain, grid, st);
+    }
+}
+
+int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                        const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st) {
+    return welch_common(bs, p, g, film, film_out, plain, nullptr, st);
+}
+
+int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                           const stater_plane *film, int plain, uint8_t *dec, hipStream_t st) {
+    return welch_common(bs, p, g, film, nullptr, plain != 0, dec, st);
+}
+
+// ============================================================ mean vars
+__global__ void __launch_bounds__(256) k_mean_var(const int32_t *n, const float *m2, float *out, int W, int ld,
+                                                  size_t plane) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const size_t pix = (size_t)y * ld + x;
+    const float nf = (float)n[pix];
+    for (int c = 0; c < 3; ++c) out[c * plane + pix] = m2[c * plane + pix] / ((nf - 1.f) * nf);
+}
+
+int launch_mean_vars(stater_bufset *bs, hipStream_t st) {
+    LaunchTimer t(bs->ctx, st, "mean_var");
+    hipLaunchKernelGGL(k_mean_var, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->n,
+                       bs->stat(0, ST_M2, 0), bs->meanVar, bs->W, bs->ld, bs->plane);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+// ==================================================== adaptive sampling
+// Integer-exact water filling.  e_p (FP32, fixed operation order) is turned
+// into q_p = floor(e_p 2^16); f(lam) = sum_p min(floor(lam q_p), h_p) with
+// h_p = cap - n_p is an integer sum, so every reduction is exact and the
+// result does not depend on the reduction order.  A search over lam in FP64
+// finds lo < hi with f(lo) <= B < f(hi); targets are min(floor(lo q), h) plus
+// the first R = B - f(lo) of the increments to hi, in pixel order.
+constexpr int kAsCand = 16;
+
+struct AsCtl {
+    unsigned long long sumq, sumh;
+    double lo, hi;
+    double cand[kAsCand];
+    unsigned long long f[kAsCand];
+    unsigned long long flo;
+    int phase;  // 0 bracket search, 1 refine, 2 done, 3 all pixels capped
+    int uniform;
+    unsigned long long B;
+    unsigned long long remaining;
+};
+
+struct AsArgs {
+    const int32_t *n;
+    const float *varD, *mean;
+    int W, H, ld;
+    size_t plane;
+    uint32_t cap;
+    uint32_t *q;
+    int32_t *h;
+    int32_t *inc;
+    int64_t *bsum;
+    AsCtl *ctl;
+    int32_t *out;
+    int64_t out_pitch;
+};
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
+    __shared__ unsigned long long sh[4];
+    const int P = a.W * a.H;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long q = 0, h = 0;
+    if (p < P) {
+#pragma clang fp contract(off)
+        const int y = p / a.W, x = p - y * a.W;
+        const size_t pix = (size_t)y * a.ld + x;
+        const int n = a.n[pix];
+        float e = 0.f;
+        if (n > 0) {
+            const float nf = (float)n;
+            float ec[3];
+            for (int c = 0; c < 3; ++c) {
+                const float vd = fmaxf(a.varD[c * a.plane + pix], 0.f);
+                const float m = fmaxf(fabsf(a.mean[c * a.plane + pix]), 1e-3f);
+                ec[c] = sqrtf(vd / nf) / m;  // correctly rounded (IEEE) square root
+            }
+            e = (ec[0] + ec[1] + ec[2]) / 3.f;
+        }
+        const float qe = e * 65536.f;
+        q = !(qe > 0.f) ? 0ull : (qe >= 4294967295.f ? 4294967295ull : (unsigned long long)qe);
+        const long long hh = (long long)a.cap - (long long)n;
+        h = hh > 0 ? (unsigned long long)hh : 0ull;
+        a.q[p] = (uint32_t)q;
+        a.h[p] = (int32_t)h;
+    }
+    const unsigned long long sq = block_sum_u64(q, sh);
+    const unsigned long long shh = block_sum_u64(h, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&a.ctl->sumq, sq);
+        atomicAdd(&a.ctl->sumh, shh);
+    }
+}
+
+__global__ void k_as_init(AsCtl *c, unsigned long long B, int P) {
+    c->B = B;
+    for (int i = 0; i < kAsCand; ++i) c->f[i] = 0;
+    if (c->sumh <= B) { c->phase = 3; return; }
+    c->uniform = c->sumq == 0;
+    const double sq = c->uniform ? (double)P : (double)c->sumq;
+    c->lo = 0.0;
+    c->hi = (double)B / sq;
+    for (int i = 0; i < kAsCand; ++i) c->cand[i] = c->hi * (double)(1ull << i);
+    c->phase = 0;
+}
+
+__device__ __forceinline__ long long as_take(double lam, uint32_t q, int32_t h) {
+    const double t = floor(lam * (double)q);
+    return t >= (double)h ? (long long)h : (long long)t;
+}
+
+__global__ void __launch_bounds__(256) k_as_count(AsArgs a) {
+    __shared__ unsigned long long sh[4];
+    AsCtl *c = a.ctl;
+    if (c->phase >= 2) return;
+    const int P = a.W * a.H;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t q = 0;
+    int32_t h = 0;
+    if (p < P) { q = c->uniform ? 1u : a.q[p]; h = a.h[p]; }
+    for (int i = 0; i < kAsCand; ++i) {
+        const unsigned long long v = p < P ? (unsigned long long)as_take(c->cand[i], q, h) : 0ull;
+        const unsigned long long s = block_sum_u64(v, sh);
+        if (threadIdx.x == 0) atomicAdd(&c->f[i], s);
+    }
+}
+
+__global__ void k_as_step(AsCtl *c) {
+    if (c->phase >= 2) return;
+    const unsigned long long B = c->B;
+    if (c->phase == 0) {
+        int j = -1;
+        for (int i = 0; i < kAsCand; ++i)
+            if (c->f[i] > B) { j = i; break; }
+        if (j < 0) {
+            c->lo = c->cand[kAsCand - 1];
+            c->flo = c->f[kAsCand - 1];
+            const double base = c->cand[kAsCand - 1] * 2.0;
+            for (int i = 0; i < kAsCand; ++i) c->cand[i] = base * (double)(1ull << i);
+        } else {
+            if (j > 0) { c->lo = c->cand[j - 1]; c->flo = c->f[j - 1]; }
+            c->hi = c->cand[j];
+            c->phase = 1;
+        }
+    } else {
+        double nlo = c->lo, nhi = c->hi;
+        unsigned long long nflo = c->flo;
+        for (int i = 0; i < kAsCand; ++i) {
+            if (c->f[i] <= B) { nlo = c->cand[i]; nflo = c->f[i]; }
+            else { nhi = c->cand[i]; break; }
+        }
+        c->lo = nlo; c->hi = nhi; c->flo = nflo;
+    }
+    if (c->phase == 1) {
+        bool moved = false;
+        for (int i = 0; i < kAsCand; ++i) {
+            const double t = c->lo + (c->hi - c->lo) * (double)(i + 1) / (double)(kAsCand + 1);
+            c->cand[i] = t;
+            if (t > c->lo && t < c->hi) moved = true;
+        }
+        if (!moved) c->phase = 2;
+    }
+    for (int i = 0; i < kAsCand; ++i) c->f[i] = 0;
+}
+
+// Per-block sums of the increments lo -> hi (first pass of the scan).
+__global__ void __launch_bounds__(256) k_as_inc(AsArgs a) {
+    __shared__ unsigned long long sh[4];
+    const AsCtl *c = a.ctl;
+    const int P = a.W * a.H;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    long long inc = 0;
+    if (p < P && c->phase != 3) {
+        const uint32_t q = c->uniform ? 1u : a.q[p];
+        const int32_t h = a.h[p];
+        inc = as_take(c->hi, q, h) - as_take(c->lo, q, h);
+        a.inc[p] = (int32_t)inc;
+    }
+    const unsigned long long s = block_sum_u64((unsigned long long)inc, sh);
+    if (threadIdx.x == 0) a.bsum[blockIdx.x] = (int64_t)s;
+}
+
+// Exclusive scan of the block sums (one workgroup, sequential chunks).
+__global__ void __launch_bounds__(256) k_as_scan_blocks(int64_t *bsum, int nb) {
+    __shared__ int64_t carry;
+    __shared__ int64_t tmp[256];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 256) {
+        const int i = base + threadIdx.x;
+        const int64_t v = i < nb ? bsum[i] : 0;
+        tmp[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int64_t t = threadIdx.x >= o ? tmp[threadIdx.x - o] : 0;
+            __syncthreads();
+            tmp[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + tmp[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 255) carry += tmp[255];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
+    __shared__ int64_t tmp[256];
+    const AsCtl *c = a.ctl;
+    const int P = a.W * a.H;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t inc = 0;
+    if (p < P && c->phase != 3) inc = a.inc[p];
+    tmp[threadIdx.x] = inc;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int64_t t = threadIdx.x >= o ? tmp[threadIdx.x - o] : 0;
+        __syncthreads();
+        tmp[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (p >= P) return;
+    const int y = p / a.W, x = p - y * a.W;
+    int32_t *o = (int32_t *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4);
+    if (c->phase == 3) { *o = a.h[p]; return; }
+    const uint32_t q = c->uniform ? 1u : a.q[p];
+    const int64_t base = as_take(c->lo, q, a.h[p]);
+    const int64_t R = (int64_t)(c->B - c->flo);
+    const int64_t E = a.bsum[blockIdx.x] + tmp[threadIdx.x] - inc;  // exclusive prefix
+    int64_t add = R - E;
+    add = add < 0 ? 0 : (add > inc ? inc : add);
+    *o = (int32_t)(base + add);
+}
+
+int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out, hipStream_t st) {
+    const int P = bs->W * bs->H;
+    const int nb = (P + 255) / 256;
+    const size_t need = (size_t)P * 12 + (size_t)nb * 8 + sizeof(AsCtl) + 256;
+    if (bs->as_scratch_bytes < need) {
+        if (bs->as_scratch) { STATER_HIP(hipStreamSynchronize(st)); STATER_HIP(hipFree(bs->as_scratch)); }
+        bs->as_scratch = nullptr;
+        bs->as_scratch_bytes = 0;
+        if (hipMalloc(&bs->as_scratch, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(STATER_E_NOMEM, "stater_calc_target_spp: scratch allocation failed");
+        }
+        bs->as_scratch_bytes = need;
+    }
+    char *sp = (char *)bs->as_scratch;
+    AsArgs a{};
+    a.n = bs->n;
+    a.varD = bs->varD;
+    a.mean = bs->stat(0, ST_MEAN, 0);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.plane = bs->plane; a.cap = cap;
+    a.ctl = (AsCtl *)sp; sp += (sizeof(AsCtl) + 255) / 256 * 256;
+    a.q = (uint32_t *)sp; sp += (size_t)P * 4;
+    a.h = (int32_t *)sp; sp += (size_t)P * 4;
+    a.inc = (int32_t *)sp; sp += (size_t)P * 4;
+    a.bsum = (int64_t *)sp;
+    a.out = out ? (int32_t *)out->ptr : bs->targetspp;
+    a.out_pitch = out ? out->pitch_bytes : (int64_t)bs->ld * 4;
+    LaunchTimer t(bs->ctx, st, "target_spp");
+    int rc = launch_zero(a.ctl, sizeof(AsCtl), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_as_err, dim3(nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_as_init, dim3(1), dim3(1), 0, st, a.ctl, (unsigned long long)budget * (unsigned long long)P, P);
+    // Each count/step pair narrows [lo, hi] 17-fold (bracketing first); 40 pairs
+    // exceed what a double can resolve, and finished searches skip the work.
+    for (int it = 0; it < 40; ++it) {
+        hipLaunchKernelGGL(k_as_count, dim3(nb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_as_step, dim3(1), dim3(1), 0, st, a.ctl);
+    }
+    hipLaunchKernelGGL(k_as_inc, dim3(nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_as_scan_blocks, dim3(1), dim3(256), 0, st, a.bsum, nb);
+    hipLaunchKernelGGL(k_as_final, dim3(nb), dim3(256), 0, st, a);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+}  // namespace stater
+
+extern "C" int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *p,
+                                           const stater_gbuffers *gbufs, const stater_plane *film, int plain,
+                                           uint8_t *dec, void *stream) {
+    if (!bs || !p || !dec) return stater::fail(STATER_E_ARG, "stater_debug_film_decisions: NULL argument");
+    return stater::launch_welch_decisions(bs, p, gbufs, film, plain, dec, stater::pick_stream(bs, stream));
+}

@@ -1,0 +1,95 @@
+// internal.hpp — library-internal objects shared by the C-ABI and the kernels.
+#pragma once
+
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "tdist.hpp"
+
+struct stater_ctx {
+    int device = 0;
+    unsigned flags = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    std::mutex mu;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    struct Acc { uint64_t launches = 0; double ms = 0.0; };
+    std::map<std::string, Acc> stats;
+    std::vector<hipEvent_t> free_events;
+    std::map<double, stater::TCritPoly> tcrit;  // fitted critical-value polynomials per level
+    std::atomic<int> live_bufsets{0};           // a context outlives its buffer sets
+};
+
+struct stater_bufset {
+    stater_ctx *ctx = nullptr;
+    int W = 0, H = 0;
+    int ld = 0;          // row pitch in elements
+    size_t plane = 0;    // elements per plane (ld * H)
+    int S = 1;           // statistics sets: 0 = untransformed, 1.. = Box-Cox
+    float set_lambda[stater::kMaxSets] = {1.f};
+    int n_lambdas = 0;
+    float lambdas[STATER_MAX_LAMBDAS] = {};
+    int slot_set[STATER_MAX_LAMBDAS] = {};
+    int mm = 5;
+    unsigned flags = 0;
+    char *base = nullptr;
+    size_t bytes = 0;
+    // device planes
+    int32_t *n = nullptr;
+    float *stats = nullptr;  // [S][7][3][plane]
+    float *lref = nullptr;   // [3][plane] Box-Cox reference point ln x1 (sets 1..)
+    float *jb = nullptr;     // [S][plane]
+    float *lnS2 = nullptr, *se2 = nullptr, *m2C = nullptr, *varD = nullptr, *varCD = nullptr,
+          *discr = nullptr;  // [S][3][plane]
+    float *meanVar = nullptr, *meanD = nullptr, *filmD = nullptr;  // [3][plane]
+    int32_t *targetspp = nullptr;
+    // adaptive-sampling scratch
+    void *as_scratch = nullptr;
+    size_t as_scratch_bytes = 0;
+
+    float *stat(int s, int st, int c) const {
+        return stats + (((size_t)s * stater::N_STATS + st) * 3 + c) * plane;
+    }
+    float *per_set(float *b, int s, int c) const { return b + ((size_t)s * 3 + c) * plane; }
+};
+
+namespace stater {
+
+// Kernel-launch timing: wraps one launch with events when the context asks.
+struct LaunchTimer {
+    stater_ctx *ctx;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    const char *name;
+    LaunchTimer(stater_ctx *c, hipStream_t s, const char *n);
+    ~LaunchTimer();
+};
+
+hipStream_t pick_stream(const stater_bufset *bs, void *stream);
+const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
+
+// kernels (accumulate.hip, denoise.hip, synth.hip)
+int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
+int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st);
+int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                      hipStream_t st);
+int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                        const stater_plane *film, const stater_plane *film_out, bool plain,
+                        hipStream_t st);
+int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out,
+                      hipStream_t st);
+int launch_mean_vars(stater_bufset *bs, hipStream_t st);
+int launch_zero(void *p, size_t bytes, hipStream_t st);
+int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
+                           const stater_plane *film, int plain, uint8_t *dec, hipStream_t st);
+int launch_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo,
+                          const stater_plane *normal, const stater_plane *depth, hipStream_t st);
+int launch_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo,
+                         uint32_t first, const stater_samples *out, hipStream_t st);
+
+}  // namespace stater

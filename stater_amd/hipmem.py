@@ -1,0 +1,115 @@
+"""Minimal HIP runtime plumbing over ctypes: device buffers and copies.
+
+Used by the Python layer, the tests and the bench to hand device memory to
+the C-ABI and to read planes back.  Copies use the synchronous hipMemcpy on
+the null stream; callers synchronise the library's stream before reading
+results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+_hip = None
+H2D, D2H, D2D = 1, 2, 3
+
+
+def hip() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise OSError("libamdhip64.so not found")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        _hip.hipGetErrorString.restype = C.c_char_p
+    return _hip
+
+
+def _ok(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {hip().hipGetErrorString(rc).decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    try:
+        rc = hip().hipGetDeviceCount(C.byref(n))
+    except OSError:
+        return 0
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """An owned device allocation."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _ok(hip().hipMalloc(C.byref(p), max(self.nbytes, 256)), "hipMalloc")
+        self.ptr = p.value
+
+    def free(self) -> None:
+        if self.ptr:
+            hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, arr: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(arr)
+        assert offset + a.nbytes <= self.nbytes
+        _ok(hip().hipMemcpy(C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D), "H2D")
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        _ok(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), out.nbytes, D2H), "D2H")
+        return out
+
+    def zero(self) -> None:
+        _ok(hip().hipMemset(C.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
+
+
+def read_plane(pl, dtype=None) -> np.ndarray:
+    """Copy a planar device plane (stater Plane) to a numpy array [channels, H, W]."""
+    dt = dtype or (np.int32 if pl.dtype == 1 else np.float32)
+    pitch_elems = pl.pitch_bytes // 4
+    out = np.empty((pl.channels, pl.height, pl.width), dtype=dt)
+    for c in range(pl.channels):
+        nbytes = pl.pitch_bytes * (pl.height - 1) + pl.width * 4
+        tmp = np.empty(nbytes // 4, dtype=dt)
+        _ok(hip().hipMemcpy(tmp.ctypes.data_as(C.c_void_p), C.c_void_p(pl.ptr + c * pl.channel_stride_bytes),
+                            nbytes, D2H), "D2H plane")
+        full = np.empty(pitch_elems * pl.height, dtype=dt)
+        full[: tmp.size] = tmp
+        out[c] = full.reshape(pl.height, pitch_elems)[:, : pl.width]
+    return out
+
+
+def write_plane(pl, arr: np.ndarray) -> None:
+    """Upload [channels, H, W] into a planar device plane (row by row pitch)."""
+    dt = np.int32 if pl.dtype == 1 else np.float32
+    a = np.asarray(arr, dtype=dt).reshape(pl.channels, pl.height, pl.width)
+    pitch_elems = pl.pitch_bytes // 4
+    for c in range(pl.channels):
+        full = np.zeros((pl.height, pitch_elems), dtype=dt)
+        full[:, : pl.width] = a[c]
+        nbytes = pl.pitch_bytes * (pl.height - 1) + pl.width * 4
+        flat = np.ascontiguousarray(full.reshape(-1)[: nbytes // 4])
+        _ok(hip().hipMemcpy(C.c_void_p(pl.ptr + c * pl.channel_stride_bytes), flat.ctypes.data_as(C.c_void_p),
+                            nbytes, H2D), "H2D plane")

@@ -1,0 +1,141 @@
+"""GPU parity of the sample generator and the moment accumulation.
+
+The untransformed statistics must be bit-identical to the oracle's FP32
+restatement of StatTile (src/statistics/estimator.h:226-324) on the same
+samples; the Box-Cox statistics are compared with the FP64 evaluation of the
+same recurrence at the relative tolerance stated below.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import stater_amd as sa
+from gpu_helpers import generate, read_stats
+from stater_amd import hipmem
+from stater_amd.workload import MT5_LAMBDAS, SampleChunk
+
+pytestmark = pytest.mark.gpu
+
+# Box-Cox statistics against FP64, relative to each statistic's natural scale.
+BOXCOX_RTOL = 1e-4
+
+
+def _scales(st64, n):
+    """Natural magnitude per statistic [7,3,H,W] for relative errors."""
+    nn = np.maximum(n, 1)[None].astype(np.float64)
+    m2 = np.abs(st64[1])
+    sd = np.sqrt(m2 / nn)
+    return np.stack([
+        np.maximum(np.abs(st64[0]), sd),            # mean
+        m2,                                         # m2
+        m2 * sd,                                    # m3 ~ n sd^3
+        np.abs(st64[3]),                            # m4
+        np.sqrt(m2 * nn / 12.0),                    # c2X ~ sqrt(n/12) sd sqrt(n)
+        np.sqrt(m2 * nn / 12.0),                    # c2Y
+        np.abs(st64[6]),                            # m1
+    ])
+
+
+def test_samples_match_cpu_generator(ctx):
+    sc, chunk, feats = generate(ctx, 2, 7, 80, 48, 6)
+    f_cpu = oracle.features(sc)
+    f_gpu = feats.to_numpy()
+    for k in ("albedo", "normal", "depth"):
+        mism = np.count_nonzero(f_cpu[k] != f_gpu[k])
+        assert mism <= 2, (k, mism)
+        np.testing.assert_allclose(f_gpu[k], f_cpu[k], rtol=2e-7, atol=0)
+    s_cpu = oracle.samples(sc, f_gpu["albedo"], 0, 6)
+    s_gpu = chunk.to_numpy()
+    mism = np.count_nonzero(s_cpu != s_gpu)
+    assert mism <= max(2, s_cpu.size // 100000), mism
+    np.testing.assert_allclose(s_gpu, s_cpu, rtol=2.5e-7, atol=0)
+
+
+def test_accumulate_plain_bitexact(ctx):
+    """128x128, 64 spp, identity transform, n/mean/M2/M3 (the plain case)."""
+    W = H = 128
+    sc, chunk, _ = generate(ctx, 1, 1, W, H, 64)
+    bs = sa.BufferSet(ctx, W, H, (), max_moment=3)
+    bs.accumulate(chunk.desc())
+    ctx.synchronize()
+    n, st = read_stats(bs, ())
+    n_ref, st_ref = oracle.accumulate(chunk.to_numpy(), (), 3)
+    assert np.array_equal(n, n_ref)
+    for k in range(3):
+        assert np.array_equal(st[0, k], st_ref[0, k]), k
+
+
+def test_accumulate_mt5_m4c(ctx):
+    W, H, spp = 96, 64, 64
+    sc, chunk, _ = generate(ctx, 2, 2, W, H, spp)
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, max_moment=5)
+    bs.accumulate(chunk.desc())
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    smp = chunk.to_numpy()
+    n_ref, st32 = oracle.accumulate(smp, MT5_LAMBDAS, 5)
+    _, st64 = oracle.accumulate(smp, MT5_LAMBDAS, 5, fp64=True)
+    assert np.array_equal(n, n_ref)
+    # untransformed set: the reference's FP32 numbers, bit for bit
+    assert np.array_equal(st[0], st32[0])
+    # every identity slot of the lambda list maps onto set 0
+    for s in range(1, st.shape[0]):
+        scale = _scales(st64[s], n)
+        fits = np.all(np.abs(st64[s]) < 1e30, axis=0)  # exclude FP32 overflow of lambda = 3
+        fits = np.broadcast_to(fits, st[s].shape)
+        err = np.abs(st[s].astype(np.float64) - st64[s]) / np.maximum(scale, 1e-30)
+        worst = float(np.max(np.where(fits, err, 0.0)))
+        assert worst <= BOXCOX_RTOL, (s, worst)
+        assert fits.mean() > 0.9
+
+
+def test_accumulate_in_chunks_continues_sequence(ctx):
+    W, H = 70, 33
+    sc, chunk, _ = generate(ctx, 2, 3, W, H, 40)
+    full = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5)
+    full.accumulate(chunk.desc())
+    parts = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5)
+    d = chunk.desc(spp=25)
+    parts.accumulate(d)
+    d2 = chunk.desc(spp=15)
+    d2.data = chunk.buf.ptr + 25 * 5 * chunk.plane_bytes
+    parts.accumulate(d2)
+    ctx.synchronize()
+    a = read_stats(full, MT5_LAMBDAS)
+    b = read_stats(parts, MT5_LAMBDAS)
+    assert np.array_equal(a[0], b[0])
+    # untransformed statistics continue bit for bit; the Box-Cox means are
+    # stored unshifted, so resuming re-derives the shifted mean (one rounding)
+    assert np.array_equal(a[1][0], b[1][0])
+    for s in range(1, a[1].shape[0]):
+        scale = _scales(a[1][s].astype(np.float64), a[0])
+        assert np.max(np.abs(a[1][s].astype(np.float64) - b[1][s]) / np.maximum(scale, 1e-30)) < 1e-5
+    full.reset()
+    ctx.synchronize()
+    n0, st0 = read_stats(full, MT5_LAMBDAS)
+    assert not n0.any() and not st0.any()
+
+
+def test_sanitize_and_counts(ctx):
+    W, H, spp = 40, 24, 16
+    rng = np.random.default_rng(5)
+    smp = rng.lognormal(0, 1, (spp, 5, H, W)).astype(np.float32)
+    smp[:, 3:] = rng.random((spp, 2, H, W), dtype=np.float32)
+    bad = rng.random((spp, H, W)) < 0.05
+    kinds = rng.integers(0, 4, (spp, H, W))
+    for k, val in enumerate((np.nan, np.inf, -3.0, -np.inf)):
+        m = bad & (kinds == k)
+        smp[:, 1][m] = val
+    counts = rng.integers(0, spp + 1, (H, W)).astype(np.int32)
+    chunk = SampleChunk(W, H, spp)
+    chunk.from_numpy(smp)
+    cbuf = hipmem.DeviceBuffer(W * H * 4)
+    cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+    hipmem.write_plane(cplane, counts[None])
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5)
+    bs.accumulate(chunk.desc(sanitize=True, counts=cplane))
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    n_ref, st_ref = oracle.accumulate(smp, MT5_LAMBDAS, 5, counts=counts, sanitize=True)
+    assert np.array_equal(n, n_ref)
+    assert np.array_equal(st[0], st_ref[0], equal_nan=True)
