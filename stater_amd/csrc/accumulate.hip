@@ -29,8 +29,10 @@
 //    (pow(x,l) - 1)/l, estimator.h:56,72-74, cancels for small l, and for
 //    l = 3 on dark pixels FP32 y = -1/3 + tiny loses the variation entirely).
 //    Central moments, c2X, c2Y and m1 do not depend on the shift; the stored
-//    mean is K + mean(y - K).  One log per sample-channel is shared by all
-//    sets.  The update is the reference recurrence in factored form,
+//    mean is K + mean(y - K); mean(y - K) itself persists in an internal
+//    plane so that a later call continues the recurrence bit for bit.  One
+//    log per sample-channel is shared by all sets; expm1 uses a range-reduced
+//    polynomial and the log the hardware log2.  The update is the reference recurrence in factored form,
 //        m2 += d^2 (1 - 1/n)
 //        m3 += -3 (d/n) m2 + d^3 (1 - 1/n^2)
 //        m4 += -4 (d/n) m3 - 6 (d^2/n^2) m2 + d^4 (1 - 1/n^3)
@@ -52,7 +54,8 @@ struct AccArgs {
     int64_t counts_pitch;
     int32_t *n;
     float *stats;
-    float *lref;  // [3][plane] ln of the reference sample of each pixel-channel
+    float *lref;    // [3][plane] ln of the reference sample of each pixel-channel
+    float *mshift;  // [S-1][3][plane] Box-Cox means relative to the reference point
     int ld;
     size_t plane;
     int W, H;
@@ -94,6 +97,26 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float nf, float x
     st[4] += (x - mean) * xD;
     st[5] += (x - mean) * yD;
     st[6] += sqrtf(d * (d - dN));  // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
+}
+
+// expm1(z) with |relative error| of a few ulp: z = k ln2 + r, |r| <= ln2/2,
+// expm1(z) = 2^k expm1(r) + (2^k - 1), expm1(r) by its Taylor polynomial to
+// r^8/8! (truncation < 6e-9 relative).  z below -104 gives -1.
+__device__ __forceinline__ float expm1_fast(float z) {
+    z = fmaxf(z, -104.f);
+    const float k = __builtin_rintf(z * 1.44269504088896341f);
+    float r = __builtin_fmaf(k, -0.693145751953125f, z);   // ln2 split: hi has 12 trailing zero bits
+    r = __builtin_fmaf(k, -1.4286068203094173e-06f, r);
+    float p = 1.f / 40320.f;
+    p = __builtin_fmaf(p, r, 1.f / 5040.f);
+    p = __builtin_fmaf(p, r, 1.f / 720.f);
+    p = __builtin_fmaf(p, r, 1.f / 120.f);
+    p = __builtin_fmaf(p, r, 1.f / 24.f);
+    p = __builtin_fmaf(p, r, 1.f / 6.f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p * r, r, r);                       // expm1(r)
+    const float two_k = __builtin_ldexpf(1.f, (int)k);
+    return __builtin_fmaf(two_k, p, two_k - 1.f);
 }
 
 // Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
@@ -149,15 +172,16 @@ __device__ __forceinline__ bool keep_sample(float r, float g, float b) {
     return !(nan || (double)y < -1e-5 || __builtin_isinf(y));
 }
 
+// The own channel is read by one wavefront only (streamed, non-temporal);
+// with the guard every wavefront also reads the other two channels, which
+// then stay cacheable for the neighbouring channel wavefronts.
 __device__ __forceinline__ float load_sample(const char *base, int64_t off, int64_t comp, int c, bool sanitize) {
-    float v = __builtin_nontemporal_load((const float *)(base + off + c * comp));
-    if (sanitize) {
-        const float r = __builtin_nontemporal_load((const float *)(base + off));
-        const float g = __builtin_nontemporal_load((const float *)(base + off + comp));
-        const float b = __builtin_nontemporal_load((const float *)(base + off + 2 * comp));
-        if (!keep_sample(r, g, b)) v = 0.f;
-    }
-    return v;
+    if (!sanitize) return __builtin_nontemporal_load((const float *)(base + off + c * comp));
+    const float r = *(const float *)(base + off);
+    const float g = *(const float *)(base + off + comp);
+    const float b = *(const float *)(base + off + 2 * comp);
+    const float v = c == 0 ? r : (c == 1 ? g : b);
+    return keep_sample(r, g, b) ? v : 0.f;
 }
 
 template <int T, int MM>
@@ -186,7 +210,7 @@ __global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
         const size_t lp = (size_t)c * a.plane + pix;
         if (n == 0) {
             const float v0 = load_sample(base, 0, a.comp, c, a.sanitize);
-            L = (v0 > 0.f && __builtin_isfinite(v0)) ? logf(v0) : 0.f;
+            L = (v0 > 0.f && __builtin_isfinite(v0)) ? __builtin_amdgcn_logf(v0) * 0.69314718055994531f : 0.f;
             a.lref[lp] = L;
         } else {
             L = a.lref[lp];
@@ -195,7 +219,7 @@ __global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
         for (int s = 1; s <= T; ++s) {
             K[s] = expm1f(a.lam[s] * L) * a.inv_lam[s];
             Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
-            st[s][0] = n ? st[s][0] - K[s] : 0.f;
+            st[s][0] = n ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
         }
     }
 
@@ -203,22 +227,26 @@ __global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
         const int64_t off = (int64_t)j * a.sstride;
         const float v = load_sample(base, off, a.comp, c, a.sanitize);
         float xD = 0.f, yD = 0.f;
-        if (MM >= 5) {
-            xD = __builtin_nontemporal_load((const float *)(base + off + 3 * a.comp)) - 0.5f;
-            yD = __builtin_nontemporal_load((const float *)(base + off + 4 * a.comp)) - 0.5f;
+        if (MM >= 5) {  // shared by the three channel wavefronts: keep in L2
+            xD = *(const float *)(base + off + 3 * a.comp) - 0.5f;
+            yD = *(const float *)(base + off + 4 * a.comp) - 0.5f;
         }
         ++n;
         const float nf = (float)n;
         meng_exact<MM>(st[0], v, nf, xD, yD);
         if (T > 0) {
             const NFac f = nfac(nf);
-            const float dl = logf(v) - L;
+            // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
+            const float dl = __builtin_amdgcn_logf(v) * 0.69314718055994531f - L;
 #pragma unroll
-            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], Wt[s] * expm1f(a.lam[s] * dl), f, xD, yD);
+            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], Wt[s] * expm1_fast(a.lam[s] * dl), f, xD, yD);
         }
     }
 #pragma unroll
-    for (int s = 1; s <= T; ++s) st[s][0] += K[s];
+    for (int s = 1; s <= T; ++s) {
+        a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[s][0];
+        st[s][0] += K[s];
+    }
 #pragma unroll
     for (int s = 0; s <= T; ++s)
 #pragma unroll
@@ -272,6 +300,7 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
     a.n = bs->n;
     a.stats = bs->stats;
     a.lref = bs->lref;
+    a.mshift = bs->mshift;
     a.ld = bs->ld;
     a.plane = bs->plane;
     a.W = bs->W;

@@ -271,7 +271,8 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     L.nst = mm >= 5 ? 7 : mm;
     L.ld = (W + kRowAlign - 1) / kRowAlign * kRowAlign;
     L.plane = (size_t)L.ld * H;
-    size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3;  // n + moments (all 7 slots) + Box-Cox reference point
+    // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
+    size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
     if (!(flags & STATER_BUFSET_NO_DERIVED)) np += (size_t)L.S * (1 + 6 * 3) + 9 + 1;
     L.n_planes = np;
     return STATER_OK;
@@ -313,6 +314,7 @@ int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, 
     bs->n = (int32_t *)p; p += L.plane;
     bs->stats = p; p += (size_t)L.S * N_STATS * 3 * L.plane;
     bs->lref = p; p += 3 * L.plane;
+    bs->mshift = p; p += (size_t)(L.S - 1) * 3 * L.plane;
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
         bs->jb = p; p += (size_t)L.S * L.plane;
         float **d[6] = {&bs->lnS2, &bs->se2, &bs->m2C, &bs->varD, &bs->varCD, &bs->discr};

@@ -125,32 +125,40 @@ int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
 }
 
 // ======================================================= variance filter
+// LDS ring layout (floats), per slot: single-float rows of HWC columns
+// (ok flag, the padded feature channels), then float2 rows: for every
+// set-channel (L, Z) = (lnS2, z^2 se2) and (V, C) = (s^2, m2C/(n-1)).  HWC is
+// a compile-time row stride so every LDS access is base + immediate offset.
+constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
+
 struct VarArgs {
     const int32_t *n;
-    const float *m2;  // set-0 base of the m2 statistic: stats + ST_M2 offset
+    const float *m2;  // set-0 base of the m2 statistic
     const float *lnS2, *se2, *m2C;
     float *varD, *varCD;
     FeatArgs f;
-    int W, H, ld, r, g0;  // g0: first set of this group
+    int W, H, ld, r, g0;       // g0: first set of this group
     size_t plane, set_stride;  // distance between sets of the statistics block
     float z2, sp_scale;        // sp_scale = ds_factor * log2 e
 };
 
-// Quantities per staged halo pixel: ok, features, then per set/channel
-// L = lnS2, Z = z^2 se2, V = s^2, C = m2C/(n-1).
-template <int G>
-__device__ __forceinline__ void var_stage_row(const VarArgs &a, float *ring, int Q, int HW, int slot, int yq, int x0) {
+__device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
+
+template <int G, bool HASW, int HWC>
+__device__ __forceinline__ void var_stage_row(const VarArgs &a, float *slot, int yq, int x0) {
+    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
+    const int HW = TX + 2 * a.r;
     const int items = HW * (1 + 3 * G);
+    float2 *pairs = (float2 *)(slot + SINGLES * HWC);
     for (int i = threadIdx.x; i < items; i += blockDim.x) {
         const int kind = i / HW, col = i - kind * HW;
         const int xq = x0 - a.r + col;
         const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
-        float *row = ring + (size_t)slot * Q * HW + col;
         if (kind == 0) {
-            float ok = 0.f;
-            if (in) ok = a.n[(size_t)yq * a.ld + xq] >= 4 ? 1.f : 0.f;
-            row[0] = ok;
-            for (int k = 0; k < a.f.nf; ++k) row[(1 + k) * HW] = in ? feat_at(a.f, k, xq, yq) : 0.f;
+            slot[col] = in && a.n[(size_t)yq * a.ld + xq] >= 4 ? 1.f : 0.f;
+            if (HASW)
+#pragma unroll
+                for (int k = 0; k < kNF; ++k) slot[(1 + k) * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
         } else {
             const int sc = kind - 1;  // g*3 + c within the group
             const int g = sc / 3, c = sc - g * 3;
@@ -162,38 +170,38 @@ __device__ __forceinline__ void var_stage_row(const VarArgs &a, float *ring, int
                 if (n >= 4) { L = a.lnS2[o]; Z = a.z2 * a.se2[o]; }
                 if (n >= 2) {
                     const float inv = 1.f / (float)(n - 1);
-                    V = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
-                    C = a.m2C[o] * inv;
+                    V = finite_or_max(a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv);
+                    C = finite_or_max(a.m2C[o] * inv);
                 }
             }
-            const int base = 1 + a.f.nf + sc * 4;
-            row[base * HW] = L;
-            row[(base + 1) * HW] = Z;
-            row[(base + 2) * HW] = V;
-            row[(base + 3) * HW] = C;
+            pairs[(sc * 2) * HWC + col] = make_float2(L, Z);
+            pairs[(sc * 2 + 1) * HWC + col] = make_float2(V, C);
         }
     }
 }
 
-template <int G, bool HASW>
+template <int G, bool HASW, int HWC>
 __global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
     extern __shared__ float ring[];
-    const int HW = TX + 2 * a.r;
-    const int Q = 1 + a.f.nf + 12 * G;
+    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
+    constexpr int SLOT = (SINGLES + 12 * G) * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < a.W && y < a.H;
 
-    // own pixel
     float Lp[G][3], Zp[G][3], den[G][3], num[G][3], numC[G][3];
-    float gp[kMaxFeat];
+    float gp[kNF];
     float okp = 0.f;
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
     if (active) {
         const size_t pix = (size_t)y * a.ld + x;
         const int n = a.n[pix];
         okp = n >= 4 ? 1.f : 0.f;
-        for (int k = 0; k < a.f.nf; ++k) gp[k] = feat_at(a.f, k, x, y);
+        if (HASW)
+#pragma unroll
+            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
 #pragma unroll
         for (int g = 0; g < G; ++g)
 #pragma unroll
@@ -213,41 +221,46 @@ __global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
             }
     }
 
-    for (int t = 0; t < TY; ++t) var_stage_row<G>(a, ring, Q, HW, t, y0 - a.r + t, x0);
+    for (int t = 0; t < TY; ++t) var_stage_row<G, HASW, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
     __syncthreads();
     for (int k = 0; k <= 2 * a.r; ++k) {
         const int dy = k - a.r;
-        const float *row = ring + (size_t)((ty + k) % TY) * Q * HW;
+        const float *slot = ring + ((ty + k) % TY) * SLOT;
         if (active && okp > 0.f) {
+            const float spy = a.sp_scale * (float)(dy * dy);
+#pragma unroll 2
             for (int dx = -a.r; dx <= a.r; ++dx) {
                 if (dx == 0 && dy == 0) continue;
-                const int col = tx + a.r + dx;
-                float w = row[col];
+                const float *q = slot + tx + a.r + dx;
+                float w = q[0];
                 if (HASW) {
-                    float lw = a.sp_scale * (float)(dx * dx + dy * dy);
-                    for (int f = 0; f < a.f.nf; ++f) {
-                        const float d = gp[f] - row[(1 + f) * HW + col];
+                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+                    for (int f = 0; f < kNF; ++f) {
+                        const float d = gp[f] - q[(1 + f) * HWC];
                         lw = __builtin_fmaf(-d, d, lw);
                     }
-                    w *= exp2f(lw);
+                    w *= __builtin_amdgcn_exp2f(lw);
                 }
-                const float *q = row + (1 + a.f.nf) * HW + col;
+                const float2 *pq = (const float2 *)(slot + SINGLES * HWC) + tx + a.r + dx;
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const float *qq = q + (g * 3 + c) * 4 * HW;
-                        const float dl = Lp[g][c] - qq[0];
-                        const bool acc = !(dl * dl > Zp[g][c] + qq[HW]);
-                        den[g][c] += acc ? w : 0.f;
-                        num[g][c] += acc ? w * qq[2 * HW] : 0.f;
-                        numC[g][c] += acc ? w * qq[3 * HW] : 0.f;
+                        const float2 lz = pq[((g * 3 + c) * 2) * HWC];
+                        const float2 vc = pq[((g * 3 + c) * 2 + 1) * HWC];
+                        const float dl = Lp[g][c] - lz.x;
+                        // NaN/inf operands abstain (accept)
+                        const float wa = (dl * dl > Zp[g][c] + lz.y) ? 0.f : w;
+                        den[g][c] += wa;
+                        num[g][c] = __builtin_fmaf(wa, vc.x, num[g][c]);
+                        numC[g][c] = __builtin_fmaf(wa, vc.y, numC[g][c]);
                     }
             }
         }
         if (k == 2 * a.r) break;
         __syncthreads();
-        var_stage_row<G>(a, ring, Q, HW, k % TY, y0 - a.r + k + TY, x0);
+        var_stage_row<G, HASW, HWC>(a, ring + (k % TY) * SLOT, y0 - a.r + k + TY, x0);
         __syncthreads();
     }
     if (!active) return;
@@ -262,21 +275,21 @@ __global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
         }
 }
 
-template <int G>
-static int var_launch(VarArgs &a, bool hasw, dim3 grid, hipStream_t st) {
-    const int Q = 1 + a.f.nf + 12 * G;
-    const size_t lds = (size_t)TY * Q * (TX + 2 * a.r) * 4;
-    if (hasw) {
-        STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_var_filter<G, true>), grid, dim3(256), lds, st, a);
-    } else {
-        STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_var_filter<G, false>), grid, dim3(256), lds, st, a);
-    }
+template <int G, bool HASW, int HWC>
+static int var_launch3(const VarArgs &a, dim3 grid, hipStream_t st) {
+    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
+    const size_t lds = (size_t)TY * (SINGLES + 12 * G) * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC>), grid, dim3(256), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
+}
+
+template <int G>
+static int var_launch(const VarArgs &a, bool hasw, dim3 grid, hipStream_t st) {
+    if (a.r <= 10) return hasw ? var_launch3<G, true, TX + 20>(a, grid, st) : var_launch3<G, false, TX + 20>(a, grid, st);
+    return hasw ? var_launch3<G, true, TX + 40>(a, grid, st) : var_launch3<G, false, TX + 40>(a, grid, st);
 }
 
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
@@ -308,6 +321,8 @@ int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const s
 }
 
 // ========================================================== Welch filter
+// LDS ring layout (floats), per slot: single rows (ok, n-1, padded features,
+// film[3], plain: s^2[3]) then one float2 row (M, V) per set-channel.
 struct WelchArgs {
     const int32_t *n;
     const float *mean;     // statistics block base (set s mean = mean + s*set_stride)
@@ -347,10 +362,7 @@ __device__ __forceinline__ float tcrit_eval(const TCritPoly &tp, float u) {
     return acc;
 }
 
-// Staged quantities per halo pixel: ok (n >= 2), nm1 = n-1, features,
-// film[3], then per set/channel M (mean; NaN = abstain) and V (v = var/n),
-// then, for the plain method, s^2 [3].
-template <int S, bool PLAIN>
+// Per-pixel test operands: M (mean; NaN = abstain) and V = var/n (discr).
 __device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, int s, int c, float &M, float &V) {
     const size_t pix = (size_t)yq * a.ld + xq;
     const int n = a.n[pix];
@@ -363,57 +375,66 @@ __device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, i
     if (!(__builtin_isfinite(M) && __builtin_isfinite(V))) { M = NAN; V = 0.f; }
 }
 
-template <int S, bool PLAIN>
-__device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *ring, int Q, int HW, int slot, int yq, int x0) {
+template <int S, bool HASW, bool PLAIN>
+struct WelchLayout {
+    static constexpr int FEAT = 2;                        // after ok, n-1
+    static constexpr int FILM = 2 + (HASW ? kNF : 0);
+    static constexpr int S2 = FILM + 3;                   // plain: s^2 [3]
+    static constexpr int SINGLES = FILM + 3 + (PLAIN ? 3 : 0);
+    static constexpr int FLOATS = SINGLES + 6 * S;        // per column
+};
+
+template <int S, bool HASW, bool PLAIN, int HWC>
+__device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot, int yq, int x0) {
+    using LY = WelchLayout<S, HASW, PLAIN>;
+    const int HW = TX + 2 * a.r;
     const int items = HW * (1 + 3 * S);
+    float2 *pairs = (float2 *)(slot + LY::SINGLES * HWC);
     for (int i = threadIdx.x; i < items; i += blockDim.x) {
         const int kind = i / HW, col = i - kind * HW;
         const int xq = x0 - a.r + col;
         const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
-        float *row = ring + (size_t)slot * Q * HW + col;
         if (kind == 0) {
             float ok = 0.f, nm1 = 1.f;
+            int n = 0;
             if (in) {
-                const int n = a.n[(size_t)yq * a.ld + xq];
+                n = a.n[(size_t)yq * a.ld + xq];
                 ok = n >= 2 ? 1.f : 0.f;
                 nm1 = n >= 2 ? (float)(n - 1) : 1.f;
             }
-            row[0] = ok;
-            row[HW] = nm1;
-            int q = 2;
-            for (int k = 0; k < a.f.nf; ++k, ++q) row[q * HW] = in ? feat_at(a.f, k, xq, yq) : 0.f;
-            for (int c = 0; c < 3; ++c, ++q)
-                row[q * HW] = in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch +
-                                                    (int64_t)xq * 4 + c * a.film_cs)
-                                 : 0.f;
-            if (PLAIN) {
-                const int base = 2 + a.f.nf + 3 + 6 * S;
+            slot[col] = ok;
+            slot[HWC + col] = nm1;
+            if (HASW)
+#pragma unroll
+                for (int k = 0; k < kNF; ++k)
+                    slot[(LY::FEAT + k) * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                slot[(LY::FILM + c) * HWC + col] =
+                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 +
+                                          c * a.film_cs)
+                       : 0.f;
+            if (PLAIN)
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float s2 = 0.f;
-                    if (in) {
-                        const size_t pix = (size_t)yq * a.ld + xq;
-                        const int n = a.n[pix];
-                        if (n >= 2) s2 = a.var[c * a.plane + pix] / (float)(n - 1);
-                    }
-                    row[(base + c) * HW] = s2;
+                    if (in && n >= 2) s2 = a.var[c * a.plane + (size_t)yq * a.ld + xq] / (float)(n - 1);
+                    slot[(LY::S2 + c) * HWC + col] = finite_or_max(s2);
                 }
-            }
         } else {
             const int sc = kind - 1, s = sc / 3, c = sc - s * 3;
             float M = 0.f, V = 0.f;
-            if (in) welch_load<S, PLAIN>(a, xq, yq, s, c, M, V);
-            const int base = 2 + a.f.nf + 3 + sc * 2;
-            row[base * HW] = M;
-            row[(base + 1) * HW] = V;
+            if (in) welch_load(a, xq, yq, s, c, M, V);
+            pairs[sc * HWC + col] = make_float2(M, V);
         }
     }
 }
 
-template <int S, bool HASW, bool PLAIN>
+template <int S, bool HASW, bool PLAIN, int HWC>
 __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
+    using LY = WelchLayout<S, HASW, PLAIN>;
     extern __shared__ float ring[];
-    const int HW = TX + 2 * a.r;
-    const int Q = 2 + a.f.nf + 3 + 6 * S + (PLAIN ? 3 : 0);
+    constexpr int SLOT = LY::FLOATS * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int x = x0 + tx, y = y0 + ty;
@@ -421,24 +442,28 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
     const int win = 2 * a.r + 1;
 
     float Mp[S][3], Vp[S][3];
-    float gp[kMaxFeat];
+    float gp[kNF];
     float okp = 0.f, nm1p = 1.f, rp = 0.f;
     float den[3], num[3], numV[3];
     bool use[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) use[s] = true;
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
     if (active) {
         const size_t pix = (size_t)y * a.ld + x;
         const int n = a.n[pix];
         okp = n >= 2 ? 1.f : 0.f;
         nm1p = n >= 2 ? (float)(n - 1) : 1.f;
         rp = __builtin_amdgcn_rcpf(nm1p);
-        for (int k = 0; k < a.f.nf; ++k) gp[k] = feat_at(a.f, k, x, y);
+        if (HASW)
+#pragma unroll
+            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                welch_load<S, PLAIN>(a, x, y, s, c, Mp[s][c], Vp[s][c]);
+                welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
                 a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
             }
         if (a.mode3) {
@@ -464,54 +489,56 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
         }
     }
 
-    for (int t = 0; t < TY; ++t) welch_stage_row<S, PLAIN>(a, ring, Q, HW, t, y0 - a.r + t, x0);
+    for (int t = 0; t < TY; ++t) welch_stage_row<S, HASW, PLAIN, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
     __syncthreads();
     for (int k = 0; k <= 2 * a.r; ++k) {
         const int dy = k - a.r;
-        const float *row = ring + (size_t)((ty + k) % TY) * Q * HW;
+        const float *slot = ring + ((ty + k) % TY) * SLOT;
         if (active) {
+            const float spy = a.sp_scale * (float)(dy * dy);
             for (int dx = -a.r; dx <= a.r; ++dx) {
                 if (dx == 0 && dy == 0) continue;
-                const int col = tx + a.r + dx;
-                const float okq = row[col];
-                const float nm1q = row[HW + col];
+                const float *q = slot + tx + a.r + dx;
+                const float okq = q[0];
+                const float nm1q = q[HWC];
                 const float rq = __builtin_amdgcn_rcpf(nm1q);
-                float w = okq * okp;
+                const float okpq = okq * okp;
+                float w = okpq;
                 if (HASW) {
-                    float lw = a.sp_scale * (float)(dx * dx + dy * dy);
-                    for (int f = 0; f < a.f.nf; ++f) {
-                        const float d = gp[f] - row[(2 + f) * HW + col];
+                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+                    for (int f = 0; f < kNF; ++f) {
+                        const float d = gp[f] - q[(LY::FEAT + f) * HWC];
                         lw = __builtin_fmaf(-d, d, lw);
                     }
-                    w *= exp2f(lw);
+                    w *= __builtin_amdgcn_exp2f(lw);
                 }
-                const float *fq = row + (2 + a.f.nf) * HW + col;
-                const float *mv = fq + 3 * HW;
-                bool acc[3] = {okq * okp > 0.f, okq * okp > 0.f, okq * okp > 0.f};
+                const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx;
+                bool acc[3] = {okpq > 0.f, okpq > 0.f, okpq > 0.f};
 #pragma unroll
                 for (int s = 0; s < S; ++s)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const float mq = mv[(s * 3 + c) * 2 * HW];
-                        const float vq = mv[((s * 3 + c) * 2 + 1) * HW];
+                        const float2 mv = pq[(s * 3 + c) * HWC];
                         const float vp = Vp[s][c];
-                        const float V = vp + vq;
-                        const float dm = Mp[s][c] - mq;
+                        const float V = vp + mv.y;
+                        const float dm = Mp[s][c] - mv.x;
                         const float iV = __builtin_amdgcn_rcpf(V);
-                        const float ap = vp * iV, aq = vq * iV;
+                        const float ap = vp * iV, aq = mv.y * iV;
                         const float u = __builtin_fmaf(ap * ap, rp, aq * aq * rq);
                         const float t2 = (dm * iV) * dm;
                         const float crit = tcrit_eval(a.tp, u);
                         bool pass = !(t2 > crit || t2 == INFINITY);
                         const bool near = fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax;
-                        if (near && acc[c] && use[s]) pass = welch_exact(Mp[s][c], vp, nm1p, mq, vq, nm1q, a.alpha);
+                        if (near && acc[c] && use[s]) pass = welch_exact(Mp[s][c], vp, nm1p, mv.x, mv.y, nm1q, a.alpha);
                         acc[c] = acc[c] && (pass || !use[s]);
                     }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    den[c] += acc[c] ? w : 0.f;
-                    num[c] += acc[c] ? w * fq[c * HW] : 0.f;
-                    if (PLAIN) numV[c] += acc[c] ? w * fq[(3 + 6 * S + c) * HW] : 0.f;
+                    const float wa = acc[c] ? w : 0.f;
+                    den[c] += wa;
+                    num[c] = __builtin_fmaf(wa, q[(LY::FILM + c) * HWC], num[c]);
+                    if (PLAIN) numV[c] = __builtin_fmaf(wa, q[(LY::S2 + c) * HWC], numV[c]);
                 }
                 if (a.dec) {
                     const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
@@ -523,7 +550,7 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
         }
         if (k == 2 * a.r) break;
         __syncthreads();
-        welch_stage_row<S, PLAIN>(a, ring, Q, HW, k % TY, y0 - a.r + k + TY, x0);
+        welch_stage_row<S, HASW, PLAIN, HWC>(a, ring + (k % TY) * SLOT, y0 - a.r + k + TY, x0);
         __syncthreads();
     }
     if (!active) return;
@@ -534,20 +561,24 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
     }
 }
 
-template <int S, bool HASW, bool PLAIN>
-static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
-    const int Q = 2 + a.f.nf + 3 + 6 * S + (PLAIN ? 3 : 0);
-    const size_t lds = (size_t)TY * Q * (TX + 2 * a.r) * 4;
-    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN>,
+template <int S, bool HASW, bool PLAIN, int HWC>
+static int welch_launch4(const WelchArgs &a, dim3 grid, hipStream_t st) {
+    const size_t lds = (size_t)TY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, HWC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN>), grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, HWC>), grid, dim3(256), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
+template <int S, bool HASW, bool PLAIN>
+static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
+    return a.r <= 10 ? welch_launch4<S, HASW, PLAIN, TX + 20>(a, grid, st)
+                     : welch_launch4<S, HASW, PLAIN, TX + 40>(a, grid, st);
+}
+
 template <int S>
-static int welch_launch2(const WelchArgs &a, bool hasw, bool plain, dim3 grid, hipStream_t st) {
-    if (plain) return welch_launch3<S, false, true>(a, grid, st);
+static int welch_launch2(const WelchArgs &a, bool hasw, dim3 grid, hipStream_t st) {
     return hasw ? welch_launch3<S, true, false>(a, grid, st) : welch_launch3<S, false, false>(a, grid, st);
 }
 
@@ -566,17 +597,16 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
         a.var_is_m2 = 0;
     }
     if (film) {
+        if (film->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film must be planar");
         a.film = (const float *)film->ptr;
         a.film_pitch = film->pitch_bytes;
-        a.film_cs = film->layout == STATER_PLANAR ? film->channel_stride_bytes : 4;
-        if (film->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film must be planar");
+        a.film_cs = film->channel_stride_bytes;
     } else {
         a.film = bs->stat(0, ST_MEAN, 0);
         a.film_pitch = (int64_t)bs->ld * 4;
         a.film_cs = (int64_t)bs->plane * 4;
     }
-    float *outp = plain ? bs->meanD : bs->filmD;
-    a.out = outp;
+    a.out = plain ? bs->meanD : bs->filmD;
     a.out_pitch = (int64_t)bs->ld * 4;
     a.out_cs = (int64_t)bs->plane * 4;
     if (film_out) {
@@ -600,13 +630,14 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
     dim3 grid((bs->W + TX - 1) / TX, (bs->H + TY - 1) / TY);
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
+    if (plain) return welch_launch3<1, false, true>(a, grid, st);
     switch (S) {
-        case 1: return welch_launch2<1>(a, hasw, plain, grid, st);
-        case 2: return welch_launch2<2>(a, hasw, plain, grid, st);
-        case 3: return welch_launch2<3>(a, hasw, plain, grid, st);
-        case 4: return welch_launch2<4>(a, hasw, plain, grid, st);
-        case 5: return welch_launch2<5>(a, hasw, plain, grid, st);
-        default: return welch_launch2<6>(a, hasw, plain, grid, st);
+        case 1: return welch_launch2<1>(a, hasw, grid, st);
+        case 2: return welch_launch2<2>(a, hasw, grid, st);
+        case 3: return welch_launch2<3>(a, hasw, grid, st);
+        case 4: return welch_launch2<4>(a, hasw, grid, st);
+        case 5: return welch_launch2<5>(a, hasw, grid, st);
+        default: return welch_launch2<6>(a, hasw, grid, st);
     }
 }
 

@@ -43,6 +43,7 @@ struct stater_bufset {
     int32_t *n = nullptr;
     float *stats = nullptr;  // [S][7][3][plane]
     float *lref = nullptr;   // [3][plane] Box-Cox reference point ln x1 (sets 1..)
+    float *mshift = nullptr; // [S-1][3][plane] Box-Cox means relative to the reference point
     float *jb = nullptr;     // [S][plane]
     float *lnS2 = nullptr, *se2 = nullptr, *m2C = nullptr, *varD = nullptr, *varCD = nullptr,
           *discr = nullptr;  // [S][3][plane]

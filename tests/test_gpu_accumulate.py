@@ -103,13 +103,8 @@ def test_accumulate_in_chunks_continues_sequence(ctx):
     ctx.synchronize()
     a = read_stats(full, MT5_LAMBDAS)
     b = read_stats(parts, MT5_LAMBDAS)
-    assert np.array_equal(a[0], b[0])
-    # untransformed statistics continue bit for bit; the Box-Cox means are
-    # stored unshifted, so resuming re-derives the shifted mean (one rounding)
-    assert np.array_equal(a[1][0], b[1][0])
-    for s in range(1, a[1].shape[0]):
-        scale = _scales(a[1][s].astype(np.float64), a[0])
-        assert np.max(np.abs(a[1][s].astype(np.float64) - b[1][s]) / np.maximum(scale, 1e-30)) < 1e-5
+    # a second call continues the recurrence bit for bit, for every set
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     full.reset()
     ctx.synchronize()
     n0, st0 = read_stats(full, MT5_LAMBDAS)
