@@ -82,7 +82,7 @@ double t2_crit_host(double nu, double alpha) {
 TCritPoly make_tcrit_poly(double alpha) {
     TCritPoly tp{};
     const int D = kTCritDeg;
-    const double umax = 0.125;
+    const double umax = 0.0625;
     // Chebyshev interpolation of g(u) = t2crit(1/u) on s = 2u/umax - 1.
     double gnode[D + 1], cheb[D + 1];
     for (int k = 0; k <= D; ++k) {

@@ -362,6 +362,19 @@ __device__ __forceinline__ float tcrit_eval(const TCritPoly &tp, float u) {
     return acc;
 }
 
+// The FP32 Welch statistic of a pair: t2 = dm^2/V, u = 1/nu evaluated in the
+// scale-free form (vp/V)^2/(np-1) + (vq/V)^2/(nq-1), and the critical t2(u).
+__device__ __forceinline__ void welch_fast(const TCritPoly &tp, float mp, float vp, float rp, float mq, float vq,
+                                           float rq, float &t2, float &crit, float &u) {
+    const float V = vp + vq;
+    const float dm = mp - mq;
+    const float iV = __builtin_amdgcn_rcpf(V);
+    const float ap = vp * iV, aq = vq * iV;
+    u = __builtin_fmaf(ap * ap, rp, aq * aq * rq);
+    t2 = (dm * iV) * dm;
+    crit = tcrit_eval(tp, u);
+}
+
 // Per-pixel test operands: M (mean; NaN = abstain) and V = var/n (discr).
 __device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, int s, int c, float &M, float &V) {
     const size_t pix = (size_t)yq * a.ld + xq;
@@ -514,25 +527,45 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
                     w *= __builtin_amdgcn_exp2f(lw);
                 }
                 const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx;
-                bool acc[3] = {okpq > 0.f, okpq > 0.f, okpq > 0.f};
+                // All fast tests first, branch-free, so the LDS reads of every
+                // set are issued together and every comparison mask dies at
+                // once; if any test of the pair lies in the band around the
+                // critical value, the pair is re-evaluated with exact tails.
+                float2 mv[S][3];
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) mv[s][c] = pq[(s * 3 + c) * HWC];
+                float accf[3] = {okpq, okpq, okpq};
+                float nearf = 0.f;
 #pragma unroll
                 for (int s = 0; s < S; ++s)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const float2 mv = pq[(s * 3 + c) * HWC];
-                        const float vp = Vp[s][c];
-                        const float V = vp + mv.y;
-                        const float dm = Mp[s][c] - mv.x;
-                        const float iV = __builtin_amdgcn_rcpf(V);
-                        const float ap = vp * iV, aq = mv.y * iV;
-                        const float u = __builtin_fmaf(ap * ap, rp, aq * aq * rq);
-                        const float t2 = (dm * iV) * dm;
-                        const float crit = tcrit_eval(a.tp, u);
-                        bool pass = !(t2 > crit || t2 == INFINITY);
-                        const bool near = fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax;
-                        if (near && acc[c] && use[s]) pass = welch_exact(Mp[s][c], vp, nm1p, mv.x, mv.y, nm1q, a.alpha);
-                        acc[c] = acc[c] && (pass || !use[s]);
+                        float t2, crit, u;
+                        welch_fast(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, t2, crit, u);
+                        accf[c] = ((t2 > crit || t2 == INFINITY) && use[s]) ? 0.f : accf[c];
+                        nearf = ((fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax) && use[s]) ? 1.f : nearf;
                     }
+                if (nearf > 0.f && okpq > 0.f) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) accf[c] = 1.f;
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            if (!use[s]) continue;
+                            float t2, crit, u;
+                            welch_fast(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, t2, crit, u);
+                            bool pass = !(t2 > crit || t2 == INFINITY);
+                            if (fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax)
+                                pass = welch_exact(Mp[s][c], Vp[s][c], nm1p, mv[s][c].x, mv[s][c].y, nm1q, a.alpha);
+                            if (!pass) accf[c] = 0.f;
+                        }
+                }
+                bool acc[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] = accf[c] > 0.f;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const float wa = acc[c] ? w : 0.f;

@@ -52,7 +52,7 @@ __host__ __device__ inline double t2_tail(double t2, double nu) {
 }
 
 // Number of polynomial coefficients of the critical-value fit.
-constexpr int kTCritDeg = 7;
+constexpr int kTCritDeg = 5;
 
 // Critical t^2 of the two-sided level-alpha test as a polynomial in
 // s = u*scale + shift (u = 1/nu in [0, umax]), valid for nu >= 1/umax.
@@ -64,7 +64,8 @@ struct TCritPoly {
     double alpha;
 };
 
-// Host: fit the polynomial for a level (Chebyshev interpolation on umax = 1/8).
+// Host: fit the polynomial for a level (Chebyshev interpolation on umax = 1/16,
+// i.e. nu >= 16; smaller nu takes the exact tail).
 TCritPoly make_tcrit_poly(double alpha);
 double t2_crit_host(double nu, double alpha);
 
