@@ -144,7 +144,8 @@ int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas,
  * the plane starts at data + j*sample_stride_bytes + k*comp_stride_bytes, with
  * rows pitch_bytes apart.  (u,v) is the in-pixel position in [0,1)^2
  * (statpath.cpp:339).  counts (optional, i32 plane) limits pixel p to its
- * first counts[p] samples of the chunk. */
+ * first clamp(counts[p] - count_offset, 0, spp) samples of the chunk, so a
+ * per-pixel target (e.g. targetspp) can be consumed in passes of spp. */
 typedef struct {
     const float   *data;
     int64_t        pitch_bytes;
@@ -154,6 +155,8 @@ typedef struct {
     int32_t        sanitize;   /* NaN / negative / inf guard of statpath.cpp:313-332 */
     const int32_t *counts;
     int64_t        counts_pitch_bytes;
+    int32_t        count_offset;
+    int32_t        reserved;
 } stater_samples;
 
 /* Sequential update in the reference's exact FP32 operation order for the
@@ -248,12 +251,15 @@ int stater_scene_init(stater_scene *s, int model, uint32_t seed, uint32_t image_
 int stater_synth_features(stater_ctx *ctx, const stater_scene *s,
                           const stater_plane *albedo, const stater_plane *normal,
                           const stater_plane *depth, void *stream);
-/* Writes samples first_sample .. first_sample+spp-1 of every pixel into a
- * chunk laid out as stater_samples describes.  albedo may be NULL unless the
- * scene modulates by albedo. */
+/* Writes samples first_sample + first[p] + j (j = 0..count-1) of every pixel
+ * into a chunk laid out as stater_samples describes; count is spp, or the
+ * clamped per-pixel count of out->counts.  first (optional i32 plane, e.g. a
+ * set's "n" view) continues each pixel's sample sequence.  albedo may be NULL
+ * unless the scene modulates by albedo. */
 int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
                          const stater_plane *albedo, uint32_t first_sample,
-                         const stater_samples *out, void *stream);
+                         const stater_plane *first, const stater_samples *out,
+                         void *stream);
 
 /* ---- testing ------------------------------------------------------------
  * Runs the Welch mean filter (plain != 0: the plain method) and also writes

@@ -40,7 +40,8 @@ class Plane(C.Structure):
 class Samples(C.Structure):
     _fields_ = [("data", C.c_void_p), ("pitch_bytes", C.c_int64), ("comp_stride_bytes", C.c_int64),
                 ("sample_stride_bytes", C.c_int64), ("spp", C.c_int32), ("sanitize", C.c_int32),
-                ("counts", C.c_void_p), ("counts_pitch_bytes", C.c_int64)]
+                ("counts", C.c_void_p), ("counts_pitch_bytes", C.c_int64), ("count_offset", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class DenoiseParams(C.Structure):
@@ -97,7 +98,7 @@ SIGNATURES = [
     ("stater_scene_init", C.c_int, [C.POINTER(Scene), C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int]),
     ("stater_synth_features", C.c_int, [_P, C.POINTER(Scene), C.POINTER(Plane), C.POINTER(Plane),
                                         C.POINTER(Plane), _P]),
-    ("stater_synth_samples", C.c_int, [_P, C.POINTER(Scene), C.POINTER(Plane), C.c_uint32,
+    ("stater_synth_samples", C.c_int, [_P, C.POINTER(Scene), C.POINTER(Plane), C.c_uint32, C.POINTER(Plane),
                                        C.POINTER(Samples), _P]),
     ("stater_debug_film_decisions", C.c_int, [_P, C.POINTER(DenoiseParams), C.POINTER(GBuffers),
                                               C.POINTER(Plane), C.c_int, _P, _P]),

@@ -52,6 +52,7 @@ struct AccArgs {
     int sanitize;
     const int32_t *counts;
     int64_t counts_pitch;
+    int count_offset;
     int32_t *n;
     float *stats;
     float *lref;    // [3][plane] ln of the reference sample of each pixel-channel
@@ -193,7 +194,9 @@ __global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
     if (x >= a.W) return;
     const size_t pix = (size_t)y * a.ld + x;
     int cnt = a.spp;
-    if (a.counts) cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4));
+    if (a.counts)
+        cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
+                           a.count_offset);
     if (cnt <= 0) return;
     uint32_t n = (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
@@ -254,6 +257,122 @@ __global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
     if (c == 0) a.n[pix] = (int32_t)n;
 }
 
+// FP64 merge of two partial moment sets (Chan, Golub & LeVeque 1979; Pebay
+// 2008): a = (na, mean, m2, m3, m4) absorbs b.
+__device__ __forceinline__ void merge64(double na, double *a, double nb, const double *b, int mm) {
+    const double n = na + nb;
+    if (nb == 0.0) return;
+    if (na == 0.0) { for (int i = 0; i < 4; ++i) a[i] = b[i]; return; }
+    const double d = b[0] - a[0], d2 = d * d;
+    const double mean = a[0] + d * nb / n;
+    double m2 = 0, m3 = 0, m4 = 0;
+    if (mm >= 2) m2 = a[1] + b[1] + d2 * na * nb / n;
+    if (mm >= 3) m3 = a[2] + b[2] + d * d2 * na * nb * (na - nb) / (n * n) + 3.0 * d * (na * b[1] - nb * a[1]) / n;
+    if (mm >= 4)
+        m4 = a[3] + b[3] + d2 * d2 * na * nb * (na * na - na * nb + nb * nb) / (n * n * n) +
+             6.0 * d2 * (na * na * b[1] + nb * nb * a[1]) / (n * n) + 4.0 * d * (na * b[2] - nb * a[2]) / n;
+    a[0] = mean; a[1] = m2; a[2] = m3; a[3] = m4;
+}
+
+// STATER_ACC_MERGE64: the call's samples build FP32 partial moments from
+// n = 0 (factored recurrence, every set), which are merged into the resident
+// moments in FP64 and rounded once; c2X, c2Y and m1, which depend on the
+// running mean of the whole sequence, continue sequentially on a running
+// mean carried from the resident state.
+template <int T, int MM>
+__global__ void __launch_bounds__(192) k_accumulate_merge(AccArgs a) {
+    constexpr int NS = nstats(MM);
+    constexpr int NP = MM >= 4 ? 4 : MM;  // partial moments kept
+    const int c = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y;
+    if (x >= a.W) return;
+    const size_t pix = (size_t)y * a.ld + x;
+    int cnt = a.spp;
+    if (a.counts)
+        cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
+                           a.count_offset);
+    if (cnt <= 0) return;
+    const uint32_t n0 = (uint32_t)a.n[pix];
+    const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
+
+    float st[T + 1][NS];
+#pragma unroll
+    for (int s = 0; s <= T; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
+    float L = 0.f, K[T + 1], Wt[T + 1];
+    if (T > 0) {
+        const size_t lp = (size_t)c * a.plane + pix;
+        if (n0 == 0) {
+            const float v0 = load_sample(base, 0, a.comp, c, a.sanitize);
+            L = (v0 > 0.f && __builtin_isfinite(v0)) ? __builtin_amdgcn_logf(v0) * 0.69314718055994531f : 0.f;
+            a.lref[lp] = L;
+        } else {
+            L = a.lref[lp];
+        }
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            K[s] = expm1f(a.lam[s] * L) * a.inv_lam[s];
+            Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
+            st[s][0] = n0 ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
+        }
+    }
+    float pm[T + 1][NP];
+    float mg[T + 1];  // running mean of the whole sequence (c2X, c2Y, m1)
+#pragma unroll
+    for (int s = 0; s <= T; ++s) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) pm[s][k] = 0.f;
+        mg[s] = st[s][0];
+    }
+
+    for (int j = 0; j < cnt; ++j) {
+        const int64_t off = (int64_t)j * a.sstride;
+        const float v = load_sample(base, off, a.comp, c, a.sanitize);
+        float xD = 0.f, yD = 0.f;
+        if (MM >= 5) {
+            xD = *(const float *)(base + off + 3 * a.comp) - 0.5f;
+            yD = *(const float *)(base + off + 4 * a.comp) - 0.5f;
+        }
+        const NFac fr = nfac((float)(j + 1));
+        NFac fg;
+        if (MM >= 5) fg = nfac((float)(n0 + j + 1));
+        const float dl = T > 0 ? __builtin_amdgcn_logf(v) * 0.69314718055994531f - L : 0.f;
+#pragma unroll
+        for (int s = 0; s <= T; ++s) {
+            const float t = s == 0 ? v : Wt[s] * expm1_fast(a.lam[s] * dl);
+            meng_factored<NP>(pm[s], t, fr, 0.f, 0.f);
+            if (MM >= 5) {
+                const float dg = t - mg[s];
+                mg[s] = __builtin_fmaf(dg, fg.b, mg[s]);
+                const float r = t - mg[s];
+                st[s][4] = __builtin_fmaf(r, xD, st[s][4]);
+                st[s][5] = __builtin_fmaf(r, yD, st[s][5]);
+                st[s][6] = __builtin_fmaf(fabsf(dg), fg.sa1, st[s][6]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s <= T; ++s) {
+        double A[4] = {st[s][0], NS > 1 ? st[s][1] : 0.0, NS > 2 ? st[s][2] : 0.0, NS > 3 ? st[s][3] : 0.0};
+        const double B[4] = {pm[s][0], NP > 1 ? pm[s][1] : 0.0, NP > 2 ? pm[s][2] : 0.0, NP > 3 ? pm[s][3] : 0.0};
+        merge64((double)n0, A, (double)cnt, B, MM);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) st[s][k] = (float)A[k];
+    }
+#pragma unroll
+    for (int s = 1; s <= T; ++s) {
+        a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[s][0];
+        st[s][0] += K[s];
+    }
+#pragma unroll
+    for (int s = 0; s <= T; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
+    if (c == 0) a.n[pix] = (int32_t)(n0 + cnt);
+}
+
 __global__ void k_zero(uint32_t *p, size_t words) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -268,23 +387,23 @@ int launch_zero(void *p, size_t bytes, hipStream_t st) {
 }
 
 template <int T, int MM>
-static void launch_tm(const AccArgs &a, dim3 grid, hipStream_t st) {
-    hipLaunchKernelGGL((k_accumulate<T, MM>), grid, dim3(192), 0, st, a);
+static void launch_tm(const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
+    if (merge) hipLaunchKernelGGL((k_accumulate_merge<T, MM>), grid, dim3(192), 0, st, a);
+    else hipLaunchKernelGGL((k_accumulate<T, MM>), grid, dim3(192), 0, st, a);
 }
 
 template <int T>
-static void launch_t(int mm, const AccArgs &a, dim3 grid, hipStream_t st) {
+static void launch_t(int mm, const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
     switch (mm) {
-        case 1: launch_tm<T, 1>(a, grid, st); break;
-        case 2: launch_tm<T, 2>(a, grid, st); break;
-        case 3: launch_tm<T, 3>(a, grid, st); break;
-        case 4: launch_tm<T, 4>(a, grid, st); break;
-        default: launch_tm<T, 5>(a, grid, st); break;
+        case 1: launch_tm<T, 1>(a, merge, grid, st); break;
+        case 2: launch_tm<T, 2>(a, merge, grid, st); break;
+        case 3: launch_tm<T, 3>(a, merge, grid, st); break;
+        case 4: launch_tm<T, 4>(a, merge, grid, st); break;
+        default: launch_tm<T, 5>(a, merge, grid, st); break;
     }
 }
 
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st) {
-    if (mode != STATER_ACC_SEQUENTIAL) return fail(STATER_E_UNSUPPORTED, "stater_accumulate: MERGE64 not built yet");
     if (s->pitch_bytes % 4 || s->comp_stride_bytes % 4 || s->sample_stride_bytes % 4 ||
         s->pitch_bytes < (int64_t)bs->W * 4)
         return fail(STATER_E_ARG, "stater_accumulate: strides must be float multiples covering a row");
@@ -297,6 +416,7 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
     a.sanitize = s->sanitize;
     a.counts = s->counts;
     a.counts_pitch = s->counts_pitch_bytes;
+    a.count_offset = s->count_offset;
     a.n = bs->n;
     a.stats = bs->stats;
     a.lref = bs->lref;
@@ -310,14 +430,15 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         a.inv_lam[k] = 1.f / a.lam[k];
     }
     dim3 grid((bs->W + 63) / 64, bs->H);
-    LaunchTimer t(bs->ctx, st, "accumulate");
+    const bool merge = mode == STATER_ACC_MERGE64;
+    LaunchTimer t(bs->ctx, st, merge ? "accumulate_merge64" : "accumulate");
     switch (bs->S - 1) {
-        case 0: launch_t<0>(bs->mm, a, grid, st); break;
-        case 1: launch_t<1>(bs->mm, a, grid, st); break;
-        case 2: launch_t<2>(bs->mm, a, grid, st); break;
-        case 3: launch_t<3>(bs->mm, a, grid, st); break;
-        case 4: launch_t<4>(bs->mm, a, grid, st); break;
-        default: launch_t<5>(bs->mm, a, grid, st); break;
+        case 0: launch_t<0>(bs->mm, a, merge, grid, st); break;
+        case 1: launch_t<1>(bs->mm, a, merge, grid, st); break;
+        case 2: launch_t<2>(bs->mm, a, merge, grid, st); break;
+        case 3: launch_t<3>(bs->mm, a, merge, grid, st); break;
+        case 4: launch_t<4>(bs->mm, a, merge, grid, st); break;
+        default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
     return STATER_OK;

@@ -523,12 +523,16 @@ int stater_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_p
 }
 
 int stater_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo, uint32_t first,
-                         const stater_samples *out, void *stream) {
+                         const stater_plane *first_plane, const stater_samples *out, void *stream) {
     if (!ctx || !s || !out || !out->data) return fail(STATER_E_ARG, "stater_synth_samples: NULL argument");
     if (s->albedo_modulation && (!albedo || !albedo->ptr))
         return fail(STATER_E_ARG, "stater_synth_samples: scene modulates by albedo; pass the albedo plane");
     if (out->spp <= 0) return fail(STATER_E_ARG, "stater_synth_samples: spp must be positive");
-    return launch_synth_samples(ctx, s, albedo, first, out, stream ? (hipStream_t)stream : ctx->stream);
+    if (first_plane && (!first_plane->ptr || first_plane->dtype != STATER_I32 || first_plane->width != s->width ||
+                        first_plane->height != s->height))
+        return fail(STATER_E_ARG, "stater_synth_samples: first must be an i32 plane of the scene's size");
+    return launch_synth_samples(ctx, s, albedo, first, first_plane, out,
+                                stream ? (hipStream_t)stream : ctx->stream);
 }
 
 int stater_scene_init(stater_scene *s, int model, uint32_t seed, uint32_t image_id, int width, int height);

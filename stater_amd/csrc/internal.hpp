@@ -91,6 +91,7 @@ int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, co
 int launch_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo,
                           const stater_plane *normal, const stater_plane *depth, hipStream_t st);
 int launch_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo,
-                         uint32_t first, const stater_samples *out, hipStream_t st);
+                         uint32_t first, const stater_plane *first_plane, const stater_samples *out,
+                         hipStream_t st);
 
 }  // namespace stater

@@ -121,14 +121,33 @@ __global__ void __launch_bounds__(256) k_features(stater_scene s, float *alb, in
     *(float *)(nb + 2 * nrm_cs) = (float)(n2 * inv);
 }
 
+struct SynthOut {
+    float *out;
+    int64_t pitch, comp, sstride;
+    int spp;
+    const int32_t *counts;
+    int64_t counts_pitch;
+    int count_offset;
+    const int32_t *first;
+    int64_t first_pitch;
+};
+
 __global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *alb, int64_t alb_pitch, int64_t alb_cs,
-                                                 uint32_t first, int spp, float *out, int64_t pitch, int64_t comp,
-                                                 int64_t sstride) {
+                                                 uint32_t first0, SynthOut o) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= s.width) return;
     const int W = s.width, H = s.height;
     const uint32_t p = (uint32_t)(y * W + x);
+    int spp = o.spp;
+    if (o.counts) {
+        const int c = *(const int32_t *)((const char *)o.counts + (int64_t)y * o.counts_pitch + (int64_t)x * 4);
+        spp = min(max(c - o.count_offset, 0), spp);
+    }
+    uint32_t first = first0;
+    if (o.first) first += (uint32_t)*(const int32_t *)((const char *)o.first + (int64_t)y * o.first_pitch + (int64_t)x * 4);
+    float *out = o.out;
+    const int64_t pitch = o.pitch, comp = o.comp, sstride = o.sstride;
     double sp = s.s_min;
     if (s.s_min != s.s_max) {
         const U4 r = philox4x32_10(p, 0xFFFFFFFFu, 0, 0, s.seed, s.image_id);
@@ -139,7 +158,7 @@ __global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *al
         const char *ab = (const char *)alb + (int64_t)y * alb_pitch + (int64_t)x * 4;
         for (int c = 0; c < 3; ++c) mod[c] = 0.2 + 0.8 * (double)*(const float *)(ab + c * alb_cs);
     }
-    char *o = (char *)out + (int64_t)y * pitch + (int64_t)x * 4;
+    char *po = (char *)out + (int64_t)y * pitch + (int64_t)x * 4;
     for (int j = 0; j < spp; ++j) {
         const uint32_t k = first + (uint32_t)j;
         const U4 r = philox4x32_10(p, k, 0, 0, s.seed, s.image_id);
@@ -160,7 +179,7 @@ __global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *al
             const double c = s.edge_vertical[e] ? X : Y;
             if (c >= s.edge_pos[e]) edge *= s.edge_factor[e];
         }
-        char *oj = o + (int64_t)j * sstride;
+        char *oj = po + (int64_t)j * sstride;
         for (int c = 0; c < 3; ++c) {
             double mu = s.base;
             for (int q = 0; q < 3; ++q)
@@ -186,13 +205,14 @@ int launch_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_p
 }
 
 int launch_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo, uint32_t first,
-                         const stater_samples *out, hipStream_t st) {
+                         const stater_plane *first_plane, const stater_samples *out, hipStream_t st) {
     dim3 grid((s->width + 255) / 256, s->height);
+    SynthOut o{(float *)out->data, out->pitch_bytes, out->comp_stride_bytes, out->sample_stride_bytes, out->spp,
+               out->counts, out->counts_pitch_bytes, out->count_offset,
+               first_plane ? (const int32_t *)first_plane->ptr : nullptr, first_plane ? first_plane->pitch_bytes : 0};
     LaunchTimer t(ctx, st, "synth_samples");
     hipLaunchKernelGGL(k_samples, grid, dim3(256), 0, st, *s, albedo ? (const float *)albedo->ptr : nullptr,
-                       albedo ? albedo->pitch_bytes : 0, albedo ? albedo->channel_stride_bytes : 0, first,
-                       out->spp, (float *)out->data, out->pitch_bytes, out->comp_stride_bytes,
-                       out->sample_stride_bytes);
+                       albedo ? albedo->pitch_bytes : 0, albedo ? albedo->channel_stride_bytes : 0, first, o);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }

@@ -31,9 +31,11 @@ class SampleChunk:
         self.ld = row_pitch(width)
         self.plane_bytes = self.ld * height * 4
         self.buf = DeviceBuffer(self.plane_bytes * 5 * spp)
-        self.samples = Samples(self.buf.ptr, self.ld * 4, self.plane_bytes, 5 * self.plane_bytes, spp, 0, None, 0)
+        self.samples = Samples(self.buf.ptr, self.ld * 4, self.plane_bytes, 5 * self.plane_bytes, spp, 0, None, 0,
+                               0, 0)
 
-    def desc(self, spp: int | None = None, sanitize: bool = False, counts: Plane | None = None) -> Samples:
+    def desc(self, spp: int | None = None, sanitize: bool = False, counts: Plane | None = None,
+             count_offset: int = 0) -> Samples:
         s = Samples.from_buffer_copy(self.samples)
         if spp is not None:
             s.spp = spp
@@ -41,6 +43,7 @@ class SampleChunk:
         if counts is not None:
             s.counts = counts.ptr
             s.counts_pitch_bytes = counts.pitch_bytes
+            s.count_offset = count_offset
         return s
 
     def to_numpy(self) -> np.ndarray:
@@ -85,8 +88,11 @@ def synth_features(ctx: Context, sc, feats: Features) -> None:
 
 
 def synth_samples(ctx: Context, sc, chunk: SampleChunk, first_sample: int = 0, feats: Features | None = None,
-                  spp: int | None = None) -> None:
+                  spp: int | None = None, first: Plane | None = None, counts: Plane | None = None,
+                  count_offset: int = 0, stream=None) -> None:
+    """Fill the chunk with samples first_sample + first[p] + j of each pixel
+    (j < spp, or < the clamped per-pixel count when counts is given)."""
     lib = load_library()
-    desc = chunk.desc(spp)
+    desc = chunk.desc(spp, counts=counts, count_offset=count_offset)
     check(lib.stater_synth_samples(ctx.h, C.byref(sc), C.byref(feats.albedo) if feats else None, first_sample,
-                                   C.byref(desc), None))
+                                   C.byref(first) if first else None, C.byref(desc), stream))

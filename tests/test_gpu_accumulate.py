@@ -21,14 +21,17 @@ BOXCOX_RTOL = 1e-4
 
 
 def _scales(st64, n):
-    """Natural magnitude per statistic [7,3,H,W] for relative errors."""
+    """Natural magnitude per statistic [7,3,H,W] for relative errors.  A sum's
+    rounding error scales with the sum of the magnitudes of its terms; for m3
+    that is sum |d|^3 <= sqrt(m2 m4) (Cauchy-Schwarz), which for heavy-tailed
+    pixels is much larger than |m3|."""
     nn = np.maximum(n, 1)[None].astype(np.float64)
     m2 = np.abs(st64[1])
     sd = np.sqrt(m2 / nn)
     return np.stack([
         np.maximum(np.abs(st64[0]), sd),            # mean
         m2,                                         # m2
-        m2 * sd,                                    # m3 ~ n sd^3
+        np.sqrt(m2 * np.abs(st64[3])),              # m3: bound of sum |d|^3
         np.abs(st64[3]),                            # m4
         np.sqrt(m2 * nn / 12.0),                    # c2X ~ sqrt(n/12) sd sqrt(n)
         np.sqrt(m2 * nn / 12.0),                    # c2Y
