@@ -261,6 +261,36 @@ int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
                          const stater_plane *first, const stater_samples *out,
                          void *stream);
 
+/* ---- batches --------------------------------------------------------------
+ * Several films of uneven size (and lambda count) processed together: per
+ * pass of chunk_spp samples one persistent launch generates every image's
+ * samples and one per group accumulates them; then prepass, variance filter
+ * and Welch filter run as persistent launches over the tiles of all images.
+ * use_graph captures the whole run once into a hipGraph and replays it.
+ * The buffers each image ends with are bit-identical to running the image on
+ * its own (stater_accumulate + stater_denoise_vars + stater_denoise_film).
+ * Replaces nothing in the reference (it renders one film per process). */
+typedef struct stater_batch stater_batch;
+typedef struct {
+    stater_bufset         *set;     /* full set (max_moment >= 4) of the scene's size */
+    const stater_scene    *scene;   /* seeded sample source (copied) */
+    const stater_plane    *albedo;  /* the scene's albedo plane when it modulates */
+    const stater_gbuffers *gbufs;   /* range-weight features or NULL (copied) */
+    int32_t                spp;     /* samples per pixel of this image */
+    int32_t                reserved;
+} stater_batch_image;
+
+int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *images,
+                        const stater_denoise_params *p, int chunk_spp, stater_batch **out);
+/* Times the built tile heights (2, 4, 8 rows) of both filters on the batch
+ * itself and keeps the fastest per group; drops a captured graph. */
+int stater_batch_tune(stater_batch *b, void *stream);
+int stater_batch_run(stater_batch *b, int use_graph, void *stream);
+/* group < 0: returns the number of groups; else fills the group's figures. */
+int stater_batch_info(const stater_batch *b, int group, int *sets, int *images,
+                      int *tile_rows_var, int *tile_rows_welch);
+int stater_batch_destroy(stater_batch *b);
+
 /* ---- testing ------------------------------------------------------------
  * Runs the Welch mean filter (plain != 0: the plain method) and also writes
  * every accept/reject decision to dec (device, uint8):

@@ -54,6 +54,11 @@ class GBuffers(C.Structure):
                 ("dr_factors", C.c_float * MAX_GBUFFERS)]
 
 
+class BatchImage(C.Structure):
+    _fields_ = [("set", C.c_void_p), ("scene", C.c_void_p), ("albedo", C.c_void_p), ("gbufs", C.c_void_p),
+                ("spp", C.c_int32), ("reserved", C.c_int32)]
+
+
 class Scene(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_uint32), ("image_id", C.c_uint32),
                 ("model", C.c_int32), ("n_edges", C.c_int32), ("base", C.c_double),
@@ -100,6 +105,13 @@ SIGNATURES = [
                                         C.POINTER(Plane), _P]),
     ("stater_synth_samples", C.c_int, [_P, C.POINTER(Scene), C.POINTER(Plane), C.c_uint32, C.POINTER(Plane),
                                        C.POINTER(Samples), _P]),
+    ("stater_batch_create", C.c_int, [_P, C.c_int, C.POINTER(BatchImage), C.POINTER(DenoiseParams), C.c_int,
+                                      C.POINTER(_P)]),
+    ("stater_batch_tune", C.c_int, [_P, _P]),
+    ("stater_batch_run", C.c_int, [_P, C.c_int, _P]),
+    ("stater_batch_info", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
+    ("stater_batch_destroy", C.c_int, [_P]),
     ("stater_debug_film_decisions", C.c_int, [_P, C.POINTER(DenoiseParams), C.POINTER(GBuffers),
                                               C.POINTER(Plane), C.c_int, _P, _P]),
 ]
@@ -268,3 +280,47 @@ class BufferSet:
 
     def calc_mean_vars(self) -> None:
         check(self.lib.stater_calc_mean_vars(self.h, None))
+
+
+class Batch:
+    """Uneven films through persistent launches (optionally one hipGraph).
+
+    images: list of (BufferSet, Scene, albedo Plane | None, GBuffers | None, spp)."""
+
+    def __init__(self, ctx: Context, images, params: DenoiseParams, chunk_spp: int = 32):
+        self.ctx, self.lib = ctx, ctx.lib
+        self._keep = []
+        arr = (BatchImage * len(images))()
+        for i, (bs, sc, alb, g, spp) in enumerate(images):
+            self._keep += [sc, alb, g]
+            arr[i] = BatchImage(bs.h.value, C.addressof(sc), C.addressof(alb) if alb is not None else None,
+                                C.addressof(g) if g is not None else None, spp, 0)
+        h = C.c_void_p()
+        check(self.lib.stater_batch_create(ctx.h, len(images), arr, C.byref(params), chunk_spp, C.byref(h)))
+        self.h = h
+
+    def tune(self) -> None:
+        check(self.lib.stater_batch_tune(self.h, None))
+
+    def run(self, use_graph: bool = True) -> None:
+        check(self.lib.stater_batch_run(self.h, 1 if use_graph else 0, None))
+
+    def groups(self) -> list[dict]:
+        n = self.lib.stater_batch_info(self.h, -1, None, None, None, None)
+        out = []
+        for g in range(n):
+            s, m, tv, tw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            check(self.lib.stater_batch_info(self.h, g, C.byref(s), C.byref(m), C.byref(tv), C.byref(tw)))
+            out.append({"sets": s.value, "images": m.value, "tile_rows_var": tv.value, "tile_rows_welch": tw.value})
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.stater_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

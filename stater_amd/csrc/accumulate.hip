@@ -41,28 +41,10 @@
 //    estimator.h:288-304 and has no cancellation at n = 1.  The Box-Cox sets
 //    are checked against an FP64 evaluation of the same recurrence.
 
-#include "internal.hpp"
+#include "kernels.hpp"
 
 namespace stater {
 
-struct AccArgs {
-    const float *data;
-    int64_t pitch, comp, sstride;
-    int spp;
-    int sanitize;
-    const int32_t *counts;
-    int64_t counts_pitch;
-    int count_offset;
-    int32_t *n;
-    float *stats;
-    float *lref;    // [3][plane] ln of the reference sample of each pixel-channel
-    float *mshift;  // [S-1][3][plane] Box-Cox means relative to the reference point
-    int ld;
-    size_t plane;
-    int W, H;
-    float lam[kMaxSets];
-    float inv_lam[kMaxSets];
-};
 
 // Reference update, estimator.h:226-305, evaluated exactly as written.
 template <int MM>
@@ -186,11 +168,11 @@ __device__ __forceinline__ float load_sample(const char *base, int64_t off, int6
 }
 
 template <int T, int MM>
-__global__ void __launch_bounds__(192) k_accumulate(AccArgs a) {
+__device__ __forceinline__ void acc_tile(const AccArgs &a, int bx, int by) {
     constexpr int NS = nstats(MM);
     const int c = threadIdx.x >> 6;  // wavefront-uniform channel
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y;
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by;
     if (x >= a.W) return;
     const size_t pix = (size_t)y * a.ld + x;
     int cnt = a.spp;
@@ -280,12 +262,12 @@ __device__ __forceinline__ void merge64(double na, double *a, double nb, const d
 // running mean of the whole sequence, continue sequentially on a running
 // mean carried from the resident state.
 template <int T, int MM>
-__global__ void __launch_bounds__(192) k_accumulate_merge(AccArgs a) {
+__device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
     constexpr int NS = nstats(MM);
     constexpr int NP = MM >= 4 ? 4 : MM;  // partial moments kept
     const int c = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y;
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by;
     if (x >= a.W) return;
     const size_t pix = (size_t)y * a.ld + x;
     int cnt = a.spp;
@@ -373,6 +355,29 @@ __global__ void __launch_bounds__(192) k_accumulate_merge(AccArgs a) {
     if (c == 0) a.n[pix] = (int32_t)(n0 + cnt);
 }
 
+template <int T, int MM>
+__global__ void __launch_bounds__(192) k_accumulate(AccArgs a) { acc_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+
+template <int T, int MM>
+__global__ void __launch_bounds__(192) k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+
+// Persistent form over the row segments of several images (one S, MM group).
+template <int T, int MM, bool MERGE>
+__global__ void __launch_bounds__(192) k_accumulate_batch(const AccArgs *args, const int *prefix, int nimg) {
+    const int ntiles = prefix[nimg];
+    XcdRange r = xcd_range(ntiles);
+    int img = 0;
+    for (int w = r.begin; w < r.end; w += r.step) {
+        while (w >= prefix[img + 1]) ++img;
+        const AccArgs &a = args[img];
+        const int t = w - prefix[img];
+        const int tiles_x = (a.W + 63) / 64;
+        const int by = t / tiles_x, bx = t - by * tiles_x;
+        if (MERGE) accm_tile<T, MM>(a, bx, by);
+        else acc_tile<T, MM>(a, bx, by);
+    }
+}
+
 __global__ void k_zero(uint32_t *p, size_t words) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -403,10 +408,7 @@ static void launch_t(int mm, const AccArgs &a, bool merge, dim3 grid, hipStream_
     }
 }
 
-int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st) {
-    if (s->pitch_bytes % 4 || s->comp_stride_bytes % 4 || s->sample_stride_bytes % 4 ||
-        s->pitch_bytes < (int64_t)bs->W * 4)
-        return fail(STATER_E_ARG, "stater_accumulate: strides must be float multiples covering a row");
+AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s) {
     AccArgs a{};
     a.data = s->data;
     a.pitch = s->pitch_bytes;
@@ -429,6 +431,14 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         a.lam[k] = k < bs->S ? bs->set_lambda[k] : 1.f;
         a.inv_lam[k] = 1.f / a.lam[k];
     }
+    return a;
+}
+
+int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st) {
+    if (s->pitch_bytes % 4 || s->comp_stride_bytes % 4 || s->sample_stride_bytes % 4 ||
+        s->pitch_bytes < (int64_t)bs->W * 4)
+        return fail(STATER_E_ARG, "stater_accumulate: strides must be float multiples covering a row");
+    const AccArgs a = make_acc_args(bs, s);
     dim3 grid((bs->W + 63) / 64, bs->H);
     const bool merge = mode == STATER_ACC_MERGE64;
     LaunchTimer t(bs->ctx, st, merge ? "accumulate_merge64" : "accumulate");
@@ -441,6 +451,36 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+template <int T>
+static const void *acc_batch_t(int mm, bool merge) {
+    switch (mm) {
+        case 1: return merge ? (const void *)k_accumulate_batch<T, 1, true> : (const void *)k_accumulate_batch<T, 1, false>;
+        case 2: return merge ? (const void *)k_accumulate_batch<T, 2, true> : (const void *)k_accumulate_batch<T, 2, false>;
+        case 3: return merge ? (const void *)k_accumulate_batch<T, 3, true> : (const void *)k_accumulate_batch<T, 3, false>;
+        case 4: return merge ? (const void *)k_accumulate_batch<T, 4, true> : (const void *)k_accumulate_batch<T, 4, false>;
+        default: return merge ? (const void *)k_accumulate_batch<T, 5, true> : (const void *)k_accumulate_batch<T, 5, false>;
+    }
+}
+
+const void *accumulate_batch_kernel(int S, int mm, bool merge) {
+    switch (S - 1) {
+        case 0: return acc_batch_t<0>(mm, merge);
+        case 1: return acc_batch_t<1>(mm, merge);
+        case 2: return acc_batch_t<2>(mm, merge);
+        case 3: return acc_batch_t<3>(mm, merge);
+        case 4: return acc_batch_t<4>(mm, merge);
+        default: return acc_batch_t<5>(mm, merge);
+    }
+}
+
+int launch_accumulate_batch(int S, int mm, bool merge, const AccArgs *args, const int *prefix, int nimg, int grid,
+                            hipStream_t st) {
+    const void *k = accumulate_batch_kernel(S, mm, merge);
+    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
+    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(192), params, 0, st));
     return STATER_OK;
 }
 

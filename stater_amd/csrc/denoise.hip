@@ -30,22 +30,15 @@
 #include <cfloat>
 #include <cmath>
 
-#include "internal.hpp"
+#include "kernels.hpp"
 
 namespace stater {
 
-constexpr int TX = 64, TY = 4;
+constexpr int TX = 64, kTY = 4;  // output tile: 64 columns x kTY rows (one wavefront per row)
 constexpr float kLog2e = 1.4426950408889634f;
-constexpr int kMaxFeat = 7;
 
-struct FeatArgs {
-    const float *ptr[kMaxFeat];
-    int64_t pitch[kMaxFeat];
-    float scale[kMaxFeat];  // sqrt(-dr log2 e): w = exp2(sp - sum (g'_p - g'_q)^2)
-    int nf;
-};
 
-static FeatArgs make_feats(const stater_gbuffers *g) {
+FeatArgs make_feats(const stater_gbuffers *g) {
     FeatArgs f{};
     if (!g) return f;
     for (int k = 0; k < g->count; ++k)
@@ -63,18 +56,10 @@ __device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y)
 }
 
 // ================================================================ prepass
-struct PreArgs {
-    const int32_t *n;
-    const float *stats;
-    float *jb, *lnS2, *se2, *m2C;
-    int S, W, H, ld, lev;
-    size_t plane;
-    float z;
-};
 
-__global__ void __launch_bounds__(256) k_prepass(PreArgs a) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+__device__ __forceinline__ void prepass_tile(const PreArgs &a, int bx, int by) {
+    const int x = bx * 256 + threadIdx.x;
+    const int y = by;
     if (x >= a.W) return;
     const size_t pix = (size_t)y * a.ld + x;
     const int n = a.n[pix];
@@ -115,9 +100,35 @@ __global__ void __launch_bounds__(256) k_prepass(PreArgs a) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_prepass(PreArgs a) { prepass_tile(a, blockIdx.x, blockIdx.y); }
+
+__global__ void __launch_bounds__(256) k_prepass_batch(const PreArgs *args, const int *prefix, int nimg) {
+    const int ntiles = prefix[nimg];
+    XcdRange r = xcd_range(ntiles);
+    int img = 0;
+    for (int w = r.begin; w < r.end; w += r.step) {
+        while (w >= prefix[img + 1]) ++img;
+        const PreArgs &a = args[img];
+        const int t = w - prefix[img];
+        const int tiles_x = (a.W + 255) / 256;
+        const int by = t / tiles_x, bx = t - by * tiles_x;
+        prepass_tile(a, bx, by);
+    }
+}
+
+PreArgs make_pre_args(stater_bufset *bs, float var_z) {
+    return PreArgs{bs->n, bs->stats, bs->jb, bs->lnS2, bs->se2, bs->m2C, bs->S, bs->W, bs->H, bs->ld,
+                   bs->mm >= 5 ? 1 : 0, bs->plane, var_z};
+}
+
+int launch_prepass_batch(const PreArgs *args, const int *prefix, int nimg, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_prepass_batch, dim3(grid), dim3(256), 0, st, args, prefix, nimg);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
 int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
-    PreArgs a{bs->n, bs->stats, bs->jb, bs->lnS2, bs->se2, bs->m2C, bs->S, bs->W, bs->H, bs->ld,
-              bs->mm >= 5 ? 1 : 0, bs->plane, var_z};
+    const PreArgs a = make_pre_args(bs, var_z);
     LaunchTimer t(bs->ctx, st, "prepass");
     hipLaunchKernelGGL(k_prepass, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, a);
     STATER_HIP(hipGetLastError());
@@ -131,16 +142,6 @@ int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
 // a compile-time row stride so every LDS access is base + immediate offset.
 constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
 
-struct VarArgs {
-    const int32_t *n;
-    const float *m2;  // set-0 base of the m2 statistic
-    const float *lnS2, *se2, *m2C;
-    float *varD, *varCD;
-    FeatArgs f;
-    int W, H, ld, r, g0;       // g0: first set of this group
-    size_t plane, set_stride;  // distance between sets of the statistics block
-    float z2, sp_scale;        // sp_scale = ds_factor * log2 e
-};
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
@@ -180,13 +181,12 @@ __device__ __forceinline__ void var_stage_row(const VarArgs &a, float *slot, int
     }
 }
 
-template <int G, bool HASW, int HWC>
-__global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
-    extern __shared__ float ring[];
+template <int G, bool HASW, int HWC, int TY>
+__device__ __forceinline__ void var_tile(const VarArgs &a, int bx, int by, float *ring) {
     constexpr int SINGLES = 1 + (HASW ? kNF : 0);
     constexpr int SLOT = (SINGLES + 12 * G) * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < a.W && y < a.H;
 
@@ -276,9 +276,32 @@ __global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
 }
 
 template <int G, bool HASW, int HWC>
+__global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
+    extern __shared__ float ring[];
+    var_tile<G, HASW, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
+}
+
+template <int G, bool HASW, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) k_var_filter_batch(const VarArgs *args, const int *prefix, int nimg) {
+    extern __shared__ float ring[];
+    const int ntiles = prefix[nimg];
+    XcdRange r = xcd_range(ntiles);
+    int img = 0;
+    for (int w = r.begin; w < r.end; w += r.step) {
+        while (w >= prefix[img + 1]) ++img;
+        const VarArgs &a = args[img];
+        const int t = w - prefix[img];
+        const int tiles_x = (a.W + TX - 1) / TX;
+        const int by = t / tiles_x, bx = t - by * tiles_x;
+        var_tile<G, HASW, HWC, TY>(a, bx, by, ring);
+        __syncthreads();  // the next tile restages the ring
+    }
+}
+
+template <int G, bool HASW, int HWC>
 static int var_launch3(const VarArgs &a, dim3 grid, hipStream_t st) {
     constexpr int SINGLES = 1 + (HASW ? kNF : 0);
-    const size_t lds = (size_t)TY * (SINGLES + 12 * G) * HWC * 4;
+    const size_t lds = (size_t)kTY * (SINGLES + 12 * G) * HWC * 4;
     STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_var_filter<G, HASW, HWC>), grid, dim3(256), lds, st, a);
@@ -290,6 +313,56 @@ template <int G>
 static int var_launch(const VarArgs &a, bool hasw, dim3 grid, hipStream_t st) {
     if (a.r <= 10) return hasw ? var_launch3<G, true, TX + 20>(a, grid, st) : var_launch3<G, false, TX + 20>(a, grid, st);
     return hasw ? var_launch3<G, true, TX + 40>(a, grid, st) : var_launch3<G, false, TX + 40>(a, grid, st);
+}
+
+VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0) {
+    VarArgs a{};
+    a.n = bs->n;
+    a.m2 = bs->stat(0, ST_M2, 0);
+    a.lnS2 = bs->lnS2; a.se2 = bs->se2; a.m2C = bs->m2C;
+    a.varD = bs->varD; a.varCD = bs->varCD;
+    a.f = make_feats(g);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius; a.g0 = g0;
+    a.plane = bs->plane;
+    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    a.z2 = p->var_z * p->var_z;
+    a.sp_scale = p->ds_factor * kLog2e;
+    return a;
+}
+
+// Batched variance filter: r <= 10 (row stride 84); tile rows 2, 4 or 8 with
+// feature weights, 4 without.
+template <int G, bool HASW, int TY>
+static const void *var_bk(size_t *lds, int) {
+    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
+    *lds = (size_t)TY * (SINGLES + 12 * G) * (TX + 20) * 4;
+    return (const void *)k_var_filter_batch<G, HASW, TX + 20, TY>;
+}
+
+template <int G>
+static const void *var_bk_g(bool hasw, int ty, size_t *lds) {
+    if (!hasw) return ty == 4 ? var_bk<G, false, 4>(lds, 0) : nullptr;
+    switch (ty) {
+        case 2: return var_bk<G, true, 2>(lds, 0);
+        case 4: return var_bk<G, true, 4>(lds, 0);
+        case 8: return var_bk<G, true, 8>(lds, 0);
+        default: return nullptr;
+    }
+}
+
+const void *var_batch_kernel(int G, bool hasw, int r, int ty, size_t *lds, int) {
+    if (r > 10) return nullptr;
+    return G == 3 ? var_bk_g<3>(hasw, ty, lds) : G == 2 ? var_bk_g<2>(hasw, ty, lds) : var_bk_g<1>(hasw, ty, lds);
+}
+
+int launch_var_batch(int G, bool hasw, int r, int ty, const VarArgs *args, const int *prefix, int nimg, int grid,
+                     hipStream_t st) {
+    size_t lds = 0;
+    const void *k = var_batch_kernel(G, hasw, r, ty, &lds, 0);
+    if (!k) return fail(STATER_E_UNSUPPORTED, "batch variance filter: configuration not built");
+    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
+    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(64 * ty), params, lds, st));
+    return STATER_OK;
 }
 
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
@@ -306,7 +379,7 @@ int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const s
     a.z2 = p->var_z * p->var_z;
     a.sp_scale = p->ds_factor * kLog2e;
     const bool hasw = a.f.nf > 0 || p->ds_factor != 0.f;
-    dim3 grid((bs->W + TX - 1) / TX, (bs->H + TY - 1) / TY);
+    dim3 grid((bs->W + TX - 1) / TX, (bs->H + kTY - 1) / kTY);
     LaunchTimer t(bs->ctx, st, "var_filter");
     for (int g0 = 0; g0 < bs->S;) {
         const int left = bs->S - g0;
@@ -323,26 +396,6 @@ int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const s
 // ========================================================== Welch filter
 // LDS ring layout (floats), per slot: single rows (ok, n-1, padded features,
 // film[3], plain: s^2[3]) then one float2 row (M, V) per set-channel.
-struct WelchArgs {
-    const int32_t *n;
-    const float *mean;     // statistics block base (set s mean = mean + s*set_stride)
-    const float *var;      // variance fed to the test, [S][3][plane] (or m2 for the raw test)
-    int var_is_m2;         // 1: var = m2/(n-1) computed on the fly from the statistics
-    const float *film;     // filtered values and set-0 test mean (planar, 3 ch)
-    int64_t film_pitch, film_cs;
-    const float *jb;
-    float *out;            // filmD
-    int64_t out_pitch, out_cs;
-    float *discr;          // [S][3][plane]
-    float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
-    uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
-    FeatArgs f;
-    int W, H, ld, r, mode3;
-    size_t plane, set_stride;
-    float sp_scale;
-    double alpha;
-    TCritPoly tp;
-};
 
 // FP64 tail for pairs near the critical value (rare; kept out of line).
 __device__ __noinline__ bool welch_exact(float mp, float vp, float nm1p, float mq, float vq, float nm1q, double alpha) {
@@ -443,13 +496,12 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
     }
 }
 
-template <int S, bool HASW, bool PLAIN, int HWC>
-__global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
+template <int S, bool HASW, bool PLAIN, int HWC, int TY>
+__device__ __forceinline__ void welch_tile(const WelchArgs &a, int bx, int by, float *ring) {
     using LY = WelchLayout<S, HASW, PLAIN>;
-    extern __shared__ float ring[];
     constexpr int SLOT = LY::FLOATS * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < a.W && y < a.H;
     const int win = 2 * a.r + 1;
@@ -595,8 +647,31 @@ __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
 }
 
 template <int S, bool HASW, bool PLAIN, int HWC>
+__global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
+    extern __shared__ float ring[];
+    welch_tile<S, HASW, PLAIN, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
+}
+
+template <int S, bool HASW, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) k_welch_batch(const WelchArgs *args, const int *prefix, int nimg) {
+    extern __shared__ float ring[];
+    const int ntiles = prefix[nimg];
+    XcdRange r = xcd_range(ntiles);
+    int img = 0;
+    for (int w = r.begin; w < r.end; w += r.step) {
+        while (w >= prefix[img + 1]) ++img;
+        const WelchArgs &a = args[img];
+        const int t = w - prefix[img];
+        const int tiles_x = (a.W + TX - 1) / TX;
+        const int by = t / tiles_x, bx = t - by * tiles_x;
+        welch_tile<S, HASW, false, HWC, TY>(a, bx, by, ring);
+        __syncthreads();  // the next tile restages the ring
+    }
+}
+
+template <int S, bool HASW, bool PLAIN, int HWC>
 static int welch_launch4(const WelchArgs &a, dim3 grid, hipStream_t st) {
-    const size_t lds = (size_t)TY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
+    const size_t lds = (size_t)kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, HWC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, HWC>), grid, dim3(256), lds, st, a);
@@ -613,6 +688,84 @@ static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
 template <int S>
 static int welch_launch2(const WelchArgs &a, bool hasw, dim3 grid, hipStream_t st) {
     return hasw ? welch_launch3<S, true, false>(a, grid, st) : welch_launch3<S, false, false>(a, grid, st);
+}
+
+static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, bool plain,
+                            WelchArgs &a);
+
+WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g) {
+    WelchArgs a{};
+    fill_welch_args(bs, p, g, false, a);
+    a.film = bs->stat(0, ST_MEAN, 0);
+    a.film_pitch = (int64_t)bs->ld * 4;
+    a.film_cs = (int64_t)bs->plane * 4;
+    return a;
+}
+
+template <int S, bool HASW, int TY>
+static const void *welch_bk(size_t *lds) {
+    *lds = (size_t)TY * WelchLayout<S, HASW, false>::FLOATS * (TX + 20) * 4;
+    return (const void *)k_welch_batch<S, HASW, TX + 20, TY>;
+}
+
+template <int S>
+static const void *welch_bk_s(bool hasw, int ty, size_t *lds) {
+    if (!hasw) return ty == 4 ? welch_bk<S, false, 4>(lds) : nullptr;
+    switch (ty) {
+        case 2: return welch_bk<S, true, 2>(lds);
+        case 4: return welch_bk<S, true, 4>(lds);
+        case 8: return welch_bk<S, true, 8>(lds);
+        default: return nullptr;
+    }
+}
+
+const void *welch_batch_kernel(int S, bool hasw, int r, int ty, size_t *lds) {
+    if (r > 10) return nullptr;
+    switch (S) {
+        case 1: return welch_bk_s<1>(hasw, ty, lds);
+        case 2: return welch_bk_s<2>(hasw, ty, lds);
+        case 3: return welch_bk_s<3>(hasw, ty, lds);
+        case 4: return welch_bk_s<4>(hasw, ty, lds);
+        case 5: return welch_bk_s<5>(hasw, ty, lds);
+        default: return welch_bk_s<6>(hasw, ty, lds);
+    }
+}
+
+int launch_welch_batch(int S, bool hasw, int r, int ty, const WelchArgs *args, const int *prefix, int nimg,
+                       int grid, hipStream_t st) {
+    size_t lds = 0;
+    const void *k = welch_batch_kernel(S, hasw, r, ty, &lds);
+    if (!k) return fail(STATER_E_UNSUPPORTED, "batch Welch filter: configuration not built");
+    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
+    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(64 * ty), params, lds, st));
+    return STATER_OK;
+}
+
+static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, bool plain,
+                            WelchArgs &a) {
+    a.n = bs->n;
+    a.mean = bs->stat(0, ST_MEAN, 0);
+    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    if (plain || !p->cascade) {
+        a.var = bs->stat(0, ST_M2, 0);
+        a.var_is_m2 = 1;
+    } else {
+        a.var = p->use_levc ? bs->varCD : bs->varD;
+        a.var_is_m2 = 0;
+    }
+    a.out = plain ? bs->meanD : bs->filmD;
+    a.out_pitch = (int64_t)bs->ld * 4;
+    a.out_cs = (int64_t)bs->plane * 4;
+    a.jb = bs->jb;
+    a.discr = bs->discr;
+    a.varW = bs->varD;
+    a.f = plain ? FeatArgs{} : make_feats(g);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
+    a.mode3 = (!plain && p->mt_mode == 3 && bs->S > 1) ? 1 : 0;
+    a.plane = bs->plane;
+    a.sp_scale = plain ? 0.f : p->ds_factor * kLog2e;
+    a.alpha = (double)p->alpha;
+    a.tp = tcrit_for(bs->ctx, (double)p->alpha);
 }
 
 static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
@@ -661,7 +814,7 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     a.tp = tcrit_for(bs->ctx, (double)p->alpha);
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
-    dim3 grid((bs->W + TX - 1) / TX, (bs->H + TY - 1) / TY);
+    dim3 grid((bs->W + TX - 1) / TX, (bs->H + kTY - 1) / kTY);
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
     if (plain) return welch_launch3<1, false, true>(a, grid, st);
     switch (S) {

@@ -1,0 +1,121 @@
+// kernels.hpp — kernel argument blocks shared by the single-image launches
+// and the batch runtime (which keeps tables of them in device memory).
+#pragma once
+
+#include "internal.hpp"
+
+namespace stater {
+
+constexpr int kMaxFeat = 7;
+
+struct AccArgs {
+    const float *data;
+    int64_t pitch, comp, sstride;
+    int spp;
+    int sanitize;
+    const int32_t *counts;
+    int64_t counts_pitch;
+    int count_offset;
+    int32_t *n;
+    float *stats;
+    float *lref;    // [3][plane] ln of the reference sample of each pixel-channel
+    float *mshift;  // [S-1][3][plane] Box-Cox means relative to the reference point
+    int ld;
+    size_t plane;
+    int W, H;
+    float lam[kMaxSets];
+    float inv_lam[kMaxSets];
+};
+
+struct FeatArgs {
+    const float *ptr[kMaxFeat];
+    int64_t pitch[kMaxFeat];
+    float scale[kMaxFeat];  // sqrt(-dr log2 e): w = exp2(sp - sum (g'_p - g'_q)^2)
+    int nf;
+};
+
+struct PreArgs {
+    const int32_t *n;
+    const float *stats;
+    float *jb, *lnS2, *se2, *m2C;
+    int S, W, H, ld, lev;
+    size_t plane;
+    float z;
+};
+
+struct VarArgs {
+    const int32_t *n;
+    const float *m2;  // set-0 base of the m2 statistic
+    const float *lnS2, *se2, *m2C;
+    float *varD, *varCD;
+    FeatArgs f;
+    int W, H, ld, r, g0;       // g0: first set of this group
+    size_t plane, set_stride;  // distance between sets of the statistics block
+    float z2, sp_scale;        // sp_scale = ds_factor * log2 e
+};
+
+struct WelchArgs {
+    const int32_t *n;
+    const float *mean;     // statistics block base (set s mean = mean + s*set_stride)
+    const float *var;      // variance fed to the test, [S][3][plane] (or m2 for the raw test)
+    int var_is_m2;         // 1: var = m2/(n-1) computed on the fly from the statistics
+    const float *film;     // filtered values and set-0 test mean (planar, 3 ch)
+    int64_t film_pitch, film_cs;
+    const float *jb;
+    float *out;            // filmD
+    int64_t out_pitch, out_cs;
+    float *discr;          // [S][3][plane]
+    float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
+    uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
+    FeatArgs f;
+    int W, H, ld, r, mode3;
+    size_t plane, set_stride;
+    float sp_scale;
+    double alpha;
+    TCritPoly tp;
+};
+
+struct SynthOut {
+    float *out;
+    int64_t pitch, comp, sstride;
+    int spp;
+    const int32_t *counts;
+    int64_t counts_pitch;
+    int count_offset;
+    const int32_t *first;
+    int64_t first_pitch;
+};
+
+struct SampleJob {
+    stater_scene s;
+    const float *alb;
+    int64_t alb_pitch, alb_cs;
+    uint32_t first0;
+    SynthOut o;
+};
+
+// Argument builders (the same ones the single-image launches use).
+AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s);
+PreArgs make_pre_args(stater_bufset *bs, float var_z);
+FeatArgs make_feats(const stater_gbuffers *g);
+VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0);
+WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g);
+SynthOut make_synth_out(const stater_samples *out, const stater_plane *first_plane);
+
+// Batched persistent launches: args and prefix are device arrays; the
+// prefix has nimg + 1 entries (tiles per image, exclusive scan).
+int launch_accumulate_batch(int S, int mm, bool merge, const AccArgs *args, const int *prefix, int nimg, int grid,
+                            hipStream_t st);
+int launch_samples_batch(const SampleJob *jobs, const int *prefix, int nimg, int grid, hipStream_t st);
+int launch_prepass_batch(const PreArgs *args, const int *prefix, int nimg, int grid, hipStream_t st);
+int launch_var_batch(int G, bool hasw, int r, int ty, const VarArgs *args, const int *prefix, int nimg, int grid,
+                     hipStream_t st);
+int launch_welch_batch(int S, bool hasw, int r, int ty, const WelchArgs *args, const int *prefix, int nimg,
+                       int grid, hipStream_t st);
+// Workgroups of a persistent launch: occupancy x CUs, a multiple of 8.
+int batch_grid(stater_ctx *ctx, const void *kernel, int block, size_t lds, int ntiles);
+const void *accumulate_batch_kernel(int S, int mm, bool merge);
+const void *var_batch_kernel(int G, bool hasw, int r, int ty, size_t *lds, int nf);
+const void *welch_batch_kernel(int S, bool hasw, int r, int ty, size_t *lds);
+
+}  // namespace stater

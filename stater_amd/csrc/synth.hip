@@ -17,7 +17,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "internal.hpp"
+#include "kernels.hpp"
 
 namespace stater {
 
@@ -121,21 +121,11 @@ __global__ void __launch_bounds__(256) k_features(stater_scene s, float *alb, in
     *(float *)(nb + 2 * nrm_cs) = (float)(n2 * inv);
 }
 
-struct SynthOut {
-    float *out;
-    int64_t pitch, comp, sstride;
-    int spp;
-    const int32_t *counts;
-    int64_t counts_pitch;
-    int count_offset;
-    const int32_t *first;
-    int64_t first_pitch;
-};
 
-__global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *alb, int64_t alb_pitch, int64_t alb_cs,
-                                                 uint32_t first0, SynthOut o) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+__device__ __forceinline__ void samples_tile(const stater_scene &s, const float *alb, int64_t alb_pitch,
+                                             int64_t alb_cs, uint32_t first0, const SynthOut &o, int bx, int by) {
+    const int x = bx * 256 + threadIdx.x;
+    const int y = by;
     if (x >= s.width) return;
     const int W = s.width, H = s.height;
     const uint32_t p = (uint32_t)(y * W + x);
@@ -204,15 +194,46 @@ int launch_synth_features(stater_ctx *ctx, const stater_scene *s, const stater_p
     return STATER_OK;
 }
 
+__global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *alb, int64_t alb_pitch, int64_t alb_cs,
+                                                 uint32_t first0, SynthOut o) {
+    samples_tile(s, alb, alb_pitch, alb_cs, first0, o, blockIdx.x, blockIdx.y);
+}
+
+
+__global__ void __launch_bounds__(256) k_samples_batch(const SampleJob *jobs, const int *prefix, int nimg) {
+    const int ntiles = prefix[nimg];
+    XcdRange r = xcd_range(ntiles);
+    int img = 0;
+    for (int w = r.begin; w < r.end; w += r.step) {
+        while (w >= prefix[img + 1]) ++img;
+        const SampleJob &j = jobs[img];
+        const int t = w - prefix[img];
+        const int tiles_x = (j.s.width + 255) / 256;
+        const int by = t / tiles_x, bx = t - by * tiles_x;
+        samples_tile(j.s, j.alb, j.alb_pitch, j.alb_cs, j.first0, j.o, bx, by);
+    }
+}
+
+SynthOut make_synth_out(const stater_samples *out, const stater_plane *first_plane) {
+    return SynthOut{(float *)out->data, out->pitch_bytes, out->comp_stride_bytes, out->sample_stride_bytes, out->spp,
+                    out->counts, out->counts_pitch_bytes, out->count_offset,
+                    first_plane ? (const int32_t *)first_plane->ptr : nullptr,
+                    first_plane ? first_plane->pitch_bytes : 0};
+}
+
 int launch_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_plane *albedo, uint32_t first,
                          const stater_plane *first_plane, const stater_samples *out, hipStream_t st) {
     dim3 grid((s->width + 255) / 256, s->height);
-    SynthOut o{(float *)out->data, out->pitch_bytes, out->comp_stride_bytes, out->sample_stride_bytes, out->spp,
-               out->counts, out->counts_pitch_bytes, out->count_offset,
-               first_plane ? (const int32_t *)first_plane->ptr : nullptr, first_plane ? first_plane->pitch_bytes : 0};
+    const SynthOut o = make_synth_out(out, first_plane);
     LaunchTimer t(ctx, st, "synth_samples");
     hipLaunchKernelGGL(k_samples, grid, dim3(256), 0, st, *s, albedo ? (const float *)albedo->ptr : nullptr,
                        albedo ? albedo->pitch_bytes : 0, albedo ? albedo->channel_stride_bytes : 0, first, o);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+int launch_samples_batch(const SampleJob *jobs, const int *prefix, int nimg, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_samples_batch, dim3(grid), dim3(256), 0, st, jobs, prefix, nimg);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }

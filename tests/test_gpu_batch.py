@@ -1,0 +1,73 @@
+"""Batched persistent launches over uneven images (plain, tuned, graph replay):
+every image's buffers must be bit-identical to running that image on its own."""
+import numpy as np
+import pytest
+
+import stater_amd as sa
+from gpu_helpers import read_set_buf, read_stats
+from stater_amd import hipmem
+from stater_amd.workload import FEATURE_SDS, Features, SampleChunk, make_scene, synth_features, synth_samples
+
+pytestmark = pytest.mark.gpu
+
+NONID = (0.01, 0.25, 0.5, 0.75, 3.0)
+SHAPES = [(96, 64, 1, 40), (130, 70, 3, 23), (64, 40, 5, 64), (200, 48, 2, 17), (77, 90, 4, 50), (40, 40, 5, 33)]
+
+
+def _image_lambdas(T):
+    return NONID[:T] + (1.0,)
+
+
+def _outputs(bs, lam):
+    n, st = read_stats(bs, lam)
+    out = {"n": n, "stats": st, "filmD": hipmem.read_plane(bs.view("filmD"))}
+    for name in ("varD", "varCD", "lnS2", "discr"):
+        out[name] = read_set_buf(bs, lam, name)
+    return out
+
+
+def _same(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+def test_batch_matches_single_images(ctx):
+    p = sa.default_params(radius=6)
+    imgs, keep = [], []
+    for i, (W, H, T, spp) in enumerate(SHAPES):
+        sc = make_scene(2, 1000 + i, W, H, image_id=i)
+        feats = Features(W, H)
+        synth_features(ctx, sc, feats)
+        bs = sa.BufferSet(ctx, W, H, _image_lambdas(T), 5)
+        g = feats.gbuffers(FEATURE_SDS)
+        imgs.append((bs, sc, feats.albedo, g, spp))
+        keep.append(feats)
+    batch = sa.Batch(ctx, imgs, p, chunk_spp=16)
+    groups = batch.groups()
+    assert sum(g["images"] for g in groups) == len(SHAPES) and len(groups) == 5
+    batch.run(use_graph=False)
+    ctx.synchronize()
+    got = [_outputs(bs, _image_lambdas(T)) for (bs, *_), (_, _, T, _) in zip(imgs, SHAPES)]
+
+    # each image alone through the single-image entry points
+    for i, ((bs, sc, alb, g, spp), feats, (W, H, T, _)) in enumerate(zip(imgs, keep, SHAPES)):
+        lam = _image_lambdas(T)
+        solo = sa.BufferSet(ctx, W, H, lam, 5)
+        chunk = SampleChunk(W, H, spp)
+        synth_samples(ctx, sc, chunk, 0, feats)
+        solo.accumulate(chunk.desc())
+        solo.denoise_vars(p, g)
+        solo.denoise_film(p, g)
+        ctx.synchronize()
+        _same(got[i], _outputs(solo, lam))
+        solo.close()
+
+    # tuned tile heights, captured graph, replayed twice: same buffers
+    batch.tune()
+    for _ in range(2):
+        batch.run(use_graph=True)
+        ctx.synchronize()
+        for i, ((bs, *_), (_, _, T, _)) in enumerate(zip(imgs, SHAPES)):
+            _same(got[i], _outputs(bs, _image_lambdas(T)))
+    print(batch.groups())
+    batch.close()
