@@ -168,13 +168,14 @@ __device__ __forceinline__ float load_sample(const char *base, int64_t off, int6
 }
 
 template <int T, int MM>
-__device__ __forceinline__ void acc_tile(const AccArgs &a, int bx, int by) {
+__device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
     constexpr int NS = nstats(MM);
-    const int c = threadIdx.x >> 6;  // wavefront-uniform channel
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront-uniform channel
     const int x = bx * 64 + (threadIdx.x & 63);
     const int y = by;
     if (x >= a.W) return;
-    const size_t pix = (size_t)y * a.ld + x;
+    // 32-bit pixel offset on uniform plane bases (width, height <= 16384)
+    uint32_t pix = (uint32_t)y * (uint32_t)a.ld + (uint32_t)x;
     int cnt = a.spp;
     if (a.counts)
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
@@ -190,7 +191,7 @@ __device__ __forceinline__ void acc_tile(const AccArgs &a, int bx, int by) {
         for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
 
     // Box-Cox reference point: the first sample the pixel-channel ever saw.
-    float L = 0.f, K[T + 1], Wt[T + 1];
+    float L = 0.f, Wt[T + 1];
     if (T > 0) {
         const size_t lp = (size_t)c * a.plane + pix;
         if (n == 0) {
@@ -202,7 +203,6 @@ __device__ __forceinline__ void acc_tile(const AccArgs &a, int bx, int by) {
         }
 #pragma unroll
         for (int s = 1; s <= T; ++s) {
-            K[s] = expm1f(a.lam[s] * L) * a.inv_lam[s];
             Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
             st[s][0] = n ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
         }
@@ -227,10 +227,13 @@ __device__ __forceinline__ void acc_tile(const AccArgs &a, int bx, int by) {
             for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], Wt[s] * expm1_fast(a.lam[s] * dl), f, xD, yD);
         }
     }
+    // The statistic addresses are rebuilt after the loop rather than held in
+    // 2 x (T+1) x NS registers across it.
+    asm volatile("" : "+v"(pix));
 #pragma unroll
     for (int s = 1; s <= T; ++s) {
         a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[s][0];
-        st[s][0] += K[s];
+        st[s][0] += expm1f(a.lam[s] * L) * a.inv_lam[s];  // K: shift of the reference point
     }
 #pragma unroll
     for (int s = 0; s <= T; ++s)
@@ -262,14 +265,15 @@ __device__ __forceinline__ void merge64(double na, double *a, double nb, const d
 // running mean of the whole sequence, continue sequentially on a running
 // mean carried from the resident state.
 template <int T, int MM>
-__device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
+__device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
     constexpr int NS = nstats(MM);
     constexpr int NP = MM >= 4 ? 4 : MM;  // partial moments kept
-    const int c = threadIdx.x >> 6;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int x = bx * 64 + (threadIdx.x & 63);
     const int y = by;
     if (x >= a.W) return;
-    const size_t pix = (size_t)y * a.ld + x;
+    // 32-bit pixel offset on uniform plane bases (width, height <= 16384)
+    uint32_t pix = (uint32_t)y * (uint32_t)a.ld + (uint32_t)x;
     int cnt = a.spp;
     if (a.counts)
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
@@ -283,7 +287,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
     for (int s = 0; s <= T; ++s)
 #pragma unroll
         for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
-    float L = 0.f, K[T + 1], Wt[T + 1];
+    float L = 0.f, Wt[T + 1];
     if (T > 0) {
         const size_t lp = (size_t)c * a.plane + pix;
         if (n0 == 0) {
@@ -295,7 +299,6 @@ __device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
         }
 #pragma unroll
         for (int s = 1; s <= T; ++s) {
-            K[s] = expm1f(a.lam[s] * L) * a.inv_lam[s];
             Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
             st[s][0] = n0 ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
         }
@@ -335,6 +338,9 @@ __device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
             }
         }
     }
+    // The statistic addresses are rebuilt after the loop rather than held in
+    // 2 x (T+1) x NS registers across it.
+    asm volatile("" : "+v"(pix));
 #pragma unroll
     for (int s = 0; s <= T; ++s) {
         double A[4] = {st[s][0], NS > 1 ? st[s][1] : 0.0, NS > 2 ? st[s][2] : 0.0, NS > 3 ? st[s][3] : 0.0};
@@ -346,7 +352,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
 #pragma unroll
     for (int s = 1; s <= T; ++s) {
         a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[s][0];
-        st[s][0] += K[s];
+        st[s][0] += expm1f(a.lam[s] * L) * a.inv_lam[s];  // K: shift of the reference point
     }
 #pragma unroll
     for (int s = 0; s <= T; ++s)
@@ -355,15 +361,19 @@ __device__ __forceinline__ void accm_tile(const AccArgs &a, int bx, int by) {
     if (c == 0) a.n[pix] = (int32_t)(n0 + cnt);
 }
 
-template <int T, int MM>
-__global__ void __launch_bounds__(192) k_accumulate(AccArgs a) { acc_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+// At least three waves per SIMD (at most 168 VGPRs); the six-set body with
+// the store addresses rebuilt after the sample loop needs about 100.
+#define STATER_ACC_OCC __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3)))
 
 template <int T, int MM>
-__global__ void __launch_bounds__(192) k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+__global__ void STATER_ACC_OCC k_accumulate(AccArgs a) { acc_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+
+template <int T, int MM>
+__global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
 
 // Persistent form over the row segments of several images (one S, MM group).
 template <int T, int MM, bool MERGE>
-__global__ void __launch_bounds__(192) k_accumulate_batch(const AccArgs *args, const int *prefix, int nimg) {
+__global__ void STATER_ACC_OCC k_accumulate_batch(const AccArgs *args, const int *prefix, int nimg) {
     const int ntiles = prefix[nimg];
     XcdRange r = xcd_range(ntiles);
     int img = 0;

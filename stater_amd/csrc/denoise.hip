@@ -57,7 +57,7 @@ __device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y)
 
 // ================================================================ prepass
 
-__device__ __forceinline__ void prepass_tile(const PreArgs &a, int bx, int by) {
+__device__ __forceinline__ void prepass_tile(const PreArgs a, int bx, int by) {
     const int x = bx * 256 + threadIdx.x;
     const int y = by;
     if (x >= a.W) return;
@@ -182,7 +182,7 @@ __device__ __forceinline__ void var_stage_row(const VarArgs &a, float *slot, int
 }
 
 template <int G, bool HASW, int HWC, int TY>
-__device__ __forceinline__ void var_tile(const VarArgs &a, int bx, int by, float *ring) {
+__device__ __forceinline__ void var_tile(const VarArgs a, int bx, int by, float *ring) {
     constexpr int SINGLES = 1 + (HASW ? kNF : 0);
     constexpr int SLOT = (SINGLES + 12 * G) * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -497,7 +497,7 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
 }
 
 template <int S, bool HASW, bool PLAIN, int HWC, int TY>
-__device__ __forceinline__ void welch_tile(const WelchArgs &a, int bx, int by, float *ring) {
+__device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, float *ring) {
     using LY = WelchLayout<S, HASW, PLAIN>;
     constexpr int SLOT = LY::FLOATS * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;

@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(256) k_features(stater_scene s, float *alb, in
 }
 
 
-__device__ __forceinline__ void samples_tile(const stater_scene &s, const float *alb, int64_t alb_pitch,
-                                             int64_t alb_cs, uint32_t first0, const SynthOut &o, int bx, int by) {
+__device__ __forceinline__ void samples_tile(const stater_scene s, const float *alb, int64_t alb_pitch,
+                                             int64_t alb_cs, uint32_t first0, const SynthOut o, int bx, int by) {
     const int x = bx * 256 + threadIdx.x;
     const int y = by;
     if (x >= s.width) return;
