@@ -81,9 +81,10 @@ double t2_crit_host(double nu, double alpha) {
 
 TCritPoly make_tcrit_poly(double alpha) {
     TCritPoly tp{};
-    const int D = kTCritDeg;
+    constexpr int D = kTCritDeg;
     const double umax = 0.0625;
-    // Chebyshev interpolation of g(u) = t2crit(1/u) on s = 2u/umax - 1.
+    // Chebyshev interpolation of g(u) = t2crit(1/u) on [0, umax] (nodes in
+    // s = 2u/umax - 1), then the monomial coefficients in u.
     double gnode[D + 1], cheb[D + 1];
     for (int k = 0; k <= D; ++k) {
         const double s = std::cos(kPi * (k + 0.5) / (D + 1));
@@ -95,34 +96,41 @@ TCritPoly make_tcrit_poly(double alpha) {
         for (int k = 0; k <= D; ++k) acc += gnode[k] * std::cos(kPi * j * (k + 0.5) / (D + 1));
         cheb[j] = acc * (j == 0 ? 1.0 : 2.0) / (D + 1);
     }
-    // Chebyshev -> monomial coefficients in s (T_{j+1} = 2 s T_j - T_{j-1}).
+    // T_j(s) as monomials in s (T_{j+1} = 2 s T_j - T_{j-1}), then s = a u - 1.
     double Tm[D + 1][D + 1] = {};
     Tm[0][0] = 1.0;
     Tm[1][1] = 1.0;
     for (int j = 2; j <= D; ++j)
         for (int i = 0; i <= D; ++i)
             Tm[j][i] = (i ? 2.0 * Tm[j - 1][i - 1] : 0.0) - Tm[j - 2][i];
-    for (int i = 0; i <= D; ++i) {
-        double a = 0;
-        for (int j = 0; j <= D; ++j) a += cheb[j] * Tm[j][i];
-        tp.c[i] = (float)a;
+    double ms[D + 1] = {}, mu[D + 1] = {};
+    for (int i = 0; i <= D; ++i)
+        for (int j = 0; j <= D; ++j) ms[i] += cheb[j] * Tm[j][i];
+    const double sa = 2.0 / umax;
+    for (int i = 0; i <= D; ++i) {  // (sa u - 1)^i by the binomial theorem
+        double binom = 1.0;
+        for (int m = 0; m <= i; ++m) {
+            mu[m] += ms[i] * binom * std::pow(sa, m) * (((i - m) & 1) ? -1.0 : 1.0);
+            binom = binom * (i - m) / (m + 1);
+        }
     }
-    tp.scale = (float)(2.0 / umax);
-    tp.shift = -1.0f;
+    for (int i = 0; i <= D; ++i) tp.c[i] = (float)mu[i];
     tp.umax = (float)umax;
     tp.alpha = alpha;
     // Measure the FP32 evaluation error against the FP64 tail on a grid and
     // size the exact-tail band well above it.
     double maxerr = 0;
-    for (int i = 0; i <= 400; ++i) {
-        const double u = umax * i / 400.0;
-        const float s = (float)u * tp.scale + tp.shift;
+    for (int i = 0; i <= 1000; ++i) {
+        const double u = umax * i / 1000.0;
+        const float uf = (float)u;
         float acc = tp.c[D];
-        for (int k = D - 1; k >= 0; --k) acc = acc * s + tp.c[k];
+        for (int k = D - 1; k >= 0; --k) acc = std::fma(acc, uf, tp.c[k]);
         const double ref = u == 0.0 ? normal_t2_crit(alpha) : t2_crit_host(1.0 / u, alpha);
         maxerr = std::max(maxerr, std::fabs((double)acc / ref - 1.0));
     }
     tp.band = (float)std::max(2e-6, 8.0 * maxerr + 2e-6);
+    tp.neg_hi = -(1.f + tp.band);
+    tp.neg_lo = -(1.f - tp.band);
     return tp;
 }
 

@@ -407,25 +407,33 @@ __device__ __noinline__ bool welch_exact(float mp, float vp, float nm1p, float m
     return t2_tail(t2, nu) > alpha;
 }
 
-__device__ __forceinline__ float tcrit_eval(const TCritPoly &tp, float u) {
-    const float s = __builtin_fmaf(u, tp.scale, tp.shift);
-    float acc = tp.c[kTCritDeg];
-#pragma unroll
-    for (int k = kTCritDeg - 1; k >= 0; --k) acc = __builtin_fmaf(acc, s, tp.c[k]);
-    return acc;
-}
+constexpr float kUBig = 281474976710656.f;  // 2^48: pushes e below zero for u > umax
 
-// The FP32 Welch statistic of a pair: t2 = dm^2/V, u = 1/nu evaluated in the
-// scale-free form (vp/V)^2/(np-1) + (vq/V)^2/(nq-1), and the critical t2(u).
-__device__ __forceinline__ void welch_fast(const TCritPoly &tp, float mp, float vp, float rp, float mq, float vq,
-                                           float rq, float &t2, float &crit, float &u) {
+// Decision margins of one FP32 Welch test, t2 = dm^2/V against crit(u) with
+// u = 1/nu in the scale-free form (vp/V)^2/(np-1) + (vq/V)^2/(nq-1):
+//   e > 0: t2 > (1 + band) crit(u), u <= umax -> rejected beyond doubt;
+//   f < 0: t2 < (1 - band) crit(u)            -> accepted beyond doubt
+// (crit grows with u, so f < 0 stays valid for u > umax).  A NaN mean
+// (abstain) or V = 0 with equal means gives NaN margins, which the max
+// reductions skip (accept); V = 0 with unequal means gives t2 = inf, e = inf.
+// SMALLN: some n - 1 of the window may be below 1/umax; otherwise u <= umax
+// holds for every pair and the range term is left out.
+template <bool SMALLN>
+__device__ __forceinline__ void welch_margin(const TCritPoly &tp, float mp, float vp, float rp, float mq, float vq,
+                                             float rq, float &e, float &f) {
     const float V = vp + vq;
-    const float dm = mp - mq;
     const float iV = __builtin_amdgcn_rcpf(V);
+    const float dm = mp - mq;
+    const float t2 = (dm * iV) * dm;
     const float ap = vp * iV, aq = vq * iV;
-    u = __builtin_fmaf(ap * ap, rp, aq * aq * rq);
-    t2 = (dm * iV) * dm;
-    crit = tcrit_eval(tp, u);
+    const float u = __builtin_fmaf(ap * ap, rp, (aq * aq) * rq);
+    const float uc = fminf(u, tp.umax);  // NaN (V = 0) -> umax
+    float crit = tp.c[kTCritDeg];
+#pragma unroll
+    for (int k = kTCritDeg - 1; k >= 0; --k) crit = __builtin_fmaf(crit, uc, tp.c[k]);
+    e = __builtin_fmaf(crit, tp.neg_hi, t2);
+    f = __builtin_fmaf(crit, tp.neg_lo, t2);
+    if (SMALLN) e = __builtin_fmaf(-kUBig, fmaxf(u - tp.umax, 0.f), e);
 }
 
 // Per-pixel test operands: M (mean; NaN = abstain) and V = var/n (discr).
@@ -496,25 +504,130 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
     }
 }
 
-template <int S, bool HASW, bool PLAIN, int HWC, int TY>
+// Below this n - 1 a pair may have nu < 1/umax (u <= max(1/(np-1), 1/(nq-1))).
+constexpr int kWelchLargeNm1 = 17;
+
+// Whether every pixel of the tile's window region with n >= 2 has
+// n - 1 >= kWelchLargeNm1 (block-uniform).
+__device__ __forceinline__ bool welch_window_large_n(const WelchArgs &a, int x0, int y0, int TYv) {
+    const int xa = max(x0 - a.r, 0), xb = min(x0 + TX + a.r, a.W);
+    const int ya = max(y0 - a.r, 0), yb = min(y0 + TYv + a.r, a.H);
+    const int w = xb - xa, cnt = w * (yb - ya);
+    int small = 0;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int yy = ya + i / w, xx = xa + i % w;
+        const int n = a.n[(size_t)yy * a.ld + xx];
+        small |= (n >= 2 && n - 1 < kWelchLargeNm1) ? 1 : 0;
+    }
+    return __syncthreads_or(small) == 0;
+}
+
+// One window row of the Welch filter for the pixel of this lane.
+// MODE3: only the set of minimum Jarque-Bera statistic (per pixel) is tested.
+template <int S, bool HASW, bool PLAIN, bool MODE3, bool SMALLN, int HWC>
+__device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot, int tx, int x, int y, int dy,
+                                          const float (&Mp)[MODE3 ? 1 : S][3], const float (&Vp)[MODE3 ? 1 : S][3],
+                                          const float (&gp)[kNF], float okp, float nm1p, float rp, int bset,
+                                          float (&den)[3], float (&num)[3], float (&numV)[3]) {
+    using LY = WelchLayout<S, HASW, PLAIN>;
+    constexpr int NT = MODE3 ? 1 : S;  // sets tested per pair
+    const int win = 2 * a.r + 1;
+    const float spy = a.sp_scale * (float)(dy * dy);
+    for (int dx = -a.r; dx <= a.r; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const float *q = slot + tx + a.r + dx;
+        const float okq = q[0];
+        const float nm1q = q[HWC];
+        const float rq = __builtin_amdgcn_rcpf(nm1q);
+        const float okpq = okq * okp;
+        float w = okpq;
+        if (HASW) {
+            float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+            for (int f = 0; f < kNF; ++f) {
+                const float d = gp[f] - q[(LY::FEAT + f) * HWC];
+                lw = __builtin_fmaf(-d, d, lw);
+            }
+            w *= __builtin_amdgcn_exp2f(lw);
+        }
+        // Operands of set s, channel c of q: pq[(s*3 + c) * HWC]; MODE3 reads
+        // the lane's own set.
+        const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx + (MODE3 ? bset * 3 * HWC : 0);
+        float E[3], F[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float2 mv = pq[(s * 3 + c) * HWC];
+                float e, f;
+                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv.x, mv.y, rq, e, f);
+                E[c] = fmaxf(E[c], e);
+                F[c] = fmaxf(F[c], f);
+            }
+        // A channel neither rejected nor accepted beyond doubt by its fast
+        // tests is decided by the exact tails of its tests.
+        bool near = false;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) near |= !(E[c] > 0.f) && F[c] >= 0.f;
+        if (near && okpq > 0.f) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (E[c] > 0.f || F[c] < 0.f) continue;
+                bool pass = true;
+                for (int s = 0; s < NT; ++s) {
+                    const float2 mv = pq[(s * 3 + c) * HWC];
+                    float e, f;
+                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv.x, mv.y, rq, e, f);
+                    if (e > 0.f) { pass = false; break; }
+                    if (f >= 0.f && !welch_exact(Mp[s][c], Vp[s][c], nm1p, mv.x, mv.y, nm1q, a.alpha)) {
+                        pass = false;
+                        break;
+                    }
+                }
+                E[c] = pass ? -1.f : 1.f;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float wa = E[c] > 0.f ? 0.f : w;
+            den[c] += wa;
+            num[c] = __builtin_fmaf(wa, q[(LY::FILM + c) * HWC], num[c]);
+            if (PLAIN) numV[c] = __builtin_fmaf(wa, q[(LY::S2 + c) * HWC], numV[c]);
+        }
+        if (a.dec) {
+            const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
+            const int xq = x + dx, yq = y + dy;
+            const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+            for (int c = 0; c < 3; ++c) a.dec[kk + c] = (in && okpq > 0.f && !(E[c] > 0.f)) ? 1 : 0;
+        }
+    }
+}
+
+template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC, int TY>
 __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, float *ring) {
     using LY = WelchLayout<S, HASW, PLAIN>;
     constexpr int SLOT = LY::FLOATS * HWC;
+    constexpr int NT = MODE3 ? 1 : S;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int x0 = bx * TX, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < a.W && y < a.H;
-    const int win = 2 * a.r + 1;
 
-    float Mp[S][3], Vp[S][3];
+    float Mp[NT][3], Vp[NT][3];
     float gp[kNF];
     float okp = 0.f, nm1p = 1.f, rp = 0.f;
     float den[3], num[3], numV[3];
-    bool use[S];
+    int bset = 0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) use[s] = true;
+    for (int s = 0; s < NT; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Mp[s][c] = Vp[s][c] = 0.f;
 #pragma unroll
     for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) den[c] = 1.f, num[c] = numV[c] = 0.f;
     if (active) {
         const size_t pix = (size_t)y * a.ld + x;
         const int n = a.n[pix];
@@ -524,114 +637,47 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
         if (HASW)
 #pragma unroll
             for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+        if (MODE3) {
+            float bj = a.jb[pix];
+            for (int s = 1; s < S; ++s) {
+                const float j = a.jb[(size_t)s * a.plane + pix];
+                if (j < bj) { bj = j; bset = s; }
+            }
+        }
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
-                a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
+                float M, V;
+                welch_load(a, x, y, s, c, M, V);
+                a.discr[((size_t)s * 3 + c) * a.plane + pix] = V;
+                if (!MODE3) Mp[s][c] = M, Vp[s][c] = V;
+                else if (s == bset) Mp[0][c] = M, Vp[0][c] = V;
             }
-        if (a.mode3) {
-            int best = 0;
-            float bj = a.jb[pix];
-            for (int s = 1; s < S; ++s) {
-                const float j = a.jb[(size_t)s * a.plane + pix];
-                if (j < bj) { bj = j; best = s; }
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) use[s] = s == best;
-        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            den[c] = 1.f;
             num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
-            numV[c] = 0.f;
             if (PLAIN) numV[c] = n >= 2 ? a.var[c * a.plane + pix] / (float)(n - 1) : 0.f;
         }
         if (a.dec) {
+            const int win = 2 * a.r + 1;
             const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
             a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
         }
     }
+    const bool large_n = welch_window_large_n(a, x0, y0, TY);
 
     for (int t = 0; t < TY; ++t) welch_stage_row<S, HASW, PLAIN, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
     __syncthreads();
     for (int k = 0; k <= 2 * a.r; ++k) {
-        const int dy = k - a.r;
         const float *slot = ring + ((ty + k) % TY) * SLOT;
         if (active) {
-            const float spy = a.sp_scale * (float)(dy * dy);
-            for (int dx = -a.r; dx <= a.r; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                const float *q = slot + tx + a.r + dx;
-                const float okq = q[0];
-                const float nm1q = q[HWC];
-                const float rq = __builtin_amdgcn_rcpf(nm1q);
-                const float okpq = okq * okp;
-                float w = okpq;
-                if (HASW) {
-                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
-#pragma unroll
-                    for (int f = 0; f < kNF; ++f) {
-                        const float d = gp[f] - q[(LY::FEAT + f) * HWC];
-                        lw = __builtin_fmaf(-d, d, lw);
-                    }
-                    w *= __builtin_amdgcn_exp2f(lw);
-                }
-                const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx;
-                // All fast tests first, branch-free, so the LDS reads of every
-                // set are issued together and every comparison mask dies at
-                // once; if any test of the pair lies in the band around the
-                // critical value, the pair is re-evaluated with exact tails.
-                float2 mv[S][3];
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) mv[s][c] = pq[(s * 3 + c) * HWC];
-                float accf[3] = {okpq, okpq, okpq};
-                float nearf = 0.f;
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        float t2, crit, u;
-                        welch_fast(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, t2, crit, u);
-                        accf[c] = ((t2 > crit || t2 == INFINITY) && use[s]) ? 0.f : accf[c];
-                        nearf = ((fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax) && use[s]) ? 1.f : nearf;
-                    }
-                if (nearf > 0.f && okpq > 0.f) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) accf[c] = 1.f;
-#pragma unroll
-                    for (int s = 0; s < S; ++s)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            if (!use[s]) continue;
-                            float t2, crit, u;
-                            welch_fast(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, t2, crit, u);
-                            bool pass = !(t2 > crit || t2 == INFINITY);
-                            if (fabsf(t2 - crit) <= a.tp.band * crit || u > a.tp.umax)
-                                pass = welch_exact(Mp[s][c], Vp[s][c], nm1p, mv[s][c].x, mv[s][c].y, nm1q, a.alpha);
-                            if (!pass) accf[c] = 0.f;
-                        }
-                }
-                bool acc[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) acc[c] = accf[c] > 0.f;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float wa = acc[c] ? w : 0.f;
-                    den[c] += wa;
-                    num[c] = __builtin_fmaf(wa, q[(LY::FILM + c) * HWC], num[c]);
-                    if (PLAIN) numV[c] = __builtin_fmaf(wa, q[(LY::S2 + c) * HWC], numV[c]);
-                }
-                if (a.dec) {
-                    const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
-                    const int xq = x + dx, yq = y + dy;
-                    const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
-                    for (int c = 0; c < 3; ++c) a.dec[kk + c] = (in && acc[c]) ? 1 : 0;
-                }
-            }
+            if (large_n)
+                welch_row<S, HASW, PLAIN, MODE3, false, HWC>(a, slot, tx, x, y, k - a.r, Mp, Vp, gp, okp, nm1p, rp,
+                                                             bset, den, num, numV);
+            else
+                welch_row<S, HASW, PLAIN, MODE3, true, HWC>(a, slot, tx, x, y, k - a.r, Mp, Vp, gp, okp, nm1p, rp,
+                                                            bset, den, num, numV);
         }
         if (k == 2 * a.r) break;
         __syncthreads();
@@ -646,10 +692,10 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
     }
 }
 
-template <int S, bool HASW, bool PLAIN, int HWC>
+template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
 __global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
     extern __shared__ float ring[];
-    welch_tile<S, HASW, PLAIN, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
+    welch_tile<S, HASW, PLAIN, MODE3, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
 }
 
 template <int S, bool HASW, int HWC, int TY>
@@ -664,41 +710,67 @@ __global__ void __launch_bounds__(64 * TY) k_welch_batch(const WelchArgs *args, 
         const int t = w - prefix[img];
         const int tiles_x = (a.W + TX - 1) / TX;
         const int by = t / tiles_x, bx = t - by * tiles_x;
-        welch_tile<S, HASW, false, HWC, TY>(a, bx, by, ring);
+        welch_tile<S, HASW, false, false, HWC, TY>(a, bx, by, ring);
         __syncthreads();  // the next tile restages the ring
     }
 }
 
-template <int S, bool HASW, bool PLAIN, int HWC>
-static int welch_launch4(const WelchArgs &a, dim3 grid, hipStream_t st) {
+template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
+static int welch_launch5(const WelchArgs &a, dim3 grid, hipStream_t st) {
     const size_t lds = (size_t)kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
-    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, HWC>,
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, MODE3, HWC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, HWC>), grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, MODE3, HWC>), grid, dim3(256), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
-template <int S, bool HASW, bool PLAIN>
+template <int S, bool HASW, bool PLAIN, bool MODE3>
 static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
-    return a.r <= 10 ? welch_launch4<S, HASW, PLAIN, TX + 20>(a, grid, st)
-                     : welch_launch4<S, HASW, PLAIN, TX + 40>(a, grid, st);
+    return a.r <= 10 ? welch_launch5<S, HASW, PLAIN, MODE3, TX + 20>(a, grid, st)
+                     : welch_launch5<S, HASW, PLAIN, MODE3, TX + 40>(a, grid, st);
 }
 
 template <int S>
 static int welch_launch2(const WelchArgs &a, bool hasw, dim3 grid, hipStream_t st) {
-    return hasw ? welch_launch3<S, true, false>(a, grid, st) : welch_launch3<S, false, false>(a, grid, st);
+    if (S > 1 && a.mode3)
+        return hasw ? welch_launch3<S, true, false, S != 1>(a, grid, st) : welch_launch3<S, false, false, S != 1>(a, grid, st);
+    return hasw ? welch_launch3<S, true, false, false>(a, grid, st) : welch_launch3<S, false, false, false>(a, grid, st);
 }
 
 static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, bool plain,
-                            WelchArgs &a);
+                            WelchArgs &a) {
+    a.n = bs->n;
+    a.mean = bs->stat(0, ST_MEAN, 0);
+    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    if (plain || !p->cascade) {
+        a.var = bs->stat(0, ST_M2, 0);
+        a.var_is_m2 = 1;
+    } else {
+        a.var = p->use_levc ? bs->varCD : bs->varD;
+        a.var_is_m2 = 0;
+    }
+    a.film = bs->stat(0, ST_MEAN, 0);
+    a.film_pitch = (int64_t)bs->ld * 4;
+    a.film_cs = (int64_t)bs->plane * 4;
+    a.out = plain ? bs->meanD : bs->filmD;
+    a.out_pitch = (int64_t)bs->ld * 4;
+    a.out_cs = (int64_t)bs->plane * 4;
+    a.jb = bs->jb;
+    a.discr = bs->discr;
+    a.varW = bs->varD;
+    a.f = plain ? FeatArgs{} : make_feats(g);
+    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
+    a.mode3 = (!plain && p->mt_mode == 3 && bs->S > 1) ? 1 : 0;
+    a.plane = bs->plane;
+    a.sp_scale = plain ? 0.f : p->ds_factor * kLog2e;
+    a.alpha = (double)p->alpha;
+    a.tp = tcrit_for(bs->ctx, (double)p->alpha);
+}
 
 WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g) {
     WelchArgs a{};
     fill_welch_args(bs, p, g, false, a);
-    a.film = bs->stat(0, ST_MEAN, 0);
-    a.film_pitch = (int64_t)bs->ld * 4;
-    a.film_cs = (int64_t)bs->plane * 4;
     return a;
 }
 
@@ -741,82 +813,29 @@ int launch_welch_batch(int S, bool hasw, int r, int ty, const WelchArgs *args, c
     return STATER_OK;
 }
 
-static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, bool plain,
-                            WelchArgs &a) {
-    a.n = bs->n;
-    a.mean = bs->stat(0, ST_MEAN, 0);
-    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
-    if (plain || !p->cascade) {
-        a.var = bs->stat(0, ST_M2, 0);
-        a.var_is_m2 = 1;
-    } else {
-        a.var = p->use_levc ? bs->varCD : bs->varD;
-        a.var_is_m2 = 0;
-    }
-    a.out = plain ? bs->meanD : bs->filmD;
-    a.out_pitch = (int64_t)bs->ld * 4;
-    a.out_cs = (int64_t)bs->plane * 4;
-    a.jb = bs->jb;
-    a.discr = bs->discr;
-    a.varW = bs->varD;
-    a.f = plain ? FeatArgs{} : make_feats(g);
-    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
-    a.mode3 = (!plain && p->mt_mode == 3 && bs->S > 1) ? 1 : 0;
-    a.plane = bs->plane;
-    a.sp_scale = plain ? 0.f : p->ds_factor * kLog2e;
-    a.alpha = (double)p->alpha;
-    a.tp = tcrit_for(bs->ctx, (double)p->alpha);
-}
-
 static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                         const stater_plane *film, const stater_plane *film_out, bool plain, uint8_t *dec,
                         hipStream_t st) {
     WelchArgs a{};
-    a.n = bs->n;
-    a.mean = bs->stat(0, ST_MEAN, 0);
-    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
-    if (plain || !p->cascade) {
-        a.var = bs->stat(0, ST_M2, 0);
-        a.var_is_m2 = 1;
-    } else {
-        a.var = p->use_levc ? bs->varCD : bs->varD;
-        a.var_is_m2 = 0;
-    }
+    fill_welch_args(bs, p, g, plain, a);
     if (film) {
         if (film->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film must be planar");
         a.film = (const float *)film->ptr;
         a.film_pitch = film->pitch_bytes;
         a.film_cs = film->channel_stride_bytes;
-    } else {
-        a.film = bs->stat(0, ST_MEAN, 0);
-        a.film_pitch = (int64_t)bs->ld * 4;
-        a.film_cs = (int64_t)bs->plane * 4;
     }
-    a.out = plain ? bs->meanD : bs->filmD;
-    a.out_pitch = (int64_t)bs->ld * 4;
-    a.out_cs = (int64_t)bs->plane * 4;
     if (film_out) {
         if (film_out->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film_out must be planar");
         a.out = (float *)film_out->ptr;
         a.out_pitch = film_out->pitch_bytes;
         a.out_cs = film_out->channel_stride_bytes;
     }
-    a.jb = bs->jb;
-    a.discr = bs->discr;
-    a.varW = bs->varD;
     a.dec = dec;
-    a.f = plain ? FeatArgs{} : make_feats(g);
-    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
-    a.mode3 = (!plain && p->mt_mode == 3 && bs->S > 1) ? 1 : 0;
-    a.plane = bs->plane;
-    a.sp_scale = plain ? 0.f : p->ds_factor * kLog2e;
-    a.alpha = (double)p->alpha;
-    a.tp = tcrit_for(bs->ctx, (double)p->alpha);
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
     dim3 grid((bs->W + TX - 1) / TX, (bs->H + kTY - 1) / kTY);
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
-    if (plain) return welch_launch3<1, false, true>(a, grid, st);
+    if (plain) return welch_launch3<1, false, true, false>(a, grid, st);
     switch (S) {
         case 1: return welch_launch2<1>(a, hasw, grid, st);
         case 2: return welch_launch2<2>(a, hasw, grid, st);

@@ -51,16 +51,18 @@ __host__ __device__ inline double t2_tail(double t2, double nu) {
     return betainc_reg(0.5 * nu, 0.5, nu / (nu + t2));
 }
 
-// Number of polynomial coefficients of the critical-value fit.
-constexpr int kTCritDeg = 5;
+// Degree of the critical-value fit.
+constexpr int kTCritDeg = 4;
 
-// Critical t^2 of the two-sided level-alpha test as a polynomial in
-// s = u*scale + shift (u = 1/nu in [0, umax]), valid for nu >= 1/umax.
+// Critical t^2 of the two-sided level-alpha test as a polynomial in u = 1/nu
+// on [0, umax], i.e. for nu >= 1/umax.  A fast test t2 > crit is trusted
+// outside t2 in [(1 - band) crit, (1 + band) crit]; inside, and for u > umax,
+// the decision takes the exact tail.
 struct TCritPoly {
-    float c[kTCritDeg + 1];  // monomial coefficients in s, c[0] constant
-    float scale, shift;
-    float umax;              // polynomial domain; larger u takes the exact tail
+    float c[kTCritDeg + 1];  // monomial coefficients in u, c[0] = z^2
+    float umax;
     float band;              // relative half-width of the exact-tail band
+    float neg_hi, neg_lo;    // -(1 + band), -(1 - band)
     double alpha;
 };
 
