@@ -142,6 +142,18 @@ int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
 // a compile-time row stride so every LDS access is base + immediate offset.
 constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
 
+// Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
+// rounded up.  The filter kernels size their dynamic LDS at compile time, so
+// the compiler is told this occupancy and schedules for latency hiding
+// (LDS reads issued ahead) instead of for occupancy it can never reach.
+constexpr int lds_waves_per_simd(int lds_bytes, int threads) {
+    const int blocks = (160 * 1024) / lds_bytes;
+    const int w = (blocks * (threads / 64) + 3) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+template <int G, bool HASW, int HWC, int TY>
+constexpr int var_lds_bytes() { return TY * ((1 + (HASW ? kNF : 0)) + 12 * G) * HWC * 4; }
+
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
@@ -228,33 +240,45 @@ __device__ __forceinline__ void var_tile(const VarArgs a, int bx, int by, float 
         const float *slot = ring + ((ty + k) % TY) * SLOT;
         if (active && okp > 0.f) {
             const float spy = a.sp_scale * (float)(dy * dy);
-#pragma unroll 2
             for (int dx = -a.r; dx <= a.r; ++dx) {
                 if (dx == 0 && dy == 0) continue;
                 const float *q = slot + tx + a.r + dx;
-                float w = q[0];
-                if (HASW) {
-                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
-#pragma unroll
-                    for (int f = 0; f < kNF; ++f) {
-                        const float d = gp[f] - q[(1 + f) * HWC];
-                        lw = __builtin_fmaf(-d, d, lw);
-                    }
-                    w *= __builtin_amdgcn_exp2f(lw);
-                }
                 const float2 *pq = (const float2 *)(slot + SINGLES * HWC) + tx + a.r + dx;
+                // Every LDS read of the pair is issued before any use, so the
+                // wave waits once per pair rather than once per test.
+                float fq[kNF];
+                float2 lz[G][3], vc[G][3];
+                const float okq = q[0];
+#pragma unroll
+                for (int f = 0; f < (HASW ? kNF : 0); ++f) fq[f] = q[(1 + f) * HWC];
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const float2 lz = pq[((g * 3 + c) * 2) * HWC];
-                        const float2 vc = pq[((g * 3 + c) * 2 + 1) * HWC];
-                        const float dl = Lp[g][c] - lz.x;
-                        // NaN/inf operands abstain (accept)
-                        const float wa = (dl * dl > Zp[g][c] + lz.y) ? 0.f : w;
+                        lz[g][c] = pq[((g * 3 + c) * 2) * HWC];
+                        vc[g][c] = pq[((g * 3 + c) * 2 + 1) * HWC];
+                    }
+                __builtin_amdgcn_sched_barrier(0);
+                float w = okq;
+                if (HASW) {
+                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+                    for (int f = 0; f < kNF; ++f) {
+                        const float d = gp[f] - fq[f];
+                        lw = __builtin_fmaf(-d, d, lw);
+                    }
+                    w *= __builtin_amdgcn_exp2f(lw);
+                }
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float dl = Lp[g][c] - lz[g][c].x;
+                        // reject iff dl^2 - z^2 se2_q > z^2 se2_p; NaN/inf operands abstain (accept)
+                        const float wa = (__builtin_fmaf(dl, dl, -lz[g][c].y) > Zp[g][c]) ? 0.f : w;
                         den[g][c] += wa;
-                        num[g][c] = __builtin_fmaf(wa, vc.x, num[g][c]);
-                        numC[g][c] = __builtin_fmaf(wa, vc.y, numC[g][c]);
+                        num[g][c] = __builtin_fmaf(wa, vc[g][c].x, num[g][c]);
+                        numC[g][c] = __builtin_fmaf(wa, vc[g][c].y, numC[g][c]);
                     }
             }
         }
@@ -276,13 +300,17 @@ __device__ __forceinline__ void var_tile(const VarArgs a, int bx, int by, float 
 }
 
 template <int G, bool HASW, int HWC>
-__global__ void __launch_bounds__(256) k_var_filter(VarArgs a) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, kTY>(), 256))))
+    k_var_filter(VarArgs a) {
     extern __shared__ float ring[];
     var_tile<G, HASW, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
 }
 
 template <int G, bool HASW, int HWC, int TY>
-__global__ void __launch_bounds__(64 * TY) k_var_filter_batch(const VarArgs *args, const int *prefix, int nimg) {
+__global__ void __launch_bounds__(64 * TY)
+    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY>(), 64 * TY))))
+    k_var_filter_batch(const VarArgs *args, const int *prefix, int nimg) {
     extern __shared__ float ring[];
     const int ntiles = prefix[nimg];
     XcdRange r = xcd_range(ntiles);
@@ -536,8 +564,23 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
     for (int dx = -a.r; dx <= a.r; ++dx) {
         if (dx == 0 && dy == 0) continue;
         const float *q = slot + tx + a.r + dx;
+        // Operands of set s, channel c of q: pq[(s*3 + c) * HWC]; MODE3 reads
+        // the lane's own set.  Every LDS read of the pair is issued before any
+        // use, so the wave waits once per pair rather than once per test.
+        const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx + (MODE3 ? bset * 3 * HWC : 0);
+        float fq[kNF], filmq[3];
+        float2 mv[NT][3];
         const float okq = q[0];
         const float nm1q = q[HWC];
+#pragma unroll
+        for (int f = 0; f < (HASW ? kNF : 0); ++f) fq[f] = q[(LY::FEAT + f) * HWC];
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[s][c] = pq[(s * 3 + c) * HWC];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) filmq[c] = q[(LY::FILM + c) * HWC];
+        __builtin_amdgcn_sched_barrier(0);
         const float rq = __builtin_amdgcn_rcpf(nm1q);
         const float okpq = okq * okp;
         float w = okpq;
@@ -545,14 +588,11 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
             float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
 #pragma unroll
             for (int f = 0; f < kNF; ++f) {
-                const float d = gp[f] - q[(LY::FEAT + f) * HWC];
+                const float d = gp[f] - fq[f];
                 lw = __builtin_fmaf(-d, d, lw);
             }
             w *= __builtin_amdgcn_exp2f(lw);
         }
-        // Operands of set s, channel c of q: pq[(s*3 + c) * HWC]; MODE3 reads
-        // the lane's own set.
-        const float2 *pq = (const float2 *)(slot + LY::SINGLES * HWC) + tx + a.r + dx + (MODE3 ? bset * 3 * HWC : 0);
         float E[3], F[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
@@ -560,9 +600,8 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
         for (int s = 0; s < NT; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float2 mv = pq[(s * 3 + c) * HWC];
                 float e, f;
-                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv.x, mv.y, rq, e, f);
+                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, e, f);
                 E[c] = fmaxf(E[c], e);
                 F[c] = fmaxf(F[c], f);
             }
@@ -593,7 +632,7 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
         for (int c = 0; c < 3; ++c) {
             const float wa = E[c] > 0.f ? 0.f : w;
             den[c] += wa;
-            num[c] = __builtin_fmaf(wa, q[(LY::FILM + c) * HWC], num[c]);
+            num[c] = __builtin_fmaf(wa, filmq[c], num[c]);
             if (PLAIN) numV[c] = __builtin_fmaf(wa, q[(LY::S2 + c) * HWC], numV[c]);
         }
         if (a.dec) {
@@ -693,13 +732,16 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
 }
 
 template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
-__global__ void __launch_bounds__(256) k_welch(WelchArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4, 256)))) k_welch(WelchArgs a) {
     extern __shared__ float ring[];
     welch_tile<S, HASW, PLAIN, MODE3, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
 }
 
 template <int S, bool HASW, int HWC, int TY>
-__global__ void __launch_bounds__(64 * TY) k_welch_batch(const WelchArgs *args, const int *prefix, int nimg) {
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(TY * WelchLayout<S, HASW, false>::FLOATS * HWC * 4, 64 * TY))))
+    k_welch_batch(const WelchArgs *args, const int *prefix, int nimg) {
     extern __shared__ float ring[];
     const int ntiles = prefix[nimg];
     XcdRange r = xcd_range(ntiles);
