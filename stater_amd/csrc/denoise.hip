@@ -437,8 +437,18 @@ __device__ __noinline__ bool welch_exact(float mp, float vp, float nm1p, float m
 
 constexpr float kUBig = 281474976710656.f;  // 2^48: pushes e below zero for u > umax
 
-// Decision margins of one FP32 Welch test, t2 = dm^2/V against crit(u) with
-// u = 1/nu in the scale-free form (vp/V)^2/(np-1) + (vq/V)^2/(nq-1):
+// Per-pair factors of u = 1/nu.  With ap = vp/V and aq = 1 - ap,
+// u = rp ap^2 + rq aq^2 = (A ap + B) ap + C, A = rp + rq, B = -2 rq, C = rq
+// (r = 1/(n-1)).  Its absolute rounding error stays below ~4 rq ulp(1),
+// negligible against the band for rq <= 1/17; windows holding a smaller n take
+// the scale-free form with aq = vq/V.
+struct PairR {
+    float rp, rq, A, B, C;
+};
+
+__device__ __forceinline__ PairR pair_r(float rp, float rq) { return PairR{rp, rq, rp + rq, -2.f * rq, rq}; }
+
+// Decision margins of one FP32 Welch test, t2 = dm^2/V against crit(u):
 //   e > 0: t2 > (1 + band) crit(u), u <= umax -> rejected beyond doubt;
 //   f < 0: t2 < (1 - band) crit(u)            -> accepted beyond doubt
 // (crit grows with u, so f < 0 stays valid for u > umax).  A NaN mean
@@ -447,14 +457,20 @@ constexpr float kUBig = 281474976710656.f;  // 2^48: pushes e below zero for u >
 // SMALLN: some n - 1 of the window may be below 1/umax; otherwise u <= umax
 // holds for every pair and the range term is left out.
 template <bool SMALLN>
-__device__ __forceinline__ void welch_margin(const TCritPoly &tp, float mp, float vp, float rp, float mq, float vq,
-                                             float rq, float &e, float &f) {
+__device__ __forceinline__ void welch_margin(const TCritPoly &tp, float mp, float vp, float mq, float vq,
+                                             const PairR &r, float &e, float &f) {
     const float V = vp + vq;
     const float iV = __builtin_amdgcn_rcpf(V);
     const float dm = mp - mq;
     const float t2 = (dm * iV) * dm;
-    const float ap = vp * iV, aq = vq * iV;
-    const float u = __builtin_fmaf(ap * ap, rp, (aq * aq) * rq);
+    const float ap = vp * iV;
+    float u;
+    if (SMALLN) {
+        const float aq = vq * iV;
+        u = __builtin_fmaf(ap * ap, r.rp, (aq * aq) * r.rq);
+    } else {
+        u = __builtin_fmaf(__builtin_fmaf(r.A, ap, r.B), ap, r.C);
+    }
     const float uc = fminf(u, tp.umax);  // NaN (V = 0) -> umax
     float crit = tp.c[kTCritDeg];
 #pragma unroll
@@ -581,7 +597,7 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
 #pragma unroll
         for (int c = 0; c < 3; ++c) filmq[c] = q[(LY::FILM + c) * HWC];
         __builtin_amdgcn_sched_barrier(0);
-        const float rq = __builtin_amdgcn_rcpf(nm1q);
+        const PairR pr = pair_r(rp, __builtin_amdgcn_rcpf(nm1q));
         const float okpq = okq * okp;
         float w = okpq;
         if (HASW) {
@@ -601,7 +617,7 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 float e, f;
-                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv[s][c].x, mv[s][c].y, rq, e, f);
+                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
                 E[c] = fmaxf(E[c], e);
                 F[c] = fmaxf(F[c], f);
             }
@@ -618,7 +634,7 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
                 for (int s = 0; s < NT; ++s) {
                     const float2 mv = pq[(s * 3 + c) * HWC];
                     float e, f;
-                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], rp, mv.x, mv.y, rq, e, f);
+                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv.x, mv.y, pr, e, f);
                     if (e > 0.f) { pass = false; break; }
                     if (f >= 0.f && !welch_exact(Mp[s][c], Vp[s][c], nm1p, mv.x, mv.y, nm1q, a.alpha)) {
                         pass = false;

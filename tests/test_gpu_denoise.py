@@ -62,12 +62,27 @@ def test_plain_method_128(ctx):
     assert frac_cross < 0.25 * frac_same, (frac_cross, frac_same)
 
 
-def _mt5_setup(ctx, W=96, H=64, spp=64, seed=2):
+def _mt5_setup(ctx, W=96, H=64, spp=64, seed=2, counts=None):
+    """Accumulate spp synthetic samples per pixel (or counts[y, x] of them)."""
     sc, chunk, feats = generate(ctx, 2, seed, W, H, spp)
     bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, max_moment=5)
-    bs.accumulate(chunk.desc())
+    cplane = None
+    if counts is not None:
+        cbuf = hipmem.DeviceBuffer(W * H * 4)
+        cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+        hipmem.write_plane(cplane, counts.astype(np.int32)[None])
+    bs.accumulate(chunk.desc(counts=cplane))
     ctx.synchronize()
     return sc, chunk, feats, bs
+
+
+def _ragged_counts(W, H, seed=11):
+    """Columns x < 40 hold 0..20 samples (small n, n < 2 included), the rest 64:
+    the 64-column tile at x >= 64 sees only n = 64 in its window."""
+    rng = np.random.default_rng(seed)
+    c = np.full((H, W), 64, np.int32)
+    c[:, :40] = rng.integers(0, 21, (H, 40))
+    return c
 
 
 def test_prepass_and_variance_filter(ctx):
@@ -108,10 +123,10 @@ def test_prepass_and_variance_filter(ctx):
     assert np.max(np.abs(gCD[ok] - vCD[ok]) / np.maximum(np.abs(vCD[ok]), 1e-30)) < FILTER_RTOL
 
 
-@pytest.mark.parametrize("mt_mode", [2, 3])
-def test_welch_filter_mt5(ctx, mt_mode):
+@pytest.mark.parametrize("mt_mode,ragged", [(2, False), (3, False), (2, True), (3, True)])
+def test_welch_filter_mt5(ctx, mt_mode, ragged):
     W, H, r = 96, 64, 10
-    sc, chunk, feats, bs = _mt5_setup(ctx, W, H)
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, counts=_ragged_counts(W, H) if ragged else None)
     p = sa.default_params(radius=r, mt_mode=mt_mode)
     g = feats.gbuffers()
     bs.denoise_vars(p, g)
@@ -134,7 +149,10 @@ def test_welch_filter_mt5(ctx, mt_mode):
     filmD = hipmem.read_plane(bs.view("filmD"))
     np.testing.assert_allclose(filmD[same], ref["filmD"][same], rtol=FILTER_RTOL)
     discr = read_set_buf(bs, MT5_LAMBDAS, "discr")
-    np.testing.assert_array_equal(discr, (varCD / n.astype(np.float32)).astype(np.float32))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        want = np.where(n >= 1, varCD / n.astype(np.float32), np.float32(0)).astype(np.float32)
+    want[~np.isfinite(want)] = 0
+    np.testing.assert_array_equal(discr, want)
 
 
 def test_mean_vars_bitexact(ctx):
