@@ -23,6 +23,8 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libstater.so"
 
 STATER_OK = 0
+(STATER_E_ARG, STATER_E_HIP, STATER_E_NOMEM, STATER_E_STATE, STATER_E_UNSUPPORTED,
+ STATER_E_NOTFOUND) = (-1, -2, -3, -4, -5, -6)
 STATER_F32, STATER_I32 = 0, 1
 STATER_PLANAR, STATER_INTERLEAVED = 0, 1
 STATER_CTX_TIMING = 1
