@@ -71,4 +71,15 @@ __device__ __forceinline__ XcdRange xcd_range(int ntiles) {
     return XcdRange{b + l, e, per};
 }
 
+// Static XCD-aware placement of a one-tile-per-workgroup grid of
+// xcd_grid(ntiles) workgroups: workgroup i runs on XCD i % 8 and takes the
+// (i / 8)-th tile of that XCD's contiguous chunk, so the tiles that share halo
+// rows are read through one L2.  Returns -1 for the few padding workgroups.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    return t < ntiles ? t : -1;
+}
+inline unsigned xcd_grid(int ntiles) { return (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds); }
+
 }  // namespace stater

@@ -304,7 +304,10 @@ __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, kTY>(), 256))))
     k_var_filter(VarArgs a) {
     extern __shared__ float ring[];
-    var_tile<G, HASW, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + kTY - 1) / kTY));
+    if (t < 0) return;
+    var_tile<G, HASW, HWC, kTY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
 template <int G, bool HASW, int HWC, int TY>
@@ -407,7 +410,7 @@ int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const s
     a.z2 = p->var_z * p->var_z;
     a.sp_scale = p->ds_factor * kLog2e;
     const bool hasw = a.f.nf > 0 || p->ds_factor != 0.f;
-    dim3 grid((bs->W + TX - 1) / TX, (bs->H + kTY - 1) / kTY);
+    const dim3 grid(xcd_grid(((bs->W + TX - 1) / TX) * ((bs->H + kTY - 1) / kTY)));
     LaunchTimer t(bs->ctx, st, "var_filter");
     for (int g0 = 0; g0 < bs->S;) {
         const int left = bs->S - g0;
@@ -751,7 +754,10 @@ template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     1, lds_waves_per_simd(kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4, 256)))) k_welch(WelchArgs a) {
     extern __shared__ float ring[];
-    welch_tile<S, HASW, PLAIN, MODE3, HWC, kTY>(a, blockIdx.x, blockIdx.y, ring);
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + kTY - 1) / kTY));
+    if (t < 0) return;
+    welch_tile<S, HASW, PLAIN, MODE3, HWC, kTY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
 template <int S, bool HASW, int HWC, int TY>
@@ -891,7 +897,7 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     a.dec = dec;
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
-    dim3 grid((bs->W + TX - 1) / TX, (bs->H + kTY - 1) / kTY);
+    const dim3 grid(xcd_grid(((bs->W + TX - 1) / TX) * ((bs->H + kTY - 1) / kTY)));
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
     if (plain) return welch_launch3<1, false, true, false>(a, grid, st);
     switch (S) {
