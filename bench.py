@@ -21,6 +21,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -59,8 +61,6 @@ def kernel_work(W, H, spp, S, r):
 
 
 def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
-    import numpy as np
-
     import oracle
 
     x0, y0, w, h = region
@@ -91,6 +91,133 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
     }
 
 
+def c4_images(n=48):
+    """The uneven-image batch: W ~ U[256, 2048], H ~ U[256, 1152], spp ~
+    LogU[16, 1024], T ~ U{1..5} (first T non-identity mt5 lambdas + identity),
+    per-image seeds 1000 + i."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(1000 + i)
+        W = int(rng.integers(256, 2049))
+        H = int(rng.integers(256, 1153))
+        spp = int(round(float(np.exp(rng.uniform(np.log(16), np.log(1024))))))
+        T = int(rng.integers(1, 6))
+        out.append((W, H, spp, T))
+    return out
+
+
+def bench_c4(args) -> int:
+    """48 uneven images: the plain path (one image per launch, images one after
+    another) against the batch (persistent launches over all images, tuned tile
+    heights, one hipGraph).  Both generate their samples on the device in
+    chunks inside the timed region and end with bit-identical buffers."""
+    import stater_amd as sa
+    from stater_amd.serve import job_lambdas
+    from stater_amd.workload import FEATURE_SDS, Features, SampleChunk, make_scene, synth_features, synth_samples
+
+    chunk_spp = 32
+    ctx = sa.Context(0)
+    p = sa.default_params(radius=args.radius)
+    imgs = []
+    for i, (W, H, spp, T) in enumerate(c4_images()):
+        sc = make_scene(2, 1000 + i, W, H, image_id=i)
+        feats = Features(W, H)
+        synth_features(ctx, sc, feats)
+        bs = sa.BufferSet(ctx, W, H, job_lambdas(T), 5)
+        imgs.append((bs, sc, feats, feats.gbuffers(FEATURE_SDS), spp, T))
+    P = sum(im[0].width * im[0].height for im in imgs)
+    samples = sum(im[0].width * im[0].height * im[4] for im in imgs)
+    maxW = max(im[0].width for im in imgs)
+    maxH = max(im[0].height for im in imgs)
+    log(f"c4: {len(imgs)} images, {P / 1e6:.1f} Mpx, {samples / 1e9:.2f} G samples")
+    big = SampleChunk(maxW, maxH, chunk_spp)
+
+    def plain():
+        for bs, sc, feats, g, spp, _ in imgs:
+            ch = SampleChunk(bs.width, bs.height, chunk_spp, buf=big.buf)  # the largest chunk's memory
+            bs.reset()
+            for off in range(0, spp, chunk_spp):
+                k = min(chunk_spp, spp - off)
+                synth_samples(ctx, sc, ch, off, feats, spp=k)
+                bs.accumulate(ch.desc(spp=k))
+            bs.denoise_vars(p, g)
+            bs.denoise_film(p, g)
+
+    def timed(fn, reps):
+        fn()
+        ctx.synchronize()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            t.append(time.perf_counter() - t0)
+        return min(t)
+
+    t_plain = timed(plain, max(1, args.steps // 2))
+    log(f"c4 plain: {t_plain * 1e3:.1f} ms")
+    batch = sa.Batch(ctx, [(bs, sc, feats.albedo, g, spp) for bs, sc, feats, g, spp, _ in imgs], p,
+                     chunk_spp=chunk_spp)
+    batch.tune()
+    t_batch = timed(lambda: batch.run(use_graph=True), max(1, args.steps // 2))
+    log(f"c4 batched: {t_batch * 1e3:.1f} ms; groups {batch.groups()}")
+    out = {
+        "metric": "C4 denoised Mpixel/s, 48 uneven images (batched persistent launches, hipGraph)",
+        "value": round(P / t_batch / 1e6, 3), "unit": "Mpixel/s", "n_gpus": 1, "higher_is_better": True,
+        "plain_mpixel_s": round(P / t_plain / 1e6, 3), "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "ms_batched": round(t_batch * 1e3, 2), "ms_plain": round(t_plain * 1e3, 2),
+        "gsample_s_batched": round(samples / t_batch / 1e9, 3), "mpixel": round(P / 1e6, 3),
+        "gsamples": round(samples / 1e9, 3), "dtype": "f32", "data": "synthetic, device-generated in the timed region",
+        "config": {"workload": "48 images W~U[256,2048] H~U[256,1152] spp~LogU[16,1024] T~U{1..5}, r=%d, "
+                               "chunk %d spp" % (args.radius, chunk_spp)},
+        "groups": batch.groups(),
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    ctx.close()
+    return 0
+
+
+def bench_c5(args) -> int:
+    """Served jobs (mix of 720p / 1080p / 4K, T in {1,3,5}, spp in {64,256,1024})
+    at Poisson arrivals, against the same jobs run one at a time."""
+    import stater_amd as sa
+    from stater_amd.serve import Mix, Server, run_poisson
+
+    ctx = sa.Context(0)
+    budget = 128 << 30
+    mix = Mix()
+    rates = [float(x) for x in args.rates.split(",")]
+    p = sa.default_params(radius=args.radius)
+
+    one = Server(ctx, max_bytes=budget, slots=1, params=p)
+    run_poisson(one, 1000.0, 4, mix, seed=1)  # warm-up
+    plain = run_poisson(one, 1e6, args.jobs, mix, seed=5)  # all at once, one at a time
+    one.close()
+    log(f"c5 one at a time: {plain['mpixel_s']:.1f} Mpixel/s, {plain['jobs_s']:.2f} jobs/s")
+    srv = Server(ctx, max_bytes=budget, slots=args.slots, params=p)
+    run_poisson(srv, 1000.0, 4, mix, seed=1)
+    sweep = []
+    for rate in rates:
+        r = run_poisson(srv, rate, args.jobs, mix, seed=5)
+        sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()})
+        log(f"c5 rate {rate}/s: {r['mpixel_s']:.1f} Mpixel/s p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms")
+    sat = run_poisson(srv, 1e6, args.jobs, mix, seed=5)
+    srv.close()
+    out = {
+        "metric": "C5 served Mpixel/s with p50/p99 job latency (720p/1080p/4K mix, T 1/3/5, spp 64/256/1024)",
+        "value": round(sat["mpixel_s"], 3), "unit": "Mpixel/s", "n_gpus": 1, "higher_is_better": True,
+        "slots": args.slots, "one_at_a_time_mpixel_s": round(plain["mpixel_s"], 3),
+        "speedup_vs_one_at_a_time": round(sat["mpixel_s"] / plain["mpixel_s"], 3),
+        "saturated": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items()},
+        "sweep": sweep, "dtype": "f32", "data": "synthetic, device-generated per job",
+        "config": {"workload": "Poisson arrivals, jobs %d per rate, admission budget %d GiB" % (args.jobs, budget >> 30)},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,11 +229,22 @@ def main() -> int:
     ap.add_argument("--radius", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-region", default="800,400,480,270")
+    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2",
+                    help="c2: the headline 1080p frame; c4: 48 uneven images, batched vs one image per launch; "
+                         "c5: served Poisson arrivals vs one job at a time")
+    ap.add_argument("--jobs", type=int, default=32, help="c5: jobs per arrival rate")
+    ap.add_argument("--rates", default="4,8,16", help="c5: offered arrival rates, jobs/s")
+    ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by image "
               f"and by row bands, see DESIGN.md", file=sys.stderr)
         return 2
+
+    if args.workload == "c4":
+        return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     import stater_amd as sa
     from stater_amd.workload import MT5_LAMBDAS, Features, SampleChunk, make_scene, synth_features, synth_samples
@@ -139,10 +277,13 @@ def main() -> int:
         ctx.synchronize()
         log(f"warmup step {i + 1}/{args.warmup} done")
     ctx.timing_reset()
+    lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         step()
-    ctx.synchronize()
+        ctx.synchronize()  # one job at a time: its latency is submit -> filmD ready
+        lat.append(time.perf_counter() - ts)
     t1 = time.perf_counter()
     log(f"timed {args.steps} steps in {t1 - t0:.3f}s")
     ms_step = (t1 - t0) * 1e3 / args.steps
@@ -202,6 +343,9 @@ def main() -> int:
                                f"albedo/normal/depth weights", "width": W, "height": H, "spp": spp,
                    "radius": r, "sets": S},
         "gsample_s": gsample,
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                       "what": "one 1080p job (reset + accumulate + denoise_vars + denoise_film) alone"},
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -72,7 +72,8 @@ extern "C" {
 typedef struct stater_ctx stater_ctx;
 typedef struct stater_bufset stater_bufset;
 
-typedef enum { STATER_F32 = 0, STATER_I32 = 1 } stater_dtype;
+/* STATER_BF16: bfloat16 storage (feature planes only). */
+typedef enum { STATER_F32 = 0, STATER_I32 = 1, STATER_BF16 = 2 } stater_dtype;
 typedef enum { STATER_PLANAR = 0, STATER_INTERLEAVED = 1 } stater_layout;
 
 /* A 2-D image plane in device memory.  PLANAR: channel c starts at
@@ -125,6 +126,14 @@ int  stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len,
 int stater_bufset_create(stater_ctx *ctx, int width, int height,
                          int n_lambdas, const float *lambdas, int max_moment,
                          unsigned flags, stater_bufset **out);
+/* The same set over caller memory (mem_bytes >= stater_bufset_bytes_for(...),
+ * 256-byte aligned, owned by the caller and kept until the set is destroyed):
+ * allocates nothing on the device; n and the moments are zeroed on the stream.
+ * For callers that pool device memory (the serving runtime's arena). */
+int stater_bufset_create_in(stater_ctx *ctx, int width, int height,
+                            int n_lambdas, const float *lambdas, int max_moment,
+                            unsigned flags, void *mem, uint64_t mem_bytes,
+                            void *stream, stater_bufset **out);
 int stater_bufset_destroy(stater_bufset *bs);
 /* Zero the accumulated statistics (n and every moment) on the stream. */
 int stater_bufset_reset(stater_bufset *bs, void *stream);
@@ -133,6 +142,10 @@ int stater_bufset_reset(stater_bufset *bs, void *stream);
  * "t0-b<j>-<stat>" (Box-Cox statistics of slot j), stat in
  * mean m2 m3 m4 c2X c2Y m1 jb lnS2 se2 m2C varD varCD discr meanVar meanD n. */
 int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out);
+/* Copy a plane into another of the same shape, converting the layout
+ * (PLANAR <-> INTERLEAVED, e.g. a caller's CV_32FC3 film) and the storage type
+ * (f32 <-> bf16, round to nearest even).  Feature planes may be bf16. */
+int stater_convert_plane(stater_ctx *ctx, const stater_plane *src, const stater_plane *dst, void *stream);
 /* Bytes of device memory the set holds (the admission unit for serving). */
 int64_t stater_bufset_bytes(const stater_bufset *bs);
 /* Bytes a set of this shape would hold, without allocating. */
@@ -290,6 +303,68 @@ int stater_batch_run(stater_batch *b, int use_graph, void *stream);
 int stater_batch_info(const stater_batch *b, int group, int *sets, int *images,
                       int *tile_rows_var, int *tile_rows_welch);
 int stater_batch_destroy(stater_batch *b);
+
+/* ---- serving ------------------------------------------------------------
+ * A request loop over one context: jobs (a film of any size and lambda count,
+ * one round or several adaptive-sampling rounds) are submitted from any
+ * thread; each is admitted when its device memory fits the server's budget
+ * (FIFO) and a slot is free.  Every slot is a worker with its own HIP stream,
+ * so admitted jobs run concurrently and their kernels overlap on the device;
+ * a job's statistics stay resident across its rounds.  Job memory comes from
+ * one arena allocated at creation: admission never calls hipMalloc/hipFree.
+ * A job's buffers are the same whether it runs alone or among others.
+ * Replaces nothing in the reference (it renders one film per process); it is
+ * the serving form of the statistics-and-denoise path. */
+typedef struct stater_server stater_server;
+typedef struct {
+    uint64_t max_bytes;       /* device memory for job state (the admission budget) */
+    int32_t  slots;           /* jobs in flight, one worker thread and HIP stream each */
+    int32_t  chunk_spp;       /* samples generated per pass */
+    int32_t  max_width, max_height;  /* largest film (sizes the per-slot sample chunks) */
+    int32_t  acc_mode;        /* STATER_ACC_SEQUENTIAL or STATER_ACC_MERGE64 */
+    float    feature_sds[3];  /* range sds of albedo, normal, depth (<= 0: feature unused) */
+    stater_denoise_params params;
+    int32_t  reserved[4];
+} stater_server_config;
+
+typedef struct {
+    int32_t  width, height;
+    int32_t  n_lambdas;
+    float    lambdas[STATER_MAX_LAMBDAS];
+    int32_t  spp;             /* samples per pixel of round 1, per-pixel budget of later rounds */
+    int32_t  rounds;          /* 1: one shot; > 1: adaptive-sampling rounds (calc_target_spp) */
+    uint32_t cap_spp;         /* per-pixel cap of the adaptive rounds */
+    int32_t  scene_model;     /* synthetic scene (stater_scene_init model) */
+    uint32_t seed, image_id;
+    int32_t  feature_dtype;   /* STATER_F32 or STATER_BF16 storage of the filters' features */
+    int32_t  keep;            /* 1: keep the buffers after completion (stater_server_view) */
+    int32_t  reserved[4];
+} stater_job;
+
+typedef struct {
+    uint64_t id;
+    int32_t  status;          /* STATER_OK or the status of the failing call */
+    int32_t  slot;
+    double   t_submit, t_admit, t_done;  /* seconds on the server clock */
+    uint64_t samples;         /* samples accumulated over all rounds */
+    uint64_t bytes;           /* admitted device memory */
+} stater_job_result;
+
+void stater_server_default_config(stater_server_config *cfg);
+int  stater_server_create(stater_ctx *ctx, const stater_server_config *cfg, stater_server **out);
+/* Device memory a job is admitted with (E_ARG / E_UNSUPPORTED as negative). */
+int64_t stater_server_job_bytes(const stater_server *s, const stater_job *job);
+int  stater_server_submit(stater_server *s, const stater_job *job, uint64_t *id);
+/* Blocks until the job has finished; fills res. */
+int  stater_server_wait(stater_server *s, uint64_t id, stater_job_result *res);
+/* A finished job's buffer by name (jobs submitted with keep = 1). */
+int  stater_server_view(stater_server *s, uint64_t id, const char *name, stater_plane *out);
+/* Frees a kept job's buffers. */
+int  stater_server_release(stater_server *s, uint64_t id);
+/* Blocks until every submitted job has finished. */
+int  stater_server_drain(stater_server *s);
+double stater_server_clock(const stater_server *s);
+int  stater_server_destroy(stater_server *s);
 
 /* ---- testing ------------------------------------------------------------
  * Runs the Welch mean filter (plain != 0: the plain method) and also writes

@@ -25,7 +25,7 @@ LIB_PATH = _PKG / "libstater.so"
 STATER_OK = 0
 (STATER_E_ARG, STATER_E_HIP, STATER_E_NOMEM, STATER_E_STATE, STATER_E_UNSUPPORTED,
  STATER_E_NOTFOUND) = (-1, -2, -3, -4, -5, -6)
-STATER_F32, STATER_I32 = 0, 1
+STATER_F32, STATER_I32, STATER_BF16 = 0, 1, 2
 STATER_PLANAR, STATER_INTERLEAVED = 0, 1
 STATER_CTX_TIMING = 1
 STATER_BUFSET_NO_DERIVED = 1
@@ -61,6 +61,24 @@ class BatchImage(C.Structure):
                 ("spp", C.c_int32), ("reserved", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_bytes", C.c_uint64), ("slots", C.c_int32), ("chunk_spp", C.c_int32),
+                ("max_width", C.c_int32), ("max_height", C.c_int32), ("acc_mode", C.c_int32),
+                ("feature_sds", C.c_float * 3), ("params", DenoiseParams), ("reserved", C.c_int32 * 4)]
+
+
+class Job(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("n_lambdas", C.c_int32),
+                ("lambdas", C.c_float * MAX_LAMBDAS), ("spp", C.c_int32), ("rounds", C.c_int32),
+                ("cap_spp", C.c_uint32), ("scene_model", C.c_int32), ("seed", C.c_uint32), ("image_id", C.c_uint32),
+                ("feature_dtype", C.c_int32), ("keep", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("slot", C.c_int32), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("samples", C.c_uint64), ("bytes", C.c_uint64)]
+
+
 class Scene(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("seed", C.c_uint32), ("image_id", C.c_uint32),
                 ("model", C.c_int32), ("n_edges", C.c_int32), ("base", C.c_double),
@@ -88,7 +106,10 @@ SIGNATURES = [
                                     C.POINTER(C.c_double)]),
     ("stater_bufset_create", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_uint,
                                        C.POINTER(_P)]),
+    ("stater_bufset_create_in", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_uint,
+                                          _P, C.c_uint64, _P, C.POINTER(_P)]),
     ("stater_bufset_destroy", C.c_int, [_P]),
+    ("stater_convert_plane", C.c_int, [_P, C.POINTER(Plane), C.POINTER(Plane), _P]),
     ("stater_bufset_reset", C.c_int, [_P, _P]),
     ("stater_bufset_view", C.c_int, [_P, C.c_char_p, C.POINTER(Plane)]),
     ("stater_bufset_bytes", C.c_int64, [_P]),
@@ -116,6 +137,16 @@ SIGNATURES = [
     ("stater_batch_destroy", C.c_int, [_P]),
     ("stater_debug_film_decisions", C.c_int, [_P, C.POINTER(DenoiseParams), C.POINTER(GBuffers),
                                               C.POINTER(Plane), C.c_int, _P, _P]),
+    ("stater_server_default_config", None, [C.POINTER(ServerConfig)]),
+    ("stater_server_create", C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("stater_server_job_bytes", C.c_int64, [_P, C.POINTER(Job)]),
+    ("stater_server_submit", C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    ("stater_server_wait", C.c_int, [_P, C.c_uint64, C.POINTER(JobResult)]),
+    ("stater_server_view", C.c_int, [_P, C.c_uint64, C.c_char_p, C.POINTER(Plane)]),
+    ("stater_server_release", C.c_int, [_P, C.c_uint64]),
+    ("stater_server_drain", C.c_int, [_P]),
+    ("stater_server_clock", C.c_double, [_P]),
+    ("stater_server_destroy", C.c_int, [_P]),
 ]
 
 _lib = None
@@ -190,10 +221,14 @@ class Context:
         check(self.lib.stater_ctx_stream(self.h, C.byref(s)))
         self.stream = s
         self._sets = weakref.WeakSet()
+        self._users = weakref.WeakSet()  # batches and servers: closed before the sets they use
 
     def close(self) -> None:
-        """Destroys the context, closing its remaining buffer sets first."""
+        """Destroys the context, closing its remaining servers, batches and
+        buffer sets first."""
         if self.h:
+            for u in list(self._users):
+                u.close()
             for bs in list(self._sets):
                 bs.close()
             check(self.lib.stater_destroy(self.h))
@@ -300,6 +335,7 @@ class Batch:
         h = C.c_void_p()
         check(self.lib.stater_batch_create(ctx.h, len(images), arr, C.byref(params), chunk_spp, C.byref(h)))
         self.h = h
+        ctx._users.add(self)
 
     def tune(self) -> None:
         check(self.lib.stater_batch_tune(self.h, None))

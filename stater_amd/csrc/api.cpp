@@ -182,7 +182,7 @@ int stater_create(int device, unsigned flags, stater_ctx **out) {
 int stater_destroy(stater_ctx *ctx) {
     if (!ctx) return STATER_OK;
     if (ctx->live_bufsets.load() > 0)
-        return fail(STATER_E_STATE, "stater_destroy: destroy the context's buffer sets first");
+        return fail(STATER_E_STATE, "stater_destroy: destroy the context's buffer sets and servers first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -252,6 +252,8 @@ int stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len, uint
 struct Layout {
     int S = 1, nst = 7, ld = 0;
     size_t plane = 0, n_planes = 0;
+    size_t as_bytes = 0;  // adaptive-sampling scratch after the planes (sets with derived planes)
+    size_t bytes() const { return n_planes * plane * 4 + as_bytes; }
     float set_lambda[kMaxSets] = {1.f};
     int slot_set[STATER_MAX_LAMBDAS] = {};
 };
@@ -281,7 +283,10 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     L.plane = (size_t)L.ld * H;
     // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
     size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
-    if (!(flags & STATER_BUFSET_NO_DERIVED)) np += (size_t)L.S * (1 + 6 * 3) + 9 + 1;
+    if (!(flags & STATER_BUFSET_NO_DERIVED)) {
+        np += (size_t)L.S * (1 + 6 * 3) + 9 + 1;
+        L.as_bytes = as_scratch_bytes_for((size_t)W * H);
+    }
     L.n_planes = np;
     return STATER_OK;
 }
@@ -290,17 +295,12 @@ int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas, const floa
                                 unsigned flags) {
     Layout L;
     if (make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L)) return -1;
-    return (int64_t)(L.n_planes * L.plane * 4);
+    return (int64_t)L.bytes();
 }
 
-int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, const float *lambdas,
-                         int max_moment, unsigned flags, stater_bufset **out) {
-    if (!ctx || !out) return fail(STATER_E_ARG, "stater_bufset_create: NULL argument");
-    *out = nullptr;
-    Layout L;
-    int rc = make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L);
-    if (rc) return rc;
-    STATER_HIP(hipSetDevice(ctx->device));
+// Carves the planes of a set out of mem (L.bytes() bytes, 256-byte aligned).
+static stater_bufset *bufset_over(stater_ctx *ctx, const Layout &L, int width, int height, int n_lambdas,
+                                  const float *lambdas, int max_moment, unsigned flags, char *mem, bool owns) {
     auto *bs = new stater_bufset();
     bs->ctx = ctx;
     bs->W = width; bs->H = height; bs->ld = L.ld; bs->plane = L.plane; bs->S = L.S;
@@ -309,15 +309,9 @@ int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, 
     for (int s = 0; s < L.S; ++s) bs->set_lambda[s] = L.set_lambda[s];
     bs->mm = max_moment;
     bs->flags = flags;
-    bs->bytes = L.n_planes * L.plane * 4;
-    hipError_t e = hipMalloc((void **)&bs->base, bs->bytes);
-    if (e != hipSuccess) {
-        delete bs;
-        (void)hipGetLastError();
-        return fail(STATER_E_NOMEM, "stater_bufset_create: device allocation failed");
-    }
-    e = hipMemset(bs->base, 0, bs->bytes);
-    if (e != hipSuccess) { (void)hipFree(bs->base); delete bs; return hip_fail(e, "hipMemset"); }
+    bs->bytes = L.bytes();
+    bs->base = mem;
+    bs->owns_memory = owns;
     float *p = (float *)bs->base;
     bs->n = (int32_t *)p; p += L.plane;
     bs->stats = p; p += (size_t)L.S * N_STATS * 3 * L.plane;
@@ -331,18 +325,57 @@ int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, 
         bs->meanD = p; p += 3 * L.plane;
         bs->filmD = p; p += 3 * L.plane;
         bs->targetspp = (int32_t *)p; p += L.plane;
+        bs->as_scratch = p;
+        bs->as_scratch_bytes = L.as_bytes;
     }
     ctx->live_bufsets.fetch_add(1);
+    return bs;
+}
+
+int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, const float *lambdas,
+                         int max_moment, unsigned flags, stater_bufset **out) {
+    if (!ctx || !out) return fail(STATER_E_ARG, "stater_bufset_create: NULL argument");
+    *out = nullptr;
+    Layout L;
+    int rc = make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L);
+    if (rc) return rc;
+    STATER_HIP(hipSetDevice(ctx->device));
+    char *mem = nullptr;
+    hipError_t e = hipMalloc((void **)&mem, L.bytes());
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(STATER_E_NOMEM, "stater_bufset_create: device allocation failed");
+    }
+    e = hipMemset(mem, 0, L.bytes());
+    if (e != hipSuccess) { (void)hipFree(mem); return hip_fail(e, "hipMemset"); }
+    *out = bufset_over(ctx, L, width, height, n_lambdas, lambdas, max_moment, flags, mem, true);
+    return STATER_OK;
+}
+
+int stater_bufset_create_in(stater_ctx *ctx, int width, int height, int n_lambdas, const float *lambdas,
+                            int max_moment, unsigned flags, void *mem, uint64_t mem_bytes, void *stream,
+                            stater_bufset **out) {
+    if (!ctx || !out || !mem) return fail(STATER_E_ARG, "stater_bufset_create_in: NULL argument");
+    *out = nullptr;
+    if ((uintptr_t)mem % 256) return fail(STATER_E_ARG, "stater_bufset_create_in: memory must be 256-byte aligned");
+    Layout L;
+    int rc = make_layout(width, height, n_lambdas, lambdas, max_moment, flags, L);
+    if (rc) return rc;
+    if (mem_bytes < L.bytes()) return fail(STATER_E_ARG, "stater_bufset_create_in: memory smaller than the layout");
+    auto *bs = bufset_over(ctx, L, width, height, n_lambdas, lambdas, max_moment, flags, (char *)mem, false);
+    rc = stater_bufset_reset(bs, stream);
+    if (rc) { stater_bufset_destroy(bs); return rc; }
     *out = bs;
     return STATER_OK;
 }
 
 int stater_bufset_destroy(stater_bufset *bs) {
     if (!bs) return STATER_OK;
-    (void)hipSetDevice(bs->ctx->device);
-    (void)hipStreamSynchronize(bs->ctx->stream);
-    if (bs->as_scratch) (void)hipFree(bs->as_scratch);
-    (void)hipFree(bs->base);
+    if (bs->owns_memory) {
+        (void)hipSetDevice(bs->ctx->device);
+        (void)hipStreamSynchronize(bs->ctx->stream);
+        (void)hipFree(bs->base);
+    }
     bs->ctx->live_bufsets.fetch_sub(1);
     delete bs;
     return STATER_OK;
@@ -437,10 +470,11 @@ static int check_params(const stater_bufset *bs, const stater_denoise_params *p,
         int total = 0;
         for (int k = 0; k < g->count; ++k) {
             const stater_plane &pl = g->planes[k];
+            const int64_t esz = pl.dtype == STATER_BF16 ? 2 : 4;
             if (!pl.ptr || pl.width != bs->W || pl.height != bs->H || pl.channels < 1 || pl.channels > 3 ||
-                pl.dtype != STATER_F32 || pl.layout != STATER_PLANAR || pl.pitch_bytes % 4 ||
-                pl.channel_stride_bytes % 4)
-                return fail(STATER_E_ARG, "denoise: gbuffer plane must be planar f32 of the film's size");
+                (pl.dtype != STATER_F32 && pl.dtype != STATER_BF16) || pl.layout != STATER_PLANAR ||
+                pl.pitch_bytes % esz || pl.channel_stride_bytes % esz)
+                return fail(STATER_E_ARG, "denoise: gbuffer plane must be planar f32 or bf16 of the film's size");
             if (!(g->dr_factors[k] <= 0.f)) return fail(STATER_E_ARG, "denoise: gbuffer factor must be <= 0");
             total += pl.channels;
         }

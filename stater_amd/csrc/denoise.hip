@@ -46,13 +46,18 @@ FeatArgs make_feats(const stater_gbuffers *g) {
             f.ptr[f.nf] = (const float *)((const char *)g->planes[k].ptr + c * g->planes[k].channel_stride_bytes);
             f.pitch[f.nf] = g->planes[k].pitch_bytes;
             f.scale[f.nf] = std::sqrt(-g->dr_factors[k] * kLog2e);
+            if (g->planes[k].dtype == STATER_BF16) f.bf16 |= 1u << f.nf;
             f.nf++;
         }
     return f;
 }
 
 __device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y) {
-    return *(const float *)((const char *)f.ptr[k] + (int64_t)y * f.pitch[k] + (int64_t)x * 4) * f.scale[k];
+    const char *row = (const char *)f.ptr[k] + (int64_t)y * f.pitch[k];
+    float v;
+    if (f.bf16 >> k & 1u) v = __uint_as_float((uint32_t)*(const uint16_t *)(row + (int64_t)x * 2) << 16);
+    else v = *(const float *)(row + (int64_t)x * 4);
+    return v * f.scale[k];
 }
 
 // ================================================================ prepass
@@ -1167,27 +1172,20 @@ __global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
 int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out, hipStream_t st) {
     const int P = bs->W * bs->H;
     const int nb = (P + 255) / 256;
-    const size_t need = (size_t)P * 12 + (size_t)nb * 8 + sizeof(AsCtl) + 256;
-    if (bs->as_scratch_bytes < need) {
-        if (bs->as_scratch) { STATER_HIP(hipStreamSynchronize(st)); STATER_HIP(hipFree(bs->as_scratch)); }
-        bs->as_scratch = nullptr;
-        bs->as_scratch_bytes = 0;
-        if (hipMalloc(&bs->as_scratch, need) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(STATER_E_NOMEM, "stater_calc_target_spp: scratch allocation failed");
-        }
-        bs->as_scratch_bytes = need;
-    }
+    static_assert(sizeof(AsCtl) <= kAsCtlBytes, "AS control block outgrew its slot");
+    if (!bs->as_scratch || bs->as_scratch_bytes < as_scratch_bytes_for((size_t)P))
+        return fail(STATER_E_STATE, "stater_calc_target_spp: set has no adaptive-sampling scratch");
     char *sp = (char *)bs->as_scratch;
     AsArgs a{};
     a.n = bs->n;
     a.varD = bs->varD;
     a.mean = bs->stat(0, ST_MEAN, 0);
     a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.plane = bs->plane; a.cap = cap;
-    a.ctl = (AsCtl *)sp; sp += (sizeof(AsCtl) + 255) / 256 * 256;
-    a.q = (uint32_t *)sp; sp += (size_t)P * 4;
-    a.h = (int32_t *)sp; sp += (size_t)P * 4;
-    a.inc = (int32_t *)sp; sp += (size_t)P * 4;
+    const size_t pb = ((size_t)P * 4 + 255) / 256 * 256;
+    a.ctl = (AsCtl *)sp; sp += kAsCtlBytes;
+    a.q = (uint32_t *)sp; sp += pb;
+    a.h = (int32_t *)sp; sp += pb;
+    a.inc = (int32_t *)sp; sp += pb;
     a.bsum = (int64_t *)sp;
     a.out = out ? (int32_t *)out->ptr : bs->targetspp;
     a.out_pitch = out ? out->pitch_bytes : (int64_t)bs->ld * 4;

@@ -49,9 +49,10 @@ struct stater_bufset {
           *discr = nullptr;  // [S][3][plane]
     float *meanVar = nullptr, *meanD = nullptr, *filmD = nullptr;  // [3][plane]
     int32_t *targetspp = nullptr;
-    // adaptive-sampling scratch
+    // adaptive-sampling scratch (inside the set's memory, after the planes)
     void *as_scratch = nullptr;
     size_t as_scratch_bytes = 0;
+    bool owns_memory = true;  // false: created over caller memory
 
     float *stat(int s, int st, int c) const {
         return stats + (((size_t)s * stater::N_STATS + st) * 3 + c) * plane;
@@ -60,6 +61,15 @@ struct stater_bufset {
 };
 
 namespace stater {
+
+// Adaptive-sampling scratch of a set: control block, then q, h and inc
+// (4 B per pixel each), then one int64 per 256-pixel block.
+constexpr size_t kAsCtlBytes = 512;
+inline size_t as_scratch_bytes_for(size_t pixels) {
+    const size_t nb = (pixels + 255) / 256;
+    auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+    return kAsCtlBytes + 3 * r(pixels * 4) + r(nb * 8);
+}
 
 // Kernel-launch timing: wraps one launch with events when the context asks.
 struct LaunchTimer {

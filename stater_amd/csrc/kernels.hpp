@@ -32,6 +32,7 @@ struct FeatArgs {
     int64_t pitch[kMaxFeat];
     float scale[kMaxFeat];  // sqrt(-dr log2 e): w = exp2(sp - sum (g'_p - g'_q)^2)
     int nf;
+    unsigned bf16;          // bit k: feature k is stored as bfloat16
 };
 
 struct PreArgs {

@@ -122,8 +122,12 @@ __global__ void __launch_bounds__(256) k_features(stater_scene s, float *alb, in
 }
 
 
-__device__ __forceinline__ void samples_tile(const stater_scene s, const float *alb, int64_t alb_pitch,
-                                             int64_t alb_cs, uint32_t first0, const SynthOut o, int bx, int by) {
+// Scene is a stater_scene held by value (a kernel argument) or by reference
+// into the batch's job table: a by-value copy of a table entry would go to
+// scratch memory, since the edge arrays are indexed at run time.
+template <class Scene>
+__device__ __forceinline__ void samples_tile(const Scene &s, const float *alb, int64_t alb_pitch, int64_t alb_cs,
+                                             uint32_t first0, const SynthOut &o, int bx, int by) {
     const int x = bx * 256 + threadIdx.x;
     const int y = by;
     if (x >= s.width) return;
@@ -200,7 +204,8 @@ __global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *al
 }
 
 
-__global__ void __launch_bounds__(256) k_samples_batch(const SampleJob *jobs, const int *prefix, int nimg) {
+__global__ void __launch_bounds__(256) k_samples_batch(const SampleJob *__restrict__ jobs,
+                                                       const int *__restrict__ prefix, int nimg) {
     const int ntiles = prefix[nimg];
     XcdRange r = xcd_range(ntiles);
     int img = 0;

@@ -26,11 +26,15 @@ def row_pitch(width: int) -> int:
 
 
 class SampleChunk:
-    def __init__(self, width: int, height: int, spp: int):
+    def __init__(self, width: int, height: int, spp: int, buf: DeviceBuffer | None = None):
+        """A chunk of spp samples for a width x height film; buf (optional)
+        lends the memory of a larger chunk."""
         self.width, self.height, self.spp = width, height, spp
         self.ld = row_pitch(width)
         self.plane_bytes = self.ld * height * 4
-        self.buf = DeviceBuffer(self.plane_bytes * 5 * spp)
+        if buf is not None and buf.nbytes < self.plane_bytes * 5 * spp:
+            raise ValueError("SampleChunk: lent buffer too small")
+        self.buf = buf if buf is not None else DeviceBuffer(self.plane_bytes * 5 * spp)
         self.samples = Samples(self.buf.ptr, self.ld * 4, self.plane_bytes, 5 * self.plane_bytes, spp, 0, None, 0,
                                0, 0)
 

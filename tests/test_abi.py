@@ -71,16 +71,23 @@ def _bytes_for(lib, W, H, lambdas, mm=5, flags=0):
     return lib.stater_bufset_bytes_for(W, H, len(lambdas), arr, mm, flags)
 
 
+def _as_scratch(P):
+    """Adaptive-sampling scratch after the planes: control block, 3 words per
+    pixel, one int64 per 256-pixel block (each part 256-byte aligned)."""
+    r = lambda b: (b + 255) // 256 * 256
+    return 512 + 3 * r(P * 4) + r((P + 255) // 256 * 8)
+
+
 def test_bufset_layout_size(lib):
     mt5 = (0.01, 0.25, 0.5, 0.75, 1.0, 3.0)
     S = 6                                  # untransformed + 5 distinct Box-Cox lambdas
     base = 1 + S * 7 * 3 + 3 + (S - 1) * 3  # n, moments, reference point, shifted means
     derived = S * (1 + 6 * 3) + 9 + 1       # jb, 6 per-set buffers, meanVar/meanD/filmD, targetspp
     ld = 1920                               # already a multiple of 64 floats
-    assert _bytes_for(lib, 1920, 1080, mt5) == (base + derived) * ld * 1080 * 4
+    assert _bytes_for(lib, 1920, 1080, mt5) == (base + derived) * ld * 1080 * 4 + _as_scratch(1920 * 1080)
     assert _bytes_for(lib, 1920, 1080, mt5, flags=sa.STATER_BUFSET_NO_DERIVED) == base * ld * 1080 * 4
     # rows padded to 256 bytes; repeated lambdas share one set; lambda 1 is the identity
-    assert _bytes_for(lib, 100, 10, (1.0,), mm=2) == (1 + 21 + 3 + 0 + 19 + 10) * 128 * 10 * 4
+    assert _bytes_for(lib, 100, 10, (1.0,), mm=2) == (1 + 21 + 3 + 0 + 19 + 10) * 128 * 10 * 4 + _as_scratch(1000)
     assert _bytes_for(lib, 100, 10, (0.5, 0.5, 1.0)) == _bytes_for(lib, 100, 10, (0.5,))
 
 
@@ -119,9 +126,23 @@ def test_null_arguments_are_rejected_without_device_work(lib):
     assert lib.stater_timing_count(None) == E_ARG
     assert lib.stater_batch_create(None, 1, None, C.byref(p), 16, C.byref(C.c_void_p())) == E_ARG
     assert lib.stater_batch_run(None, 0, None) == E_ARG
+    assert lib.stater_server_create(None, None, C.byref(C.c_void_p())) == E_ARG
+    assert lib.stater_server_submit(None, None, None) == E_ARG
+    assert lib.stater_server_wait(None, 1, None) == E_ARG
+    assert lib.stater_server_destroy(None) == 0
+    assert lib.stater_convert_plane(None, None, None, None) == E_ARG
+    assert lib.stater_bufset_create_in(None, 8, 8, 0, None, 5, 0, None, 0, None, C.byref(C.c_void_p())) == E_ARG
     assert lib.stater_destroy(None) == 0 and lib.stater_bufset_destroy(None) == 0
     assert lib.stater_bufset_bytes(None) == -1
     assert "NULL" in lib.stater_last_error().decode() or "bad" in lib.stater_last_error().decode()
+
+
+def test_server_default_config(lib):
+    c = sa.ServerConfig()
+    lib.stater_server_default_config(C.byref(c))
+    assert c.slots >= 1 and c.chunk_spp >= 1 and c.max_bytes >= 1 << 30
+    assert (c.max_width, c.max_height) == (3840, 2160)
+    assert np.allclose(list(c.feature_sds), [0.02, 0.1, 0.5]) and c.params.radius == 10
 
 
 def test_scene_init_is_deterministic(lib):
