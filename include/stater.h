@@ -275,13 +275,14 @@ int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
                          void *stream);
 
 /* ---- batches --------------------------------------------------------------
- * Several films of uneven size (and lambda count) processed together: per
- * pass of chunk_spp samples one persistent launch generates every image's
- * samples and one per group accumulates them; then prepass, variance filter
- * and Welch filter run as persistent launches over the tiles of all images.
- * use_graph captures the whole run once into a hipGraph and replays it.
- * The buffers each image ends with are bit-identical to running the image on
- * its own (stater_accumulate + stater_denoise_vars + stater_denoise_film).
+ * Several films of uneven size (and lambda count) processed together: each
+ * film's chain (reset, passes of chunk_spp generated samples + accumulation,
+ * prepass, variance filter, Welch filter) is spread with the others over a
+ * few HIP streams (longest first, least loaded stream), so small films'
+ * launches fill the device beside large films' tails.  use_graph captures the
+ * whole fan-out/fan-in once into a hipGraph and replays it.  The buffers each
+ * film ends with are bit-identical to running it on its own
+ * (stater_accumulate + stater_denoise_vars + stater_denoise_film).
  * Replaces nothing in the reference (it renders one film per process). */
 typedef struct stater_batch stater_batch;
 typedef struct {
@@ -295,12 +296,13 @@ typedef struct {
 
 int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *images,
                         const stater_denoise_params *p, int chunk_spp, stater_batch **out);
-/* Times the built tile heights (2, 4, 8 rows) of both filters on the batch
- * itself and keeps the fastest per group; drops a captured graph. */
+/* Times the filters' tile heights (2, 4, 8 rows) on every film and keeps the
+ * fastest per film; drops a captured graph. */
 int stater_batch_tune(stater_batch *b, void *stream);
 int stater_batch_run(stater_batch *b, int use_graph, void *stream);
-/* group < 0: returns the number of groups; else fills the group's figures. */
-int stater_batch_info(const stater_batch *b, int group, int *sets, int *images,
+/* image < 0: returns the number of films; else the film's set count, stream
+ * (branch) and chosen tile heights. */
+int stater_batch_info(const stater_batch *b, int image, int *sets, int *branch,
                       int *tile_rows_var, int *tile_rows_welch);
 int stater_batch_destroy(stater_batch *b);
 

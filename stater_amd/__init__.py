@@ -320,7 +320,7 @@ class BufferSet:
 
 
 class Batch:
-    """Uneven films through persistent launches (optionally one hipGraph).
+    """Uneven films processed together over several streams (optionally one hipGraph).
 
     images: list of (BufferSet, Scene, albedo Plane | None, GBuffers | None, spp)."""
 
@@ -343,13 +343,14 @@ class Batch:
     def run(self, use_graph: bool = True) -> None:
         check(self.lib.stater_batch_run(self.h, 1 if use_graph else 0, None))
 
-    def groups(self) -> list[dict]:
+    def images(self) -> list[dict]:
+        """Per film: statistics sets, stream (branch) and the tuned tile heights."""
         n = self.lib.stater_batch_info(self.h, -1, None, None, None, None)
         out = []
-        for g in range(n):
-            s, m, tv, tw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-            check(self.lib.stater_batch_info(self.h, g, C.byref(s), C.byref(m), C.byref(tv), C.byref(tw)))
-            out.append({"sets": s.value, "images": m.value, "tile_rows_var": tv.value, "tile_rows_welch": tw.value})
+        for i in range(n):
+            s, br, tv, tw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            check(self.lib.stater_batch_info(self.h, i, C.byref(s), C.byref(br), C.byref(tv), C.byref(tw)))
+            out.append({"sets": s.value, "branch": br.value, "tile_rows_var": tv.value, "tile_rows_welch": tw.value})
         return out
 
     def close(self) -> None:

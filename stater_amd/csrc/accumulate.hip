@@ -371,23 +371,6 @@ __global__ void STATER_ACC_OCC k_accumulate(AccArgs a) { acc_tile<T, MM>(a, bloc
 template <int T, int MM>
 __global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
 
-// Persistent form over the row segments of several images (one S, MM group).
-template <int T, int MM, bool MERGE>
-__global__ void STATER_ACC_OCC k_accumulate_batch(const AccArgs *args, const int *prefix, int nimg) {
-    const int ntiles = prefix[nimg];
-    XcdRange r = xcd_range(ntiles);
-    int img = 0;
-    for (int w = r.begin; w < r.end; w += r.step) {
-        while (w >= prefix[img + 1]) ++img;
-        const AccArgs &a = args[img];
-        const int t = w - prefix[img];
-        const int tiles_x = (a.W + 63) / 64;
-        const int by = t / tiles_x, bx = t - by * tiles_x;
-        if (MERGE) accm_tile<T, MM>(a, bx, by);
-        else acc_tile<T, MM>(a, bx, by);
-    }
-}
-
 __global__ void k_zero(uint32_t *p, size_t words) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -461,36 +444,6 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
-    return STATER_OK;
-}
-
-template <int T>
-static const void *acc_batch_t(int mm, bool merge) {
-    switch (mm) {
-        case 1: return merge ? (const void *)k_accumulate_batch<T, 1, true> : (const void *)k_accumulate_batch<T, 1, false>;
-        case 2: return merge ? (const void *)k_accumulate_batch<T, 2, true> : (const void *)k_accumulate_batch<T, 2, false>;
-        case 3: return merge ? (const void *)k_accumulate_batch<T, 3, true> : (const void *)k_accumulate_batch<T, 3, false>;
-        case 4: return merge ? (const void *)k_accumulate_batch<T, 4, true> : (const void *)k_accumulate_batch<T, 4, false>;
-        default: return merge ? (const void *)k_accumulate_batch<T, 5, true> : (const void *)k_accumulate_batch<T, 5, false>;
-    }
-}
-
-const void *accumulate_batch_kernel(int S, int mm, bool merge) {
-    switch (S - 1) {
-        case 0: return acc_batch_t<0>(mm, merge);
-        case 1: return acc_batch_t<1>(mm, merge);
-        case 2: return acc_batch_t<2>(mm, merge);
-        case 3: return acc_batch_t<3>(mm, merge);
-        case 4: return acc_batch_t<4>(mm, merge);
-        default: return acc_batch_t<5>(mm, merge);
-    }
-}
-
-int launch_accumulate_batch(int S, int mm, bool merge, const AccArgs *args, const int *prefix, int nimg, int grid,
-                            hipStream_t st) {
-    const void *k = accumulate_batch_kernel(S, mm, merge);
-    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
-    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(192), params, 0, st));
     return STATER_OK;
 }
 

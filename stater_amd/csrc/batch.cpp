@@ -1,19 +1,16 @@
-// batch.cpp — uneven images through persistent launches, captured in a hipGraph.
+// batch.cpp — films of uneven size processed together.
 //
-// A batch owns, per image, a device chunk of chunk_spp samples and, per
-// (sets, max moment, feature weights) group, tables of kernel argument
-// blocks in device memory.  One run is: reset -> for every pass of chunk_spp
-// samples: generate (all images, one persistent launch) + accumulate (one
-// persistent launch per group) -> prepass, variance filter and Welch filter
-// (persistent launches per group).  Every persistent launch walks a work list
-// of tiles over all images of its group with the XCD-aware static schedule
-// of common.hpp, so small and large images share the chip without tails.
-// The filters' tile height (wavefronts per tile) is chosen per group by
-// stater_batch_tune, which times the built candidates on the batch itself.
-// With use_graph the whole run is captured once and replayed.
+// Every film's chain (reset; passes of chunk_spp generated samples, each
+// followed by accumulation; prepass; variance filter; Welch filter) runs as
+// the single-film launches, and the chains are spread over a few HIP streams
+// (longest chain first, onto the least loaded stream), so the launches of
+// small films fill the device beside the large films' tails and nothing waits
+// for the host.  With use_graph the fan-out and fan-in are captured once into
+// a hipGraph and replayed.  stater_batch_tune times the filters' tile heights
+// (2, 4 and 8 rows) on every film and keeps the fastest for it.
 //
-// Each pixel's arithmetic is the single-image path's (same tile bodies, same
-// per-pixel loop order), so a batch produces bit-identical buffers.
+// Each film runs the same kernels in the same order as on its own, so a batch
+// leaves bit-identical buffers.
 
 #include <algorithm>
 #include <cstring>
@@ -25,17 +22,7 @@ using namespace stater;
 
 namespace {
 
-// Host staging of all descriptor tables; uploaded once into one allocation.
-struct Arena {
-    std::vector<char> host;
-    size_t add(const void *p, size_t bytes) {
-        const size_t off = (host.size() + 255) / 256 * 256;
-        host.resize(off + bytes);
-        std::memcpy(host.data() + off, p, bytes);
-        return off;
-    }
-};
-
+constexpr int kBranches = 4;
 constexpr int kTyOptions[3] = {2, 4, 8};
 
 }  // namespace
@@ -44,7 +31,6 @@ struct stater_batch {
     stater_ctx *ctx = nullptr;
     stater_denoise_params p{};
     int chunk_spp = 32;
-    int passes = 0;
     struct Img {
         stater_bufset *bs;
         stater_scene sc;
@@ -53,81 +39,60 @@ struct stater_batch {
         stater_gbuffers g;
         bool has_g;
         int spp;
-        size_t chunk_off;
+        int branch;
+        int ty_var, ty_welch;
+        double cost;
     };
     std::vector<Img> imgs;
-    char *chunk_mem = nullptr;
-    char *dev = nullptr;  // descriptor tables
-    // generator, per pass
-    std::vector<size_t> synth_jobs, synth_prefix;
-    std::vector<int> synth_grid;
-    int nimg = 0;
-    struct Group {
-        int S, mm;
-        bool hasw;
-        std::vector<int> members;
-        std::vector<size_t> acc_args, acc_prefix;  // per pass
-        std::vector<int> acc_grid;
-        size_t pre_args = 0, pre_prefix = 0;
-        int pre_grid = 0;
-        std::vector<int> g0s, Gs;
-        std::vector<size_t> var_args;              // per g0
-        size_t welch_args = 0;
-        size_t tile_prefix[3] = {0, 0, 0};         // per tile-height option
-        int var_grid[3][3] = {};                   // [g0 index][ty option]
-        int welch_grid[3] = {};
-        int ty_var = 1, ty_welch = 1;              // index into kTyOptions (4 rows)
-    };
-    std::vector<Group> groups;
+    std::vector<std::vector<int>> order;  // films of each branch, in launch order
+    std::vector<hipStream_t> streams;
+    std::vector<char *> chunks;           // one sample chunk per branch
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
-template <class T>
-const T *devp(const stater_batch *b, size_t off) { return (const T *)(b->dev + off); }
-
-int grid_for(stater_ctx *ctx, const void *k, int block, size_t lds, int ntiles) {
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, block, lds) != hipSuccess || occ < 1) occ = 1;
-    int g = std::min(occ * ctx->num_cu, std::max(ntiles, 1));
-    g = (g + kXcds - 1) / kXcds * kXcds;
-    return std::max(g, kXcds);
+stater_samples chunk_desc(const stater_batch *b, const stater_batch::Img &I, int spp) {
+    stater_samples s{};
+    s.data = (const float *)b->chunks[I.branch];
+    s.pitch_bytes = (int64_t)I.bs->ld * 4;
+    s.comp_stride_bytes = (int64_t)I.bs->plane * 4;
+    s.sample_stride_bytes = 5 * s.comp_stride_bytes;
+    s.spp = spp;
+    return s;
 }
 
-bool ty_built(bool hasw, int ti) { return hasw || kTyOptions[ti] == 4; }
+const stater_gbuffers *gb(const stater_batch::Img &I) { return I.has_g ? &I.g : nullptr; }
+
+// One film's chain on stream st.
+int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t st) {
+    int rc = stater_bufset_reset(I.bs, st);
+    for (int off = 0; off < I.spp && !rc; off += b->chunk_spp) {
+        const stater_samples s = chunk_desc(b, I, std::min(b->chunk_spp, I.spp - off));
+        rc = launch_synth_samples(b->ctx, &I.sc, I.has_alb ? &I.albedo : nullptr, (uint32_t)off, nullptr, &s, st);
+        if (!rc) rc = launch_accumulate(I.bs, &s, STATER_ACC_SEQUENTIAL, st);
+    }
+    if (!rc) rc = launch_prepass(I.bs, b->p.var_z, st);
+    if (!rc) rc = launch_var_filter(I.bs, &b->p, gb(I), st, I.ty_var);
+    if (!rc) rc = launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, I.ty_welch);
+    return rc;
+}
 
 int enqueue(stater_batch *b, hipStream_t st) {
-    for (auto &im : b->imgs) {
-        int rc = launch_zero(im.bs->base, (1 + (size_t)im.bs->S * N_STATS * 3) * im.bs->plane * 4, st);
-        if (rc) return rc;
-    }
-    for (int k = 0; k < b->passes; ++k) {
-        int rc = launch_samples_batch(devp<SampleJob>(b, b->synth_jobs[k]), devp<int>(b, b->synth_prefix[k]), b->nimg,
-                                      b->synth_grid[k], st);
-        if (rc) return rc;
-        for (auto &g : b->groups) {
-            if (!g.acc_grid[k]) continue;
-            rc = launch_accumulate_batch(g.S, g.mm, false, devp<AccArgs>(b, g.acc_args[k]), devp<int>(b, g.acc_prefix[k]),
-                                         (int)g.members.size(), g.acc_grid[k], st);
+    STATER_HIP(hipEventRecord(b->fork, st));
+    for (size_t k = 0; k < b->streams.size(); ++k) STATER_HIP(hipStreamWaitEvent(b->streams[k], b->fork, 0));
+    for (size_t k = 0; k < b->streams.size(); ++k)
+        for (int i : b->order[k]) {
+            const int rc = enqueue_film(b, b->imgs[i], b->streams[k]);
             if (rc) return rc;
         }
+    for (size_t k = 0; k < b->streams.size(); ++k) {
+        STATER_HIP(hipEventRecord(b->joins[k], b->streams[k]));
+        STATER_HIP(hipStreamWaitEvent(st, b->joins[k], 0));
     }
-    for (auto &g : b->groups) {
-        const int m = (int)g.members.size();
-        int rc = launch_prepass_batch(devp<PreArgs>(b, g.pre_args), devp<int>(b, g.pre_prefix), m, g.pre_grid, st);
-        if (rc) return rc;
-        for (size_t i = 0; i < g.g0s.size(); ++i) {
-            rc = launch_var_batch(g.Gs[i], g.hasw, b->p.radius, kTyOptions[g.ty_var], devp<VarArgs>(b, g.var_args[i]),
-                                  devp<int>(b, g.tile_prefix[g.ty_var]), m, g.var_grid[i][g.ty_var], st);
-            if (rc) return rc;
-        }
-        rc = launch_welch_batch(g.S, g.hasw, b->p.radius, kTyOptions[g.ty_welch], devp<WelchArgs>(b, g.welch_args),
-                                devp<int>(b, g.tile_prefix[g.ty_welch]), m, g.welch_grid[g.ty_welch], st);
-        if (rc) return rc;
-    }
-    STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
@@ -138,6 +103,18 @@ void drop_graph(stater_batch *b) {
     b->graph = nullptr;
 }
 
+void destroy(stater_batch *b) {
+    drop_graph(b);
+    for (auto s : b->streams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (auto c : b->chunks) (void)hipFree(c);
+    for (auto e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    delete b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -146,10 +123,8 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
                         int chunk_spp, stater_batch **out) {
     if (!ctx || !images || !p || !out || count <= 0) return fail(STATER_E_ARG, "stater_batch_create: bad arguments");
     *out = nullptr;
-    if (p->radius < 0 || p->radius > 10)
-        return fail(STATER_E_UNSUPPORTED, "stater_batch_create: batched filters are built for radius <= 10");
-    if (p->mt_mode != 2) return fail(STATER_E_UNSUPPORTED, "stater_batch_create: batches run mode 2");
-    if (!(p->alpha > 0.f && p->alpha < 1.f) || !(p->var_z > 0.f) || !(p->ds_factor <= 0.f))
+    if (p->radius < 0 || p->radius > STATER_MAX_RADIUS || !(p->alpha > 0.f && p->alpha < 1.f) || !(p->var_z > 0.f) ||
+        !(p->ds_factor <= 0.f) || (p->mt_mode != 2 && p->mt_mode != 3))
         return fail(STATER_E_ARG, "stater_batch_create: bad denoise parameters");
     if (chunk_spp <= 0 || chunk_spp > 1024) return fail(STATER_E_ARG, "stater_batch_create: chunk_spp in 1..1024");
     STATER_HIP(hipSetDevice(ctx->device));
@@ -157,9 +132,6 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
     b->ctx = ctx;
     b->p = *p;
     b->chunk_spp = chunk_spp;
-    b->nimg = count;
-    size_t chunk_total = 0;
-    int max_spp = 0;
     for (int i = 0; i < count; ++i) {
         const stater_batch_image &im = images[i];
         if (!im.set || !im.scene || im.spp <= 0 || (im.set->flags & STATER_BUFSET_NO_DERIVED) || im.set->ctx != ctx ||
@@ -177,143 +149,46 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
         I.has_g = im.gbufs != nullptr;
         if (I.has_g) I.g = *im.gbufs;
         I.spp = im.spp;
-        I.chunk_off = chunk_total;
-        chunk_total += (size_t)chunk_spp * 5 * im.set->plane * 4;
-        max_spp = std::max(max_spp, im.spp);
+        I.ty_var = I.ty_welch = 4;
+        // work estimate: samples x sets (generation + accumulation) + window tests
+        const double P = (double)im.set->W * im.set->H, S = im.set->S;
+        const double win = (2.0 * p->radius + 1) * (2.0 * p->radius + 1);
+        I.cost = P * ((double)im.spp * (2.0 + S) + 0.05 * win * S * 3.0);
         b->imgs.push_back(I);
     }
-    b->passes = (max_spp + chunk_spp - 1) / chunk_spp;
-    if (hipMalloc((void **)&b->chunk_mem, chunk_total) != hipSuccess) {
-        (void)hipGetLastError();
-        delete b;
-        return fail(STATER_E_NOMEM, "stater_batch_create: sample chunks do not fit");
+    // longest chain first, each onto the least loaded branch
+    const int nb = std::min(kBranches, count);
+    std::vector<int> idx(count);
+    for (int i = 0; i < count; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return b->imgs[a].cost > b->imgs[c].cost; });
+    std::vector<double> load(nb, 0.0);
+    b->order.assign(nb, {});
+    for (int i : idx) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->imgs[i].branch = k;
+        load[k] += b->imgs[i].cost;
+        b->order[k].push_back(i);
     }
-
-    Arena ar;
-    auto samples_of = [&](const stater_batch::Img &I, int k) {
-        stater_samples s{};
-        s.data = (const float *)(b->chunk_mem + I.chunk_off);
-        s.pitch_bytes = (int64_t)I.bs->ld * 4;
-        s.comp_stride_bytes = (int64_t)I.bs->plane * 4;
-        s.sample_stride_bytes = 5 * s.comp_stride_bytes;
-        s.spp = std::max(0, std::min(chunk_spp, I.spp - k * chunk_spp));
-        return s;
+    auto fail_out = [&](int rc, const std::string &what) {
+        destroy(b);
+        (void)hipGetLastError();
+        return fail(rc, "stater_batch_create: " + what);
     };
-    // generator tables
-    for (int k = 0; k < b->passes; ++k) {
-        std::vector<SampleJob> jobs(count);
-        std::vector<int> prefix(count + 1, 0);
-        for (int i = 0; i < count; ++i) {
-            const auto &I = b->imgs[i];
-            stater_samples s = samples_of(I, k);
-            SampleJob &j = jobs[i];
-            j.s = I.sc;
-            j.alb = I.has_alb ? (const float *)I.albedo.ptr : nullptr;
-            j.alb_pitch = I.has_alb ? I.albedo.pitch_bytes : 0;
-            j.alb_cs = I.has_alb ? I.albedo.channel_stride_bytes : 0;
-            j.first0 = (uint32_t)(k * chunk_spp);
-            j.o = make_synth_out(&s, nullptr);
-            prefix[i + 1] = prefix[i] + (s.spp > 0 ? ((I.bs->W + 255) / 256) * I.bs->H : 0);
-        }
-        b->synth_jobs.push_back(ar.add(jobs.data(), jobs.size() * sizeof(SampleJob)));
-        b->synth_prefix.push_back(ar.add(prefix.data(), prefix.size() * sizeof(int)));
-        b->synth_grid.push_back(std::max(kXcds, std::min(4 * ctx->num_cu, (prefix[count] + 7) / 8 * 8)));
-    }
-    // groups by (sets, max moment, weights)
-    for (int i = 0; i < count; ++i) {
-        const auto &I = b->imgs[i];
-        const FeatArgs f = make_feats(I.has_g ? &I.g : nullptr);
-        const bool hasw = f.nf > 0 || p->ds_factor != 0.f;
-        auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const stater_batch::Group &g) {
-            return g.S == I.bs->S && g.mm == I.bs->mm && g.hasw == hasw;
-        });
-        if (it == b->groups.end()) {
-            stater_batch::Group g;
-            g.S = I.bs->S;
-            g.mm = I.bs->mm;
-            g.hasw = hasw;
-            b->groups.push_back(g);
-            it = b->groups.end() - 1;
-        }
-        it->members.push_back(i);
-    }
-    for (auto &g : b->groups) {
-        const int m = (int)g.members.size();
-        for (int k = 0; k < b->passes; ++k) {
-            std::vector<AccArgs> args(m);
-            std::vector<int> prefix(m + 1, 0);
-            for (int j = 0; j < m; ++j) {
-                const auto &I = b->imgs[g.members[j]];
-                stater_samples s = samples_of(I, k);
-                args[j] = make_acc_args(I.bs, &s);
-                prefix[j + 1] = prefix[j] + (s.spp > 0 ? ((I.bs->W + 63) / 64) * I.bs->H : 0);
-            }
-            g.acc_args.push_back(ar.add(args.data(), args.size() * sizeof(AccArgs)));
-            g.acc_prefix.push_back(ar.add(prefix.data(), prefix.size() * sizeof(int)));
-            g.acc_grid.push_back(prefix[m] ? grid_for(ctx, accumulate_batch_kernel(g.S, g.mm, false), 192, 0, prefix[m])
-                                           : 0);
-        }
-        std::vector<PreArgs> pre(m);
-        std::vector<int> pprefix(m + 1, 0);
-        for (int j = 0; j < m; ++j) {
-            const auto &I = b->imgs[g.members[j]];
-            pre[j] = make_pre_args(I.bs, p->var_z);
-            pprefix[j + 1] = pprefix[j] + ((I.bs->W + 255) / 256) * I.bs->H;
-        }
-        g.pre_args = ar.add(pre.data(), pre.size() * sizeof(PreArgs));
-        g.pre_prefix = ar.add(pprefix.data(), pprefix.size() * sizeof(int));
-        g.pre_grid = std::max(kXcds, std::min(4 * ctx->num_cu, (pprefix[m] + 7) / 8 * 8));
-        for (int g0 = 0; g0 < g.S;) {
-            const int G = std::min(3, g.S - g0);
-            std::vector<VarArgs> va(m);
-            for (int j = 0; j < m; ++j) {
-                const auto &I = b->imgs[g.members[j]];
-                va[j] = make_var_args(I.bs, p, I.has_g ? &I.g : nullptr, g0);
-            }
-            g.g0s.push_back(g0);
-            g.Gs.push_back(G);
-            g.var_args.push_back(ar.add(va.data(), va.size() * sizeof(VarArgs)));
-            g0 += G;
-        }
-        std::vector<WelchArgs> wa(m);
-        for (int j = 0; j < m; ++j) {
-            const auto &I = b->imgs[g.members[j]];
-            wa[j] = make_welch_args(I.bs, p, I.has_g ? &I.g : nullptr);
-        }
-        g.welch_args = ar.add(wa.data(), wa.size() * sizeof(WelchArgs));
-        for (int ti = 0; ti < 3; ++ti) {
-            const int ty = kTyOptions[ti];
-            std::vector<int> tprefix(m + 1, 0);
-            for (int j = 0; j < m; ++j) {
-                const auto &I = b->imgs[g.members[j]];
-                tprefix[j + 1] = tprefix[j] + ((I.bs->W + 63) / 64) * ((I.bs->H + ty - 1) / ty);
-            }
-            g.tile_prefix[ti] = ar.add(tprefix.data(), tprefix.size() * sizeof(int));
-            if (!ty_built(g.hasw, ti)) continue;
-            for (size_t gi = 0; gi < g.Gs.size(); ++gi) {
-                size_t lds = 0;
-                const void *k = var_batch_kernel(g.Gs[gi], g.hasw, p->radius, ty, &lds, 0);
-                (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-                g.var_grid[gi][ti] = grid_for(ctx, k, 64 * ty, lds, tprefix[m]);
-            }
-            size_t lds = 0;
-            const void *k = welch_batch_kernel(g.S, g.hasw, p->radius, ty, &lds);
-            (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            g.welch_grid[ti] = grid_for(ctx, k, 64 * ty, lds, tprefix[m]);
-        }
-    }
-    if (hipMalloc((void **)&b->dev, std::max<size_t>(ar.host.size(), 256)) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(b->chunk_mem);
-        delete b;
-        return fail(STATER_E_NOMEM, "stater_batch_create: descriptor tables do not fit");
-    }
-    hipError_t e = hipMemcpy(b->dev, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(b->dev);
-        (void)hipFree(b->chunk_mem);
-        delete b;
-        return hip_fail(e, "hipMemcpy");
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        return fail_out(STATER_E_HIP, "event creation failed");
+    for (int k = 0; k < nb; ++k) {
+        size_t need = 0;
+        for (int i : b->order[k]) need = std::max(need, (size_t)chunk_spp * 5 * b->imgs[i].bs->plane * 4);
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        char *c = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return fail_out(STATER_E_HIP, "stream");
+        b->streams.push_back(s);
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail_out(STATER_E_HIP, "event");
+        b->joins.push_back(e);
+        if (hipMalloc((void **)&c, std::max<size_t>(need, 256)) != hipSuccess)
+            return fail_out(STATER_E_NOMEM, "sample chunks do not fit");
+        b->chunks.push_back(c);
     }
     *out = b;
     return STATER_OK;
@@ -321,11 +196,9 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
 
 int stater_batch_destroy(stater_batch *b) {
     if (!b) return STATER_OK;
+    (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    drop_graph(b);
-    (void)hipFree(b->dev);
-    (void)hipFree(b->chunk_mem);
-    delete b;
+    destroy(b);
     return STATER_OK;
 }
 
@@ -355,36 +228,33 @@ int stater_batch_tune(stater_batch *b, void *stream) {
     hipEvent_t e0, e1;
     STATER_HIP(hipEventCreate(&e0));
     STATER_HIP(hipEventCreate(&e1));
-    for (auto &g : b->groups) {
-        const int m = (int)g.members.size();
+    auto time_one = [&](auto &&launch, float *ms) {
+        int r = 0;
+        for (int rep = 0; rep < 2 && !r; ++rep) {  // the first launch warms up
+            (void)hipEventRecord(e0, st);
+            r = launch();
+            (void)hipEventRecord(e1, st);
+            if (r) break;
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(ms, e0, e1);
+        }
+        return r;
+    };
+    for (auto &I : b->imgs) {
+        const bool hasw = I.has_g || b->p.ds_factor != 0.f;
+        if (!hasw || b->p.radius > 10) continue;  // only 4-row tiles are built there
         float best_v = 1e30f, best_w = 1e30f;
-        for (int ti = 0; ti < 3; ++ti) {
-            if (!ty_built(g.hasw, ti)) continue;
-            const int ty = kTyOptions[ti];
-            for (int rep = 0; rep < 2; ++rep) {  // first launch warms up
-                (void)hipEventRecord(e0, st);
-                for (size_t i = 0; i < g.g0s.size() && !rc; ++i)
-                    rc = launch_var_batch(g.Gs[i], g.hasw, b->p.radius, ty, devp<VarArgs>(b, g.var_args[i]),
-                                          devp<int>(b, g.tile_prefix[ti]), m, g.var_grid[i][ti], st);
-                (void)hipEventRecord(e1, st);
-                if (rc) break;
-                (void)hipEventSynchronize(e1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (rep && ms < best_v) { best_v = ms; g.ty_var = ti; }
-            }
-            for (int rep = 0; rep < 2 && !rc; ++rep) {
-                (void)hipEventRecord(e0, st);
-                rc = launch_welch_batch(g.S, g.hasw, b->p.radius, ty, devp<WelchArgs>(b, g.welch_args),
-                                        devp<int>(b, g.tile_prefix[ti]), m, g.welch_grid[ti], st);
-                (void)hipEventRecord(e1, st);
-                if (rc) break;
-                (void)hipEventSynchronize(e1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (rep && ms < best_w) { best_w = ms; g.ty_welch = ti; }
-            }
+        for (int ty : kTyOptions) {
+            float ms = 0.f;
+            rc = time_one([&] { return launch_var_filter(I.bs, &b->p, gb(I), st, ty); }, &ms);
             if (rc) break;
+            if (ms < best_v) { best_v = ms; I.ty_var = ty; }
+            if (b->p.mt_mode == 2) {
+                rc = time_one([&] { return launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, ty); },
+                              &ms);
+                if (rc) break;
+                if (ms < best_w) { best_w = ms; I.ty_welch = ty; }
+            }
         }
         if (rc) break;
     }
@@ -394,16 +264,16 @@ int stater_batch_tune(stater_batch *b, void *stream) {
     return rc;
 }
 
-int stater_batch_info(const stater_batch *b, int group, int *sets, int *images, int *tile_rows_var,
+int stater_batch_info(const stater_batch *b, int image, int *sets, int *branch, int *tile_rows_var,
                       int *tile_rows_welch) {
     if (!b) return fail(STATER_E_ARG, "stater_batch_info: NULL batch");
-    if (group < 0) return (int)b->groups.size();
-    if (group >= (int)b->groups.size()) return fail(STATER_E_ARG, "stater_batch_info: no such group");
-    const auto &g = b->groups[group];
-    if (sets) *sets = g.S;
-    if (images) *images = (int)g.members.size();
-    if (tile_rows_var) *tile_rows_var = kTyOptions[g.ty_var];
-    if (tile_rows_welch) *tile_rows_welch = kTyOptions[g.ty_welch];
+    if (image < 0) return (int)b->imgs.size();
+    if (image >= (int)b->imgs.size()) return fail(STATER_E_ARG, "stater_batch_info: no such image");
+    const auto &I = b->imgs[image];
+    if (sets) *sets = I.bs->S;
+    if (branch) *branch = I.branch;
+    if (tile_rows_var) *tile_rows_var = I.ty_var;
+    if (tile_rows_welch) *tile_rows_welch = I.ty_welch;
     return STATER_OK;
 }
 

@@ -56,20 +56,7 @@ __host__ __device__ inline double unit24_open(uint32_t r) {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Static, XCD-aware schedule of a persistent grid over ntiles work items.
-// Workgroup i runs on XCD i % 8 (round-robin dispatch), so the items are cut
-// into 8 contiguous chunks, one per XCD, and the XCD's workgroups stride
-// through their chunk: neighbouring tiles, which share halo rows, stay in one
-// L2.  gridDim.x must be a multiple of 8; every workgroup's loop ends.
-constexpr int kXcds = 8;
-struct XcdRange { int begin, end, step; };
-__device__ __forceinline__ XcdRange xcd_range(int ntiles) {
-    const int xcd = blockIdx.x % kXcds, l = blockIdx.x / kXcds, per = gridDim.x / kXcds;
-    const int chunk = (ntiles + kXcds - 1) / kXcds;
-    const int b = xcd * chunk;
-    const int e = b + chunk < ntiles ? b + chunk : ntiles;
-    return XcdRange{b + l, e, per};
-}
+constexpr int kXcds = 8;  // accelerator complex dies: workgroup i runs on XCD i % 8
 
 // Static XCD-aware placement of a one-tile-per-workgroup grid of
 // xcd_grid(ntiles) workgroups: workgroup i runs on XCD i % 8 and takes the

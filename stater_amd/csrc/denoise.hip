@@ -107,29 +107,9 @@ __device__ __forceinline__ void prepass_tile(const PreArgs a, int bx, int by) {
 
 __global__ void __launch_bounds__(256) k_prepass(PreArgs a) { prepass_tile(a, blockIdx.x, blockIdx.y); }
 
-__global__ void __launch_bounds__(256) k_prepass_batch(const PreArgs *args, const int *prefix, int nimg) {
-    const int ntiles = prefix[nimg];
-    XcdRange r = xcd_range(ntiles);
-    int img = 0;
-    for (int w = r.begin; w < r.end; w += r.step) {
-        while (w >= prefix[img + 1]) ++img;
-        const PreArgs &a = args[img];
-        const int t = w - prefix[img];
-        const int tiles_x = (a.W + 255) / 256;
-        const int by = t / tiles_x, bx = t - by * tiles_x;
-        prepass_tile(a, bx, by);
-    }
-}
-
 PreArgs make_pre_args(stater_bufset *bs, float var_z) {
     return PreArgs{bs->n, bs->stats, bs->jb, bs->lnS2, bs->se2, bs->m2C, bs->S, bs->W, bs->H, bs->ld,
                    bs->mm >= 5 ? 1 : 0, bs->plane, var_z};
-}
-
-int launch_prepass_batch(const PreArgs *args, const int *prefix, int nimg, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_prepass_batch, dim3(grid), dim3(256), 0, st, args, prefix, nimg);
-    STATER_HIP(hipGetLastError());
-    return STATER_OK;
 }
 
 int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
@@ -304,51 +284,37 @@ __device__ __forceinline__ void var_tile(const VarArgs a, int bx, int by, float 
         }
 }
 
-template <int G, bool HASW, int HWC>
-__global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, kTY>(), 256))))
-    k_var_filter(VarArgs a) {
-    extern __shared__ float ring[];
-    const int tiles_x = (a.W + TX - 1) / TX;
-    const int t = xcd_tile(tiles_x * ((a.H + kTY - 1) / kTY));
-    if (t < 0) return;
-    var_tile<G, HASW, HWC, kTY>(a, t % tiles_x, t / tiles_x, ring);
-}
-
+// One tile of 64 x TY pixels per workgroup (one wavefront per row).  Tile
+// heights 2 and 8 are built for the r <= 10 feature-weighted filter (chosen
+// per film by the batch tuner); everything else runs 4.
 template <int G, bool HASW, int HWC, int TY>
 __global__ void __launch_bounds__(64 * TY)
     __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY>(), 64 * TY))))
-    k_var_filter_batch(const VarArgs *args, const int *prefix, int nimg) {
+    k_var_filter(VarArgs a) {
     extern __shared__ float ring[];
-    const int ntiles = prefix[nimg];
-    XcdRange r = xcd_range(ntiles);
-    int img = 0;
-    for (int w = r.begin; w < r.end; w += r.step) {
-        while (w >= prefix[img + 1]) ++img;
-        const VarArgs &a = args[img];
-        const int t = w - prefix[img];
-        const int tiles_x = (a.W + TX - 1) / TX;
-        const int by = t / tiles_x, bx = t - by * tiles_x;
-        var_tile<G, HASW, HWC, TY>(a, bx, by, ring);
-        __syncthreads();  // the next tile restages the ring
-    }
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    if (t < 0) return;
+    var_tile<G, HASW, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
-template <int G, bool HASW, int HWC>
-static int var_launch3(const VarArgs &a, dim3 grid, hipStream_t st) {
-    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
-    const size_t lds = (size_t)kTY * (SINGLES + 12 * G) * HWC * 4;
-    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC>,
+template <int G, bool HASW, int HWC, int TY>
+static int var_launch4(const VarArgs &a, hipStream_t st) {
+    const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY>();
+    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC>), grid, dim3(256), lds, st, a);
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
 template <int G>
-static int var_launch(const VarArgs &a, bool hasw, dim3 grid, hipStream_t st) {
-    if (a.r <= 10) return hasw ? var_launch3<G, true, TX + 20>(a, grid, st) : var_launch3<G, false, TX + 20>(a, grid, st);
-    return hasw ? var_launch3<G, true, TX + 40>(a, grid, st) : var_launch3<G, false, TX + 40>(a, grid, st);
+static int var_launch(const VarArgs &a, bool hasw, int ty, hipStream_t st) {
+    if (a.r > 10) return hasw ? var_launch4<G, true, TX + 40, kTY>(a, st) : var_launch4<G, false, TX + 40, kTY>(a, st);
+    if (!hasw) return var_launch4<G, false, TX + 20, kTY>(a, st);
+    return ty == 2 ? var_launch4<G, true, TX + 20, 2>(a, st)
+                   : ty == 8 ? var_launch4<G, true, TX + 20, 8>(a, st) : var_launch4<G, true, TX + 20, kTY>(a, st);
 }
 
 VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0) {
@@ -366,63 +332,17 @@ VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const s
     return a;
 }
 
-// Batched variance filter: r <= 10 (row stride 84); tile rows 2, 4 or 8 with
-// feature weights, 4 without.
-template <int G, bool HASW, int TY>
-static const void *var_bk(size_t *lds, int) {
-    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
-    *lds = (size_t)TY * (SINGLES + 12 * G) * (TX + 20) * 4;
-    return (const void *)k_var_filter_batch<G, HASW, TX + 20, TY>;
-}
-
-template <int G>
-static const void *var_bk_g(bool hasw, int ty, size_t *lds) {
-    if (!hasw) return ty == 4 ? var_bk<G, false, 4>(lds, 0) : nullptr;
-    switch (ty) {
-        case 2: return var_bk<G, true, 2>(lds, 0);
-        case 4: return var_bk<G, true, 4>(lds, 0);
-        case 8: return var_bk<G, true, 8>(lds, 0);
-        default: return nullptr;
-    }
-}
-
-const void *var_batch_kernel(int G, bool hasw, int r, int ty, size_t *lds, int) {
-    if (r > 10) return nullptr;
-    return G == 3 ? var_bk_g<3>(hasw, ty, lds) : G == 2 ? var_bk_g<2>(hasw, ty, lds) : var_bk_g<1>(hasw, ty, lds);
-}
-
-int launch_var_batch(int G, bool hasw, int r, int ty, const VarArgs *args, const int *prefix, int nimg, int grid,
-                     hipStream_t st) {
-    size_t lds = 0;
-    const void *k = var_batch_kernel(G, hasw, r, ty, &lds, 0);
-    if (!k) return fail(STATER_E_UNSUPPORTED, "batch variance filter: configuration not built");
-    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
-    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(64 * ty), params, lds, st));
-    return STATER_OK;
-}
-
-int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
-                      hipStream_t st) {
-    VarArgs a{};
-    a.n = bs->n;
-    a.m2 = bs->stat(0, ST_M2, 0);
-    a.lnS2 = bs->lnS2; a.se2 = bs->se2; a.m2C = bs->m2C;
-    a.varD = bs->varD; a.varCD = bs->varCD;
-    a.f = make_feats(g);
-    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius;
-    a.plane = bs->plane;
-    a.set_stride = (size_t)N_STATS * 3 * bs->plane;
-    a.z2 = p->var_z * p->var_z;
-    a.sp_scale = p->ds_factor * kLog2e;
+int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, hipStream_t st,
+                      int ty) {
+    VarArgs a = make_var_args(bs, p, g, 0);
     const bool hasw = a.f.nf > 0 || p->ds_factor != 0.f;
-    const dim3 grid(xcd_grid(((bs->W + TX - 1) / TX) * ((bs->H + kTY - 1) / kTY)));
     LaunchTimer t(bs->ctx, st, "var_filter");
     for (int g0 = 0; g0 < bs->S;) {
         const int left = bs->S - g0;
         const int G = left >= 3 ? 3 : left;
         a.g0 = g0;
-        int rc = G == 3 ? var_launch<3>(a, hasw, grid, st) : G == 2 ? var_launch<2>(a, hasw, grid, st)
-                                                                     : var_launch<1>(a, hasw, grid, st);
+        int rc = G == 3 ? var_launch<3>(a, hasw, ty, st) : G == 2 ? var_launch<2>(a, hasw, ty, st)
+                                                                   : var_launch<1>(a, hasw, ty, st);
         if (rc) return rc;
         g0 += G;
     }
@@ -755,56 +675,42 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
     }
 }
 
-template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    1, lds_waves_per_simd(kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4, 256)))) k_welch(WelchArgs a) {
+// One tile of 64 x TY pixels per workgroup, as for the variance filter.
+template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(TY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4, 64 * TY)))) k_welch(WelchArgs a) {
     extern __shared__ float ring[];
     const int tiles_x = (a.W + TX - 1) / TX;
-    const int t = xcd_tile(tiles_x * ((a.H + kTY - 1) / kTY));
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
     if (t < 0) return;
-    welch_tile<S, HASW, PLAIN, MODE3, HWC, kTY>(a, t % tiles_x, t / tiles_x, ring);
+    welch_tile<S, HASW, PLAIN, MODE3, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
-template <int S, bool HASW, int HWC, int TY>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
-    1, lds_waves_per_simd(TY * WelchLayout<S, HASW, false>::FLOATS * HWC * 4, 64 * TY))))
-    k_welch_batch(const WelchArgs *args, const int *prefix, int nimg) {
-    extern __shared__ float ring[];
-    const int ntiles = prefix[nimg];
-    XcdRange r = xcd_range(ntiles);
-    int img = 0;
-    for (int w = r.begin; w < r.end; w += r.step) {
-        while (w >= prefix[img + 1]) ++img;
-        const WelchArgs &a = args[img];
-        const int t = w - prefix[img];
-        const int tiles_x = (a.W + TX - 1) / TX;
-        const int by = t / tiles_x, bx = t - by * tiles_x;
-        welch_tile<S, HASW, false, false, HWC, TY>(a, bx, by, ring);
-        __syncthreads();  // the next tile restages the ring
-    }
-}
-
-template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC>
-static int welch_launch5(const WelchArgs &a, dim3 grid, hipStream_t st) {
-    const size_t lds = (size_t)kTY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
-    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, MODE3, HWC>,
+template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC, int TY>
+static int welch_launch5(const WelchArgs &a, hipStream_t st) {
+    const size_t lds = (size_t)TY * WelchLayout<S, HASW, PLAIN>::FLOATS * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch<S, HASW, PLAIN, MODE3, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, MODE3, HWC>), grid, dim3(256), lds, st, a);
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    hipLaunchKernelGGL((k_welch<S, HASW, PLAIN, MODE3, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
 template <int S, bool HASW, bool PLAIN, bool MODE3>
-static int welch_launch3(const WelchArgs &a, dim3 grid, hipStream_t st) {
-    return a.r <= 10 ? welch_launch5<S, HASW, PLAIN, MODE3, TX + 20>(a, grid, st)
-                     : welch_launch5<S, HASW, PLAIN, MODE3, TX + 40>(a, grid, st);
+static int welch_launch3(const WelchArgs &a, int ty, hipStream_t st) {
+    if (a.r > 10) return welch_launch5<S, HASW, PLAIN, MODE3, TX + 40, kTY>(a, st);
+    if (HASW && !PLAIN && !MODE3 && ty != kTY)  // tile heights 2 and 8: the tuned mode-2 filter
+        return ty == 2 ? welch_launch5<S, HASW, PLAIN, MODE3, TX + 20, 2>(a, st)
+                       : welch_launch5<S, HASW, PLAIN, MODE3, TX + 20, 8>(a, st);
+    return welch_launch5<S, HASW, PLAIN, MODE3, TX + 20, kTY>(a, st);
 }
 
 template <int S>
-static int welch_launch2(const WelchArgs &a, bool hasw, dim3 grid, hipStream_t st) {
+static int welch_launch2(const WelchArgs &a, bool hasw, int ty, hipStream_t st) {
     if (S > 1 && a.mode3)
-        return hasw ? welch_launch3<S, true, false, S != 1>(a, grid, st) : welch_launch3<S, false, false, S != 1>(a, grid, st);
-    return hasw ? welch_launch3<S, true, false, false>(a, grid, st) : welch_launch3<S, false, false, false>(a, grid, st);
+        return hasw ? welch_launch3<S, true, false, S != 1>(a, ty, st) : welch_launch3<S, false, false, S != 1>(a, ty, st);
+    return hasw ? welch_launch3<S, true, false, false>(a, ty, st) : welch_launch3<S, false, false, false>(a, ty, st);
 }
 
 static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, bool plain,
@@ -843,48 +749,9 @@ WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, con
     return a;
 }
 
-template <int S, bool HASW, int TY>
-static const void *welch_bk(size_t *lds) {
-    *lds = (size_t)TY * WelchLayout<S, HASW, false>::FLOATS * (TX + 20) * 4;
-    return (const void *)k_welch_batch<S, HASW, TX + 20, TY>;
-}
-
-template <int S>
-static const void *welch_bk_s(bool hasw, int ty, size_t *lds) {
-    if (!hasw) return ty == 4 ? welch_bk<S, false, 4>(lds) : nullptr;
-    switch (ty) {
-        case 2: return welch_bk<S, true, 2>(lds);
-        case 4: return welch_bk<S, true, 4>(lds);
-        case 8: return welch_bk<S, true, 8>(lds);
-        default: return nullptr;
-    }
-}
-
-const void *welch_batch_kernel(int S, bool hasw, int r, int ty, size_t *lds) {
-    if (r > 10) return nullptr;
-    switch (S) {
-        case 1: return welch_bk_s<1>(hasw, ty, lds);
-        case 2: return welch_bk_s<2>(hasw, ty, lds);
-        case 3: return welch_bk_s<3>(hasw, ty, lds);
-        case 4: return welch_bk_s<4>(hasw, ty, lds);
-        case 5: return welch_bk_s<5>(hasw, ty, lds);
-        default: return welch_bk_s<6>(hasw, ty, lds);
-    }
-}
-
-int launch_welch_batch(int S, bool hasw, int r, int ty, const WelchArgs *args, const int *prefix, int nimg,
-                       int grid, hipStream_t st) {
-    size_t lds = 0;
-    const void *k = welch_batch_kernel(S, hasw, r, ty, &lds);
-    if (!k) return fail(STATER_E_UNSUPPORTED, "batch Welch filter: configuration not built");
-    void *params[] = {(void *)&args, (void *)&prefix, (void *)&nimg};
-    STATER_HIP(hipLaunchKernel(k, dim3(grid), dim3(64 * ty), params, lds, st));
-    return STATER_OK;
-}
-
 static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                         const stater_plane *film, const stater_plane *film_out, bool plain, uint8_t *dec,
-                        hipStream_t st) {
+                        hipStream_t st, int ty) {
     WelchArgs a{};
     fill_welch_args(bs, p, g, plain, a);
     if (film) {
@@ -902,27 +769,26 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     a.dec = dec;
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
-    const dim3 grid(xcd_grid(((bs->W + TX - 1) / TX) * ((bs->H + kTY - 1) / kTY)));
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
-    if (plain) return welch_launch3<1, false, true, false>(a, grid, st);
+    if (plain) return welch_launch3<1, false, true, false>(a, kTY, st);
     switch (S) {
-        case 1: return welch_launch2<1>(a, hasw, grid, st);
-        case 2: return welch_launch2<2>(a, hasw, grid, st);
-        case 3: return welch_launch2<3>(a, hasw, grid, st);
-        case 4: return welch_launch2<4>(a, hasw, grid, st);
-        case 5: return welch_launch2<5>(a, hasw, grid, st);
-        default: return welch_launch2<6>(a, hasw, grid, st);
+        case 1: return welch_launch2<1>(a, hasw, ty, st);
+        case 2: return welch_launch2<2>(a, hasw, ty, st);
+        case 3: return welch_launch2<3>(a, hasw, ty, st);
+        case 4: return welch_launch2<4>(a, hasw, ty, st);
+        case 5: return welch_launch2<5>(a, hasw, ty, st);
+        default: return welch_launch2<6>(a, hasw, ty, st);
     }
 }
 
 int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
-                        const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st) {
-    return welch_common(bs, p, g, film, film_out, plain, nullptr, st);
+                        const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st, int ty) {
+    return welch_common(bs, p, g, film, film_out, plain, nullptr, st, ty);
 }
 
 int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                            const stater_plane *film, int plain, uint8_t *dec, hipStream_t st) {
-    return welch_common(bs, p, g, film, nullptr, plain != 0, dec, st);
+    return welch_common(bs, p, g, film, nullptr, plain != 0, dec, st, kTY);
 }
 
 // ============================================================ mean vars

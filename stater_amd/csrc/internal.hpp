@@ -87,11 +87,13 @@ const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
 // kernels (accumulate.hip, denoise.hip, synth.hip)
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
 int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st);
+// ty: tile rows of the filters (2, 4 or 8; 2 and 8 only for the r <= 10
+// feature-weighted mode-2 filters, otherwise 4).
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
-                      hipStream_t st);
+                      hipStream_t st, int ty = 4);
 int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
-                        const stater_plane *film, const stater_plane *film_out, bool plain,
-                        hipStream_t st);
+                        const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st,
+                        int ty = 4);
 int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out,
                       hipStream_t st);
 int launch_mean_vars(stater_bufset *bs, hipStream_t st);

@@ -87,36 +87,13 @@ struct SynthOut {
     int64_t first_pitch;
 };
 
-struct SampleJob {
-    stater_scene s;
-    const float *alb;
-    int64_t alb_pitch, alb_cs;
-    uint32_t first0;
-    SynthOut o;
-};
 
-// Argument builders (the same ones the single-image launches use).
+// Argument builders of the launches.
 AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s);
 PreArgs make_pre_args(stater_bufset *bs, float var_z);
 FeatArgs make_feats(const stater_gbuffers *g);
 VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0);
 WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g);
 SynthOut make_synth_out(const stater_samples *out, const stater_plane *first_plane);
-
-// Batched persistent launches: args and prefix are device arrays; the
-// prefix has nimg + 1 entries (tiles per image, exclusive scan).
-int launch_accumulate_batch(int S, int mm, bool merge, const AccArgs *args, const int *prefix, int nimg, int grid,
-                            hipStream_t st);
-int launch_samples_batch(const SampleJob *jobs, const int *prefix, int nimg, int grid, hipStream_t st);
-int launch_prepass_batch(const PreArgs *args, const int *prefix, int nimg, int grid, hipStream_t st);
-int launch_var_batch(int G, bool hasw, int r, int ty, const VarArgs *args, const int *prefix, int nimg, int grid,
-                     hipStream_t st);
-int launch_welch_batch(int S, bool hasw, int r, int ty, const WelchArgs *args, const int *prefix, int nimg,
-                       int grid, hipStream_t st);
-// Workgroups of a persistent launch: occupancy x CUs, a multiple of 8.
-int batch_grid(stater_ctx *ctx, const void *kernel, int block, size_t lds, int ntiles);
-const void *accumulate_batch_kernel(int S, int mm, bool merge);
-const void *var_batch_kernel(int G, bool hasw, int r, int ty, size_t *lds, int nf);
-const void *welch_batch_kernel(int S, bool hasw, int r, int ty, size_t *lds);
 
 }  // namespace stater

@@ -122,12 +122,8 @@ __global__ void __launch_bounds__(256) k_features(stater_scene s, float *alb, in
 }
 
 
-// Scene is a stater_scene held by value (a kernel argument) or by reference
-// into the batch's job table: a by-value copy of a table entry would go to
-// scratch memory, since the edge arrays are indexed at run time.
-template <class Scene>
-__device__ __forceinline__ void samples_tile(const Scene &s, const float *alb, int64_t alb_pitch, int64_t alb_cs,
-                                             uint32_t first0, const SynthOut &o, int bx, int by) {
+__device__ __forceinline__ void samples_tile(const stater_scene &s, const float *alb, int64_t alb_pitch,
+                                             int64_t alb_cs, uint32_t first0, const SynthOut &o, int bx, int by) {
     const int x = bx * 256 + threadIdx.x;
     const int y = by;
     if (x >= s.width) return;
@@ -204,20 +200,6 @@ __global__ void __launch_bounds__(256) k_samples(stater_scene s, const float *al
 }
 
 
-__global__ void __launch_bounds__(256) k_samples_batch(const SampleJob *__restrict__ jobs,
-                                                       const int *__restrict__ prefix, int nimg) {
-    const int ntiles = prefix[nimg];
-    XcdRange r = xcd_range(ntiles);
-    int img = 0;
-    for (int w = r.begin; w < r.end; w += r.step) {
-        while (w >= prefix[img + 1]) ++img;
-        const SampleJob &j = jobs[img];
-        const int t = w - prefix[img];
-        const int tiles_x = (j.s.width + 255) / 256;
-        const int by = t / tiles_x, bx = t - by * tiles_x;
-        samples_tile(j.s, j.alb, j.alb_pitch, j.alb_cs, j.first0, j.o, bx, by);
-    }
-}
 
 SynthOut make_synth_out(const stater_samples *out, const stater_plane *first_plane) {
     return SynthOut{(float *)out->data, out->pitch_bytes, out->comp_stride_bytes, out->sample_stride_bytes, out->spp,
@@ -237,11 +219,6 @@ int launch_synth_samples(stater_ctx *ctx, const stater_scene *s, const stater_pl
     return STATER_OK;
 }
 
-int launch_samples_batch(const SampleJob *jobs, const int *prefix, int nimg, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_samples_batch, dim3(grid), dim3(256), 0, st, jobs, prefix, nimg);
-    STATER_HIP(hipGetLastError());
-    return STATER_OK;
-}
 
 }  // namespace stater
 

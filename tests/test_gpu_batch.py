@@ -1,5 +1,5 @@
-"""Batched persistent launches over uneven images (plain, tuned, graph replay):
-every image's buffers must be bit-identical to running that image on its own."""
+"""Batches of uneven films (multi-stream, tuned tile heights, graph replay):
+every film's buffers must be bit-identical to running that film on its own."""
 import numpy as np
 import pytest
 
@@ -43,8 +43,9 @@ def test_batch_matches_single_images(ctx):
         imgs.append((bs, sc, feats.albedo, g, spp))
         keep.append(feats)
     batch = sa.Batch(ctx, imgs, p, chunk_spp=16)
-    groups = batch.groups()
-    assert sum(g["images"] for g in groups) == len(SHAPES) and len(groups) == 5
+    films = batch.images()
+    assert len(films) == len(SHAPES) and [f["sets"] for f in films] == [T + 1 for _, _, T, _ in SHAPES]
+    assert len({f["branch"] for f in films}) == 4
     batch.run(use_graph=False)
     ctx.synchronize()
     got = [_outputs(bs, _image_lambdas(T)) for (bs, *_), (_, _, T, _) in zip(imgs, SHAPES)]
@@ -69,5 +70,5 @@ def test_batch_matches_single_images(ctx):
         ctx.synchronize()
         for i, ((bs, *_), (_, _, T, _)) in enumerate(zip(imgs, SHAPES)):
             _same(got[i], _outputs(bs, _image_lambdas(T)))
-    print(batch.groups())
+    print(batch.images())
     batch.close()
