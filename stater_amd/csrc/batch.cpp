@@ -149,7 +149,8 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
         I.has_g = im.gbufs != nullptr;
         if (I.has_g) I.g = *im.gbufs;
         I.spp = im.spp;
-        I.ty_var = I.ty_welch = 4;
+        I.ty_var = kVarTileRows;
+        I.ty_welch = 4;
         // work estimate: samples x sets (generation + accumulation) + window tests
         const double P = (double)im.set->W * im.set->H, S = im.set->S;
         const double win = (2.0 * p->radius + 1) * (2.0 * p->radius + 1);

@@ -88,9 +88,13 @@ const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
 int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st);
 // ty: tile rows of the filters (2, 4 or 8; 2 and 8 only for the r <= 10
-// feature-weighted mode-2 filters, otherwise 4).
+// feature-weighted mode-2 filters, otherwise 4).  The variance filter's
+// default is 8 rows: its ring of halo rows is restaged for every tile, and at
+// 8 rows (one 512-thread tile per CU) that costs 3.5 staged rows per output row
+// instead of 6.
+constexpr int kVarTileRows = 8;
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
-                      hipStream_t st, int ty = 4);
+                      hipStream_t st, int ty = kVarTileRows);
 int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                         const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st,
                         int ty = 4);
