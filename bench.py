@@ -60,6 +60,16 @@ def kernel_work(W, H, spp, S, r):
     }
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
     import oracle
 
@@ -86,7 +96,7 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
         "kind": "port",
         "sample": f"{w}x{h} crop at ({x0},{y0}) of the same 1080p frame, {spp} spp mt5; oracle accumulate "
                   f"(reference FP32 recurrence, 16x16 tiles) {t1 - t0:.2f}s + FP64 prepass/variance/Welch "
-                  f"filters {t2 - t1:.2f}s on {nthreads} threads",
+                  f"filters {t2 - t1:.2f}s on {nthreads} threads of {cpu_model()}",
         "accumulate_gsample_s": round(w * h * spp / (t1 - t0) / 1e9, 4),
     }
 
@@ -229,7 +239,7 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=256)
     ap.add_argument("--radius", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-region", default="800,400,480,270")
+    ap.add_argument("--cpu-region", default="480,270,960,540", help="x0,y0,w,h of the CPU baseline's crop")
     ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2",
                     help="c2: the headline 1080p frame; c4: 48 uneven images, batched vs one image per launch; "
                          "c5: served Poisson arrivals vs one job at a time")

@@ -88,8 +88,10 @@ def counters(path: str, cmd: str) -> str:
             v = sum(cs["SQ_INSTS_VALU"]) / len(cs["SQ_INSTS_VALU"])
             util = f"{v * CYCLES_PER_VALU / (N_SIMD * d * CLOCK_GHZ):.2f}"
         out.append(f"| `{k}` | " + " | ".join(cells) + f" | {util} |")
-    out += ["", "VALU issue utilisation = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x dispatch time x 2.4 GHz); "
-            "dispatch times are those of the counter run."]
+    out += ["", "VALU issue utilisation = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x dispatch time x 2.4 GHz), "
+            "counting every VALU instruction as a full-rate wave64 op on a 16-lane SIMD; dispatch times are those "
+            "of the counter run.  Values near or above 1 mean the kernel issues VALU work every cycle it can "
+            "(instructions that complete in fewer than 4 cycles push the figure past 1)."]
     return "\n".join(out) + "\n"
 
 
