@@ -27,11 +27,9 @@ CLOCK_GHZ = 2.4
 CYCLES_PER_VALU = 4     # a wave64 FP32 VALU instruction on a 16-lane SIMD
 
 # kernel symbol -> the launch names the library's timers use
-TIMER_NAMES = [("k_welch_batch", "welch_batch"), ("k_welch", "welch_filter"),
-               ("k_var_filter_batch", "var_batch"), ("k_var_filter", "var_filter"),
-               ("k_accumulate_merge", "accumulate_merge64"), ("k_accumulate_batch", "accumulate_batch"),
-               ("k_accumulate", "accumulate"), ("k_prepass", "prepass"), ("k_samples", "samples"),
-               ("k_zero", "zero")]
+TIMER_NAMES = [("k_welch", "welch_filter"), ("k_var_filter", "var_filter"),
+               ("k_accumulate_merge", "accumulate_merge64"), ("k_accumulate", "accumulate"),
+               ("k_prepass", "prepass")]
 
 
 def short(name: str) -> str:
