@@ -294,29 +294,30 @@ class BufferSet:
         check(self.lib.stater_bufset_view(self.h, name.encode(), C.byref(p)))
         return p
 
-    def reset(self) -> None:
-        check(self.lib.stater_bufset_reset(self.h, None))
+    # Every call enqueues on `stream` (a hipStream_t handle; None: the context's stream).
+    def reset(self, stream=None) -> None:
+        check(self.lib.stater_bufset_reset(self.h, stream))
 
-    def accumulate(self, samples: Samples, mode: int = STATER_ACC_SEQUENTIAL) -> None:
-        check(self.lib.stater_accumulate(self.h, C.byref(samples), mode, None))
+    def accumulate(self, samples: Samples, mode: int = STATER_ACC_SEQUENTIAL, stream=None) -> None:
+        check(self.lib.stater_accumulate(self.h, C.byref(samples), mode, stream))
 
-    def denoise_vars(self, params: DenoiseParams, gbufs: GBuffers | None = None) -> None:
-        check(self.lib.stater_denoise_vars(self.h, C.byref(params), C.byref(gbufs) if gbufs else None, None))
+    def denoise_vars(self, params: DenoiseParams, gbufs: GBuffers | None = None, stream=None) -> None:
+        check(self.lib.stater_denoise_vars(self.h, C.byref(params), C.byref(gbufs) if gbufs else None, stream))
 
     def denoise_film(self, params: DenoiseParams, gbufs: GBuffers | None = None, film: Plane | None = None,
-                     film_out: Plane | None = None) -> None:
+                     film_out: Plane | None = None, stream=None) -> None:
         check(self.lib.stater_denoise_film(self.h, C.byref(params), C.byref(gbufs) if gbufs else None,
                                            C.byref(film) if film else None, C.byref(film_out) if film_out else None,
-                                           None))
+                                           stream))
 
-    def denoise_plain(self, radius: int = 10, alpha: float = 0.005) -> None:
-        check(self.lib.stater_denoise_plain(self.h, radius, alpha, None))
+    def denoise_plain(self, radius: int = 10, alpha: float = 0.005, stream=None) -> None:
+        check(self.lib.stater_denoise_plain(self.h, radius, alpha, stream))
 
-    def calc_target_spp(self, budget_spp: int, cap: int, out: Plane | None = None) -> None:
-        check(self.lib.stater_calc_target_spp(self.h, budget_spp, cap, C.byref(out) if out else None, None))
+    def calc_target_spp(self, budget_spp: int, cap: int, out: Plane | None = None, stream=None) -> None:
+        check(self.lib.stater_calc_target_spp(self.h, budget_spp, cap, C.byref(out) if out else None, stream))
 
-    def calc_mean_vars(self) -> None:
-        check(self.lib.stater_calc_mean_vars(self.h, None))
+    def calc_mean_vars(self, stream=None) -> None:
+        check(self.lib.stater_calc_mean_vars(self.h, stream))
 
 
 class Batch:

@@ -284,7 +284,7 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
     size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
-        np += (size_t)L.S * (1 + 6 * 3) + 9 + 1;
+        np += (size_t)L.S * (1 + 6 * 3) + 9 + 1 + 3 * kBitsWords;
         L.as_bytes = as_scratch_bytes_for((size_t)W * H);
     }
     L.n_planes = np;
@@ -325,6 +325,7 @@ static stater_bufset *bufset_over(stater_ctx *ctx, const Layout &L, int width, i
         bs->meanD = p; p += 3 * L.plane;
         bs->filmD = p; p += 3 * L.plane;
         bs->targetspp = (int32_t *)p; p += L.plane;
+        bs->welch_bits = (uint32_t *)p; p += 3 * kBitsWords * L.plane;
         bs->as_scratch = p;
         bs->as_scratch_bytes = L.as_bytes;
     }

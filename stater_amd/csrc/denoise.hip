@@ -675,6 +675,344 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
     }
 }
 
+// ====================================================== symmetric Welch filter
+// The mode-2 decision of a pair (every set's Welch test, per channel) is
+// symmetric in p and q, so each unordered pair is tested once.  Pass 1 tests
+// every pixel against its forward half-window (dy > 0, or dy = 0 and dx > 0)
+// and stores one bit per channel and offset (1: accepted, both pixels valid);
+// pass 2 weights the full window and takes a backward neighbour's bit from
+// that neighbour's record.  Each pair's decision is the one the one-pass form
+// takes: the fast test outside the exact-tail band is correct either way
+// round, and the FP64 tail is symmetric.
+
+// Index of forward offset (dx, dy) among the (2r+1)^2/2 forward offsets.
+__device__ __forceinline__ int fwd_index(int dx, int dy, int r) {
+    return dy == 0 ? dx - 1 : r + (dy - 1) * (2 * r + 1) + dx + r;
+}
+
+// Pass 1 LDS slot: ok, n - 1, then (M, V) per set-channel.
+template <int S>
+struct DecideLayout {
+    static constexpr int FLOATS = 2 + 6 * S;
+};
+
+template <int S, int HWC>
+__device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot, int yq, int x0) {
+    const int HW = TX + 2 * a.r;
+    const int items = HW * (1 + 3 * S);
+    float2 *pairs = (float2 *)(slot + 2 * HWC);
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        const int kind = i / HW, col = i - kind * HW;
+        const int xq = x0 - a.r + col;
+        const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+        if (kind == 0) {
+            const int n = in ? a.n[(size_t)yq * a.ld + xq] : 0;
+            slot[col] = n >= 2 ? 1.f : 0.f;
+            slot[HWC + col] = n >= 2 ? (float)(n - 1) : 1.f;
+        } else {
+            const int sc = kind - 1, s = sc / 3, c = sc - s * 3;
+            float M = 0.f, V = 0.f;
+            if (in) welch_load(a, xq, yq, s, c, M, V);
+            pairs[sc * HWC + col] = make_float2(M, V);
+        }
+    }
+}
+
+// Pass 1, one window row dy >= 0 of the lane's pixel: dx from dx0 to r.
+template <int S, bool SMALLN, int HWC>
+__device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot, int tx, int dy, int dx0,
+                                           const float (&Mp)[S][3], const float (&Vp)[S][3], float okp, float nm1p,
+                                           float rp, uint32_t (&word)[3], size_t pix) {
+    const int nfwd = 2 * a.r * a.r + 2 * a.r;
+    for (int dx = dx0; dx <= a.r; ++dx) {
+        const float *q = slot + tx + a.r + dx;
+        const float2 *pq = (const float2 *)(slot + 2 * HWC) + tx + a.r + dx;
+        float2 mv[S][3];
+        const float okq = q[0];
+        const float nm1q = q[HWC];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[s][c] = pq[(s * 3 + c) * HWC];
+        __builtin_amdgcn_sched_barrier(0);
+        const PairR pr = pair_r(rp, __builtin_amdgcn_rcpf(nm1q));
+        const float okpq = okq * okp;
+        float E[3], F[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float e, f;
+                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
+                E[c] = fmaxf(E[c], e);
+                F[c] = fmaxf(F[c], f);
+            }
+        bool near = false;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) near |= !(E[c] > 0.f) && F[c] >= 0.f;
+        if (near && okpq > 0.f) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (E[c] > 0.f || F[c] < 0.f) continue;
+                bool pass = true;
+                for (int s = 0; s < S; ++s) {
+                    const float2 m = pq[(s * 3 + c) * HWC];
+                    float e, f;
+                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], m.x, m.y, pr, e, f);
+                    if (e > 0.f) { pass = false; break; }
+                    if (f >= 0.f && !welch_exact(Mp[s][c], Vp[s][c], nm1p, m.x, m.y, nm1q, a.alpha)) {
+                        pass = false;
+                        break;
+                    }
+                }
+                E[c] = pass ? -1.f : 1.f;
+            }
+        }
+        const int idx = fwd_index(dx, dy, a.r);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) word[c] |= (okpq > 0.f && !(E[c] > 0.f) ? 1u : 0u) << (idx & 31);
+        if ((idx & 31) == 31 || idx == nfwd - 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                a.bits[((size_t)c * kBitsWords + (idx >> 5)) * a.plane + pix] = word[c];
+                word[c] = 0u;
+            }
+        }
+    }
+}
+
+template <int S, int HWC, int TY>
+__device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, float *ring) {
+    constexpr int SLOT = DecideLayout<S>::FLOATS * HWC;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < a.W && y < a.H;
+    const size_t pix = (size_t)y * a.ld + x;
+
+    float Mp[S][3], Vp[S][3];
+    float okp = 0.f, nm1p = 1.f, rp = 0.f;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Mp[s][c] = Vp[s][c] = 0.f;
+    if (active) {
+        const int n = a.n[pix];
+        okp = n >= 2 ? 1.f : 0.f;
+        nm1p = n >= 2 ? (float)(n - 1) : 1.f;
+        rp = __builtin_amdgcn_rcpf(nm1p);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
+                a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
+            }
+    }
+    const bool large_n = welch_window_large_n(a, x0, y0, TY);
+
+    for (int t = 0; t < TY; ++t) decide_stage_row<S, HWC>(a, ring + t * SLOT, y0 + t, x0);
+    __syncthreads();
+    uint32_t word[3] = {0u, 0u, 0u};
+    for (int k = 0; k <= a.r; ++k) {  // dy = k
+        const float *slot = ring + ((ty + k) % TY) * SLOT;
+        const int dx0 = k == 0 ? 1 : -a.r;
+        if (active) {
+            if (large_n) decide_row<S, false, HWC>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            else decide_row<S, true, HWC>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+        }
+        if (k == a.r) break;
+        __syncthreads();
+        decide_stage_row<S, HWC>(a, ring + (k % TY) * SLOT, y0 + k + TY, x0);
+        __syncthreads();
+    }
+}
+
+template <int S, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(TY * DecideLayout<S>::FLOATS * HWC * 4, 64 * TY)))) k_welch_decide(WelchArgs a) {
+    extern __shared__ float ring[];
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    if (t < 0) return;
+    decide_tile<S, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
+}
+
+// Pass 2 LDS slot: features, film[3], then the 3 x kBitsWords decision words.
+template <bool HASW>
+struct AccumLayout {
+    static constexpr int FILM = HASW ? kNF : 0;
+    static constexpr int WORDS = FILM + 3;
+    static constexpr int FLOATS = WORDS + 3 * kBitsWords;
+};
+
+template <bool HASW, int HWC>
+__device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot, int yq, int x0) {
+    using LY = AccumLayout<HASW>;
+    const int HW = TX + 2 * a.r;
+    const int items = HW * (1 + 3 * kBitsWords);
+    uint32_t *words = (uint32_t *)(slot + LY::WORDS * HWC);
+    for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        const int kind = i / HW, col = i - kind * HW;
+        const int xq = x0 - a.r + col;
+        const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
+        if (kind == 0) {
+            if (HASW)
+#pragma unroll
+                for (int k = 0; k < kNF; ++k) slot[k * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                slot[(LY::FILM + c) * HWC + col] =
+                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 +
+                                          c * a.film_cs)
+                       : 0.f;
+        } else {
+            const int w = kind - 1;  // c * kBitsWords + k
+            words[w * HWC + col] = in ? a.bits[(size_t)w * a.plane + (size_t)yq * a.ld + xq] : 0u;
+        }
+    }
+}
+
+// Pass 2, one window row: weights and the decisions of pass 1.
+template <bool HASW, int HWC>
+__device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot, int tx, int x, int y, int dy,
+                                          const float (&gp)[kNF], float (&den)[3], float (&num)[3], size_t pix) {
+    using LY = AccumLayout<HASW>;
+    const int win = 2 * a.r + 1;
+    const float spy = a.sp_scale * (float)(dy * dy);
+    const uint32_t *words = (const uint32_t *)(slot + LY::WORDS * HWC);
+    // this pixel's own words for the forward offsets of the row (at most two per channel)
+    uint32_t own0[3] = {0u, 0u, 0u}, own1[3] = {0u, 0u, 0u};
+    int k0 = 0;
+    if (dy >= 0) {
+        const int i0 = fwd_index(dy == 0 ? 1 : -a.r, dy, a.r), i1 = fwd_index(a.r, dy, a.r);
+        k0 = i0 >> 5;
+        if (a.r > 0)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                own0[c] = a.bits[((size_t)c * kBitsWords + k0) * a.plane + pix];
+                own1[c] = (i1 >> 5) != k0 ? a.bits[((size_t)c * kBitsWords + k0 + 1) * a.plane + pix] : 0u;
+            }
+    }
+    for (int dx = -a.r; dx <= a.r; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const float *q = slot + tx + a.r + dx;
+        float fq[kNF], filmq[3];
+        uint32_t wq[3];
+        const bool fwd = dy > 0 || (dy == 0 && dx > 0);
+        const int idx = fwd ? fwd_index(dx, dy, a.r) : fwd_index(-dx, -dy, a.r);
+#pragma unroll
+        for (int f = 0; f < (HASW ? kNF : 0); ++f) fq[f] = q[f * HWC];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) filmq[c] = q[(LY::FILM + c) * HWC];
+        if (!fwd)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) wq[c] = words[(c * kBitsWords + (idx >> 5)) * HWC + tx + a.r + dx];
+        __builtin_amdgcn_sched_barrier(0);
+        float w = 1.f;
+        if (HASW) {
+            float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+            for (int f = 0; f < kNF; ++f) {
+                const float d = gp[f] - fq[f];
+                lw = __builtin_fmaf(-d, d, lw);
+            }
+            w = __builtin_amdgcn_exp2f(lw);
+        }
+        bool acc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t wd = fwd ? ((idx >> 5) == k0 ? own0[c] : own1[c]) : wq[c];
+            acc[c] = (wd >> (idx & 31)) & 1u;
+            const float wa = acc[c] ? w : 0.f;
+            den[c] += wa;
+            num[c] = __builtin_fmaf(wa, filmq[c], num[c]);
+        }
+        if (a.dec) {
+            const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
+            for (int c = 0; c < 3; ++c) a.dec[kk + c] = acc[c] ? 1 : 0;
+        }
+    }
+}
+
+template <bool HASW, int HWC, int TY>
+__device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, float *ring) {
+    constexpr int SLOT = AccumLayout<HASW>::FLOATS * HWC;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < a.W && y < a.H;
+    const size_t pix = (size_t)y * a.ld + x;
+    float gp[kNF], den[3], num[3];
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) den[c] = 1.f, num[c] = 0.f;  // the pixel itself, weight 1
+    if (active) {
+        if (HASW)
+#pragma unroll
+            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+        if (a.dec) {
+            const int win = 2 * a.r + 1;
+            const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
+            a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
+        }
+    }
+    for (int t = 0; t < TY; ++t) accum_stage_row<HASW, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
+    __syncthreads();
+    for (int k = 0; k <= 2 * a.r; ++k) {
+        const float *slot = ring + ((ty + k) % TY) * SLOT;
+        if (active) accum_row<HASW, HWC>(a, slot, tx, x, y, k - a.r, gp, den, num, pix);
+        if (k == 2 * a.r) break;
+        __syncthreads();
+        accum_stage_row<HASW, HWC>(a, ring + (k % TY) * SLOT, y0 - a.r + k + TY, x0);
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = num[c] / den[c];
+}
+
+template <bool HASW, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(TY * AccumLayout<HASW>::FLOATS * HWC * 4, 64 * TY)))) k_welch_accum(WelchArgs a) {
+    extern __shared__ float ring[];
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    if (t < 0) return;
+    accum_tile<HASW, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
+}
+
+template <int S>
+static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
+    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = kTY;
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    size_t lds = (size_t)TY * DecideLayout<S>::FLOATS * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, TY>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
+    STATER_HIP(hipGetLastError());
+    if (hasw) {
+        lds = (size_t)TY * AccumLayout<true>::FLOATS * HWC * 4;
+        STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<true, HWC, TY>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_welch_accum<true, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
+    } else {
+        lds = (size_t)TY * AccumLayout<false>::FLOATS * HWC * 4;
+        STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<false, HWC, TY>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_welch_accum<false, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
+    }
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
 // One tile of 64 x TY pixels per workgroup, as for the variance filter.
 template <int S, bool HASW, bool PLAIN, bool MODE3, int HWC, int TY>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
@@ -767,10 +1105,21 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
         a.out_cs = film_out->channel_stride_bytes;
     }
     a.dec = dec;
+    a.bits = bs->welch_bits;
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
     if (plain) return welch_launch3<1, false, true, false>(a, kTY, st);
+    if (!a.mode3 && a.bits && a.r >= 1 && a.r <= kBitsMaxRadius) {  // mode 2: each pair tested once
+        switch (S) {
+            case 1: return welch_sym_launch<1>(a, hasw, st);
+            case 2: return welch_sym_launch<2>(a, hasw, st);
+            case 3: return welch_sym_launch<3>(a, hasw, st);
+            case 4: return welch_sym_launch<4>(a, hasw, st);
+            case 5: return welch_sym_launch<5>(a, hasw, st);
+            default: return welch_sym_launch<6>(a, hasw, st);
+        }
+    }
     switch (S) {
         case 1: return welch_launch2<1>(a, hasw, ty, st);
         case 2: return welch_launch2<2>(a, hasw, ty, st);

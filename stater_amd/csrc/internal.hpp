@@ -49,6 +49,9 @@ struct stater_bufset {
           *discr = nullptr;  // [S][3][plane]
     float *meanVar = nullptr, *meanD = nullptr, *filmD = nullptr;  // [3][plane]
     int32_t *targetspp = nullptr;
+    // Welch decisions of the forward half-window (radius <= kBitsMaxRadius):
+    // [3 channels][kBitsWords][plane] words, bit i of word k = offset 32k + i
+    uint32_t *welch_bits = nullptr;
     // adaptive-sampling scratch (inside the set's memory, after the planes)
     void *as_scratch = nullptr;
     size_t as_scratch_bytes = 0;
@@ -61,6 +64,11 @@ struct stater_bufset {
 };
 
 namespace stater {
+
+// The symmetric Welch filter stores one decision bit per channel for every
+// forward neighbour offset (half of the (2r+1)^2 - 1 window) of every pixel.
+constexpr int kBitsMaxRadius = 10;
+constexpr int kBitsWords = (2 * kBitsMaxRadius * kBitsMaxRadius + 2 * kBitsMaxRadius + 31) / 32;  // 7
 
 // Adaptive-sampling scratch of a set: control block, then q, h and inc
 // (4 B per pixel each), then one int64 per 256-pixel block.

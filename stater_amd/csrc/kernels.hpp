@@ -68,6 +68,7 @@ struct WelchArgs {
     float *discr;          // [S][3][plane]
     float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
     uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
+    uint32_t *bits;        // forward half-window decisions [3][kBitsWords][plane] (symmetric form)
     FeatArgs f;
     int W, H, ld, r, mode3;
     size_t plane, set_stride;

@@ -33,6 +33,9 @@ def hip() -> C.CDLL:
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
         _hip.hipGetErrorString.restype = C.c_char_p
+        _hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        _hip.hipStreamDestroy.argtypes = [C.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
     return _hip
 
 
@@ -48,6 +51,30 @@ def device_count() -> int:
     except OSError:
         return 0
     return n.value if rc == 0 else 0
+
+
+class Stream:
+    """An owned non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _ok(hip().hipStreamCreateWithFlags(C.byref(h), 1), "hipStreamCreateWithFlags")
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _ok(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def close(self) -> None:
+        if self.handle:
+            hip().hipStreamSynchronize(self.handle)
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
