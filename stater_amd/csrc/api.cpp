@@ -252,8 +252,9 @@ int stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len, uint
 struct Layout {
     int S = 1, nst = 7, ld = 0;
     size_t plane = 0, n_planes = 0;
-    size_t as_bytes = 0;  // adaptive-sampling scratch after the planes (sets with derived planes)
-    size_t bytes() const { return n_planes * plane * 4 + as_bytes; }
+    size_t as_bytes = 0;    // adaptive-sampling scratch after the planes (sets with derived planes)
+    size_t zero_bytes = 0;  // the variance filter's zero record, last
+    size_t bytes() const { return n_planes * plane * 4 + as_bytes + zero_bytes; }
     float set_lambda[kMaxSets] = {1.f};
     int slot_set[STATER_MAX_LAMBDAS] = {};
 };
@@ -284,8 +285,9 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
     size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
-        np += (size_t)L.S * (1 + 6 * 3) + 9 + 1 + 3 * kBitsWords;
+        np += (size_t)L.S * (1 + 6 * 3) + 9 + 1 + 3 * kBitsWords + kVarRecFloats;
         L.as_bytes = as_scratch_bytes_for((size_t)W * H);
+        L.zero_bytes = kVarZeroBytes;
     }
     L.n_planes = np;
     return STATER_OK;
@@ -326,8 +328,10 @@ static stater_bufset *bufset_over(stater_ctx *ctx, const Layout &L, int width, i
         bs->filmD = p; p += 3 * L.plane;
         bs->targetspp = (int32_t *)p; p += L.plane;
         bs->welch_bits = (uint32_t *)p; p += 3 * kBitsWords * L.plane;
+        bs->var_rec = p; p += (size_t)kVarRecFloats * L.plane;
         bs->as_scratch = p;
         bs->as_scratch_bytes = L.as_bytes;
+        bs->var_zero = (float *)((char *)p + L.as_bytes);
     }
     ctx->live_bufsets.fetch_add(1);
     return bs;

@@ -23,7 +23,8 @@ using namespace stater;
 namespace {
 
 constexpr int kBranches = 4;
-constexpr int kTyOptions[3] = {2, 4, 8};
+constexpr int kVarRowOptions[2] = {4, 8};        // variance filter: output rows per tile
+constexpr int kWelchRowOptions[3] = {2, 4, 8};  // Welch filter
 
 }  // namespace
 
@@ -245,17 +246,19 @@ int stater_batch_tune(stater_batch *b, void *stream) {
         const bool hasw = I.has_g || b->p.ds_factor != 0.f;
         if (!hasw || b->p.radius > 10) continue;  // only 4-row tiles are built there
         float best_v = 1e30f, best_w = 1e30f;
-        for (int ty : kTyOptions) {
+        for (int ty : kVarRowOptions) {
             float ms = 0.f;
             rc = time_one([&] { return launch_var_filter(I.bs, &b->p, gb(I), st, ty); }, &ms);
             if (rc) break;
             if (ms < best_v) { best_v = ms; I.ty_var = ty; }
-            if (b->p.mt_mode == 2) {
-                rc = time_one([&] { return launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, ty); },
-                              &ms);
-                if (rc) break;
-                if (ms < best_w) { best_w = ms; I.ty_welch = ty; }
-            }
+        }
+        for (int ty : kWelchRowOptions) {
+            if (rc || b->p.mt_mode != 2) break;
+            float ms = 0.f;
+            rc = time_one([&] { return launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, ty); },
+                          &ms);
+            if (rc) break;
+            if (ms < best_w) { best_w = ms; I.ty_welch = ty; }
         }
         if (rc) break;
     }

@@ -121,10 +121,21 @@ int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
 }
 
 // ======================================================= variance filter
-// LDS ring layout (floats), per slot: single-float rows of HWC columns
-// (ok flag, the padded feature channels), then float2 rows: for every
-// set-channel (L, Z) = (lnS2, z^2 se2) and (V, C) = (s^2, m2C/(n-1)).  HWC is
-// a compile-time row stride so every LDS access is base + immediate offset.
+// Two passes per group of up to 3 sets.  The pack pass writes one record per
+// pixel: (ok, f0, f1, f2), (f3 .. f6) with features, then (L, Z, V, C) =
+// (lnS2, z^2 se2, s^2, m2C/(n-1)) per set-channel, padded to an odd number of
+// 16-byte items.  The filter pass stages rows of records into an LDS ring with
+// direct global->LDS loads issued by a dedicated loader wavefront, so the
+// loads of the next row are in flight while the other wavefronts test pairs.
+//
+// Each filtering lane handles two vertically adjacent pixels, A (row 2 ty of
+// the tile) and B (row 2 ty + 1): every staged neighbour row a wavefront
+// reads serves both, which halves the LDS reads per output.  Wavefront ty
+// reads row 2 ty + k at step k, so 2 TY - 1 consecutive rows are live; the
+// ring has 2 TY slots, the extra one receiving the next row.  In LDS a
+// record keeps its odd 16-byte stride, so the 16-byte reads of 8 (or 16)
+// consecutive lanes fall in distinct banks, and one base address serves all
+// of a pair's reads as immediate offsets.
 constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
 
 // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
@@ -136,185 +147,345 @@ constexpr int lds_waves_per_simd(int lds_bytes, int threads) {
     const int w = (blocks * (threads / 64) + 3) / 4;
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
-template <int G, bool HASW, int HWC, int TY>
-constexpr int var_lds_bytes() { return TY * ((1 + (HASW ? kNF : 0)) + 12 * G) * HWC * 4; }
 
+template <int G, bool HASW>
+struct VarRec {
+    static constexpr int SINGLE4 = HASW ? 2 : 1;                     // float4 items of ok + features
+    static constexpr int ITEMS = SINGLE4 + 3 * G;                    // float4 items per record
+    static constexpr int FLOAT4S = (ITEMS % 2) ? ITEMS : ITEMS + 1;  // odd
+    static_assert(FLOAT4S * 4 <= kVarRecFloats, "record larger than its planes");
+};
+
+// Ring slot: the halo row's HWC records plus two padding columns for the
+// pair loop's even column count, rounded up to whole 64-item loads (the
+// loader fills everything past the halo with zeros).  The ring holds the
+// 2 TY - 1 live rows and kVarAhead rows in flight.
+constexpr int kVarAhead = 2;
+template <int G, bool HASW, int HWC>
+constexpr int var_loads_per_row() { return ((HWC + 2) * VarRec<G, HASW>::FLOAT4S + 63) / 64; }
+template <int G, bool HASW, int HWC>
+constexpr int var_slot_float4s() { return var_loads_per_row<G, HASW, HWC>() * 64; }
+template <int TY>
+constexpr int var_ring_slots() { return 2 * TY - 1 + kVarAhead; }
+template <int G, bool HASW, int HWC, int TY>
+constexpr int var_lds_bytes() { return var_ring_slots<TY>() * var_slot_float4s<G, HASW, HWC>() * 16; }
+
+// s_waitcnt immediate that waits for vector-memory count <= v only.
+constexpr int waitcnt_vm(int v) { return (v & 15) | ((v >> 4) << 14) | (7 << 4) | (15 << 8); }
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
-template <int G, bool HASW, int HWC>
-__device__ __forceinline__ void var_stage_row(const VarArgs &a, float *slot, int yq, int x0) {
-    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
-    const int HW = TX + 2 * a.r;
-    const int items = HW * (1 + 3 * G);
-    float2 *pairs = (float2 *)(slot + SINGLES * HWC);
-    for (int i = threadIdx.x; i < items; i += blockDim.x) {
-        const int kind = i / HW, col = i - kind * HW;
-        const int xq = x0 - a.r + col;
-        const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
-        if (kind == 0) {
-            slot[col] = in && a.n[(size_t)yq * a.ld + xq] >= 4 ? 1.f : 0.f;
-            if (HASW)
-#pragma unroll
-                for (int k = 0; k < kNF; ++k) slot[(1 + k) * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
+// Item `item` of the record of pixel (x, y), inside the image.
+template <int G, bool HASW>
+__device__ __forceinline__ float4 var_rec_item(const VarArgs &a, int x, int y, int item) {
+    using R = VarRec<G, HASW>;
+    const size_t pix = (size_t)y * a.ld + x;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (item < R::SINGLE4) {
+        auto f = [&](int k) { return k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f; };
+        if (item == 0) {
+            v.x = a.n[pix] >= 4 ? 1.f : 0.f;
+            if (HASW) { v.y = f(0); v.z = f(1); v.w = f(2); }
         } else {
-            const int sc = kind - 1;  // g*3 + c within the group
-            const int g = sc / 3, c = sc - g * 3;
-            float L = 0.f, Z = 0.f, V = 0.f, C = 0.f;
-            if (in) {
-                const size_t pix = (size_t)yq * a.ld + xq;
-                const int n = a.n[pix];
-                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-                if (n >= 4) { L = a.lnS2[o]; Z = a.z2 * a.se2[o]; }
-                if (n >= 2) {
-                    const float inv = 1.f / (float)(n - 1);
-                    V = finite_or_max(a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv);
-                    C = finite_or_max(a.m2C[o] * inv);
-                }
-            }
-            pairs[(sc * 2) * HWC + col] = make_float2(L, Z);
-            pairs[(sc * 2 + 1) * HWC + col] = make_float2(V, C);
+            v = make_float4(f(3), f(4), f(5), f(6));
+        }
+    } else if (item < R::ITEMS) {
+        const int sc = item - R::SINGLE4;  // g*3 + c within the group
+        const int g = sc / 3, c = sc - g * 3;
+        const int n = a.n[pix];
+        const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+        if (n >= 4) { v.x = a.lnS2[o]; v.y = a.z2 * a.se2[o]; }
+        if (n >= 2) {
+            const float inv = 1.f / (float)(n - 1);
+            v.z = finite_or_max(a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv);
+            v.w = finite_or_max(a.m2C[o] * inv);
         }
     }
+    return v;
 }
 
-template <int G, bool HASW, int HWC, int TY>
-__device__ __forceinline__ void var_tile(const VarArgs a, int bx, int by, float *ring) {
-    constexpr int SINGLES = 1 + (HASW ? kNF : 0);
-    constexpr int SLOT = (SINGLES + 12 * G) * HWC;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int x0 = bx * TX, y0 = by * TY;
-    const int x = x0 + tx, y = y0 + ty;
-    const bool active = x < a.W && y < a.H;
+// Pack pass: one thread per 16-byte record item, rows of records at pitch
+// ld (consecutive threads write consecutive items).
+template <int G, bool HASW>
+__global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
+    using R = VarRec<G, HASW>;
+    const int y = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j >= a.W * R::FLOAT4S) return;
+    const int x = j / R::FLOAT4S, item = j - x * R::FLOAT4S;
+    a.rec[(size_t)y * a.ld * R::FLOAT4S + j] = var_rec_item<G, HASW>(a, x, y, item);
+}
 
+// One output pixel's state: its own statistics and the running sums.
+template <int G>
+struct VarPix {
     float Lp[G][3], Zp[G][3], den[G][3], num[G][3], numC[G][3];
     float gp[kNF];
-    float okp = 0.f;
-#pragma unroll
-    for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
-    if (active) {
-        const size_t pix = (size_t)y * a.ld + x;
-        const int n = a.n[pix];
-        okp = n >= 4 ? 1.f : 0.f;
-        if (HASW)
-#pragma unroll
-            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-                Lp[g][c] = n >= 4 ? a.lnS2[o] : 0.f;
-                Zp[g][c] = n >= 4 ? a.z2 * a.se2[o] : 0.f;
-                float V = 0.f, C = 0.f;
-                if (n >= 2) {
-                    const float inv = 1.f / (float)(n - 1);
-                    V = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
-                    C = a.m2C[o] * inv;
-                }
-                den[g][c] = 1.f;  // the pixel itself, weight 1
-                num[g][c] = V;
-                numC[g][c] = C;
-            }
-    }
+};
 
-    for (int t = 0; t < TY; ++t) var_stage_row<G, HASW, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
-    __syncthreads();
-    for (int k = 0; k <= 2 * a.r; ++k) {
-        const int dy = k - a.r;
-        const float *slot = ring + ((ty + k) % TY) * SLOT;
-        if (active && okp > 0.f) {
-            const float spy = a.sp_scale * (float)(dy * dy);
-            for (int dx = -a.r; dx <= a.r; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                const float *q = slot + tx + a.r + dx;
-                const float2 *pq = (const float2 *)(slot + SINGLES * HWC) + tx + a.r + dx;
-                // Every LDS read of the pair is issued before any use, so the
-                // wave waits once per pair rather than once per test.
-                float fq[kNF];
-                float2 lz[G][3], vc[G][3];
-                const float okq = q[0];
+template <int G, bool HASW>
+__device__ __forceinline__ void var_pix_init(const VarArgs &a, int x, int y, bool active, VarPix<G> &P) {
 #pragma unroll
-                for (int f = 0; f < (HASW ? kNF : 0); ++f) fq[f] = q[(1 + f) * HWC];
+    for (int k = 0; k < kNF; ++k) P.gp[k] = 0.f;
 #pragma unroll
-                for (int g = 0; g < G; ++g)
+    for (int g = 0; g < G; ++g)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        lz[g][c] = pq[((g * 3 + c) * 2) * HWC];
-                        vc[g][c] = pq[((g * 3 + c) * 2 + 1) * HWC];
-                    }
-                __builtin_amdgcn_sched_barrier(0);
-                float w = okq;
-                if (HASW) {
-                    float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
-#pragma unroll
-                    for (int f = 0; f < kNF; ++f) {
-                        const float d = gp[f] - fq[f];
-                        lw = __builtin_fmaf(-d, d, lw);
-                    }
-                    w *= __builtin_amdgcn_exp2f(lw);
-                }
-#pragma unroll
-                for (int g = 0; g < G; ++g)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const float dl = Lp[g][c] - lz[g][c].x;
-                        // reject iff dl^2 - z^2 se2_q > z^2 se2_p; NaN/inf operands abstain (accept)
-                        const float wa = (__builtin_fmaf(dl, dl, -lz[g][c].y) > Zp[g][c]) ? 0.f : w;
-                        den[g][c] += wa;
-                        num[g][c] = __builtin_fmaf(wa, vc[g][c].x, num[g][c]);
-                        numC[g][c] = __builtin_fmaf(wa, vc[g][c].y, numC[g][c]);
-                    }
-            }
+        for (int c = 0; c < 3; ++c) {
+            P.Lp[g][c] = P.Zp[g][c] = P.num[g][c] = P.numC[g][c] = 0.f;
+            P.den[g][c] = 1.f;  // the pixel itself, weight 1
         }
-        if (k == 2 * a.r) break;
-        __syncthreads();
-        var_stage_row<G, HASW, HWC>(a, ring + (k % TY) * SLOT, y0 - a.r + k + TY, x0);
-        __syncthreads();
-    }
     if (!active) return;
     const size_t pix = (size_t)y * a.ld + x;
+    const int n = a.n[pix];
+    if (HASW)
+#pragma unroll
+        for (int k = 0; k < kNF; ++k) P.gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-            a.varD[o] = num[g][c] / den[g][c];
-            a.varCD[o] = numC[g][c] / den[g][c];
+            P.Lp[g][c] = n >= 4 ? a.lnS2[o] : 0.f;
+            P.Zp[g][c] = n >= 4 ? a.z2 * a.se2[o] : 0.f;
+            if (n >= 2) {
+                const float inv = 1.f / (float)(n - 1);
+                P.num[g][c] = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
+                P.numC[g][c] = a.m2C[o] * inv;
+            }
         }
 }
 
-// One tile of 64 x TY pixels per workgroup (one wavefront per row).  Tile
-// heights 2 and 8 are built for the r <= 10 feature-weighted filter (chosen
-// per film by the batch tuner); everything else runs 4.
+template <int G, bool HASW>
+struct VarCol {
+    float4 s0, s1, v[G][3];
+};
+
+template <int G, bool HASW>
+__device__ __forceinline__ void var_col_load(const float4 *q, VarCol<G, HASW> &c) {
+    using R = VarRec<G, HASW>;
+    c.s0 = q[0];
+    c.s1 = HASW ? q[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c.v[g][k] = q[R::SINGLE4 + g * 3 + k];
+}
+
+template <bool HASW>
+__device__ __forceinline__ float var_weight(const VarArgs &a, const float (&gp)[kNF], float okq,
+                                            const float (&fq)[kNF], float lw0, float m, float dx2) {
+    float w = okq;
+    if (HASW) {
+        float lw = __builtin_fmaf(a.sp_scale, dx2, lw0);
+#pragma unroll
+        for (int f = 0; f < kNF; ++f) {
+            const float d = gp[f] - fq[f];
+            lw = __builtin_fmaf(-d, d, lw);
+        }
+        w *= __builtin_amdgcn_exp2f(lw);
+    } else {
+        w *= m;
+    }
+    return w;
+}
+
+__device__ __forceinline__ void var_test(float Lp, float Zp, float4 q, float w, float &den, float &num, float &numC) {
+    const float dl = Lp - q.x;
+    // reject iff dl^2 - z^2 se2_q > z^2 se2_p; NaN/inf operands abstain (accept)
+    const float wa = (__builtin_fmaf(dl, dl, -q.y) > Zp) ? 0.f : w;
+    den += wa;
+    num = __builtin_fmaf(wa, q.z, num);
+    numC = __builtin_fmaf(wa, q.w, numC);
+}
+
+// The pairs of neighbour column dx with A and B: both weights, then every
+// set-channel's log-variance test for A and for B side by side.  A pixel
+// whose window does not reach this row (offA/offB), and the pixel itself
+// (dx = 0 on its own row), get weight exactly 0, which leaves every sum as it
+// is.  lw0 = sp_scale dy^2, or -inf for weight 0; m (unweighted filter) is 1
+// or 0.
+template <int G, bool HASW>
+__device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, HASW> &c, int dx, bool offA, bool offB,
+                                             bool selfA, bool selfB, float lwA, float lwB, VarPix<G> &A,
+                                             VarPix<G> &B) {
+    const float fq[kNF] = {c.s0.y, c.s0.z, c.s0.w, c.s1.x, c.s1.y, c.s1.z, c.s1.w};
+    const float dx2 = (float)(dx * dx);
+    const bool zA = offA || (selfA && dx == 0), zB = offB || (selfB && dx == 0);
+    const float wA = var_weight<HASW>(a, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : 1.f, dx2);
+    const float wB = var_weight<HASW>(a, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : 1.f, dx2);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            var_test(A.Lp[g][k], A.Zp[g][k], c.v[g][k], wA, A.den[g][k], A.num[g][k], A.numC[g][k]);
+            var_test(B.Lp[g][k], B.Zp[g][k], c.v[g][k], wB, B.den[g][k], B.num[g][k], B.numC[g][k]);
+        }
+}
+
+// Step k of the tile: the staged row at dy = k - r from A (k <= 2r) and
+// k - r - 1 from B (k >= 1).  The LDS reads of column dx + 1 are issued
+// before the tests of column dx, in two register sets that alternate.  The
+// loop runs an even number of columns: the 2r + 2nd (dx = r + 1) is the
+// slot's zeroed padding column, tested with weight 0; the reads of the last
+// iteration's next column land in the second padding column and are unused.
+template <int G, bool HASW>
+__device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, int tx, int k, VarPix<G> &A,
+                                        VarPix<G> &B) {
+    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
+    const int dyA = k - a.r, dyB = k - a.r - 1;
+    const bool offA = k > 2 * a.r, offB = k < 1;  // row outside the pixel's window
+    const float lwA = a.sp_scale * (float)(dyA * dyA), lwB = a.sp_scale * (float)(dyB * dyB);
+    const bool selfA = dyA == 0, selfB = dyB == 0;
+    const float4 *q = slot + tx * F4;  // column of dx = -r
+    VarCol<G, HASW> X, Y;
+    var_col_load<G, HASW>(q, X);
+    for (int dx = -a.r; dx <= a.r; dx += 2, q += 2 * F4) {
+        var_col_load<G, HASW>(q + F4, Y);
+        __builtin_amdgcn_sched_barrier(0);
+        var_col_test<G, HASW>(a, X, dx, offA, offB, selfA, selfB, lwA, lwB, A, B);
+        var_col_load<G, HASW>(q + 2 * F4, X);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool pad = dx + 1 > a.r;
+        var_col_test<G, HASW>(a, Y, dx + 1, offA || pad, offB || pad, selfA, selfB, lwA, lwB, A, B);
+    }
+}
+
+// A pixel with fewer than 4 samples is not filtered: it keeps its own
+// variance (the initial sums, den = 1), read again here so that the pair
+// loop runs for it like for every other lane.  Both pixels of the lane are
+// written in one pass over the set-channels, with plain per-pixel values
+// (no selection between the two pixels' state).
+template <int G>
+__device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool actA, bool actB, const VarPix<G> &A,
+                                          const VarPix<G> &B) {
+    const size_t pA = (size_t)yA * a.ld + x, pB = pA + a.ld;
+    const int nA = actA ? a.n[pA] : 4, nB = actB ? a.n[pB] : 4;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t off = ((size_t)(a.g0 + g) * 3 + c) * a.plane;
+            const size_t m2off = (size_t)(a.g0 + g) * a.set_stride + c * a.plane;
+            float VA = A.num[g][c] / A.den[g][c], CA = A.numC[g][c] / A.den[g][c];
+            float VB = B.num[g][c] / B.den[g][c], CB = B.numC[g][c] / B.den[g][c];
+            if (nA < 4) {
+                const float inv = nA >= 2 ? 1.f / (float)(nA - 1) : 0.f;
+                VA = nA >= 2 ? a.m2[m2off + pA] * inv : 0.f;
+                CA = nA >= 2 ? a.m2C[off + pA] * inv : 0.f;
+            }
+            if (nB < 4) {
+                const float inv = nB >= 2 ? 1.f / (float)(nB - 1) : 0.f;
+                VB = nB >= 2 ? a.m2[m2off + pB] * inv : 0.f;
+                CB = nB >= 2 ? a.m2C[off + pB] * inv : 0.f;
+            }
+            if (actA) { a.varD[off + pA] = VA; a.varCD[off + pA] = CA; }
+            if (actB) { a.varD[off + pB] = VB; a.varCD[off + pB] = CB; }
+        }
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Loader wavefront: row yq of the tile's halo (columns x0 - r .. x0 + 63 + r)
+// into one ring slot with the same number of 64-item direct loads for every
+// row, so that a count of outstanding loads names a number of rows.  Items
+// of columns and rows outside the image, and every item past the halo, read
+// the zero record.
+template <int G, bool HASW, int HWC>
+__device__ __forceinline__ void var_load_row(const VarArgs &a, float4 *dst, int yq, int x0, int lane) {
+    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
+    const int nch = (TX + 2 * a.r) * F4;
+    const bool row_in = yq >= 0 && yq < a.H;
+    const float4 *row = a.rec + (size_t)(row_in ? yq : 0) * a.ld * F4;
+#pragma unroll
+    for (int i = 0; i < var_loads_per_row<G, HASW, HWC>(); ++i) {
+        const int ch = i * 64 + lane;
+        const int col = ch / F4, item = ch - col * F4;
+        const int xq = x0 - a.r + col;
+        const bool real = ch < nch && row_in && xq >= 0 && xq < a.W;
+        const float4 *src = real ? row + (size_t)xq * F4 + item : a.zero + item;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(dst + i * 64), 16, 0, 0);
+    }
+}
+
+// One tile of 64 x 2 TY pixels per workgroup: TY filtering wavefronts (two
+// rows each) and one loader wavefront.  Wavefront ty reads row 2 ty + k at
+// step k, so rows k .. k + P - 1 (P = 2 TY - 1) are live during step k while
+// the loader keeps kVarAhead rows in flight; row j lives in slot j mod
+// (P + kVarAhead).  Both roles pass the same 2r + 3 barriers: the first once
+// rows 0 .. P - 1 have landed, then one at the end of every step k, by which
+// row k + P has landed (it replaces nothing live: its slot held row k - 1 or
+// was never used).  Tiles of 4 and 8 rows are built for the r <= 10
+// feature-weighted filter (chosen per film by the batch tuner); everything
+// else runs 4 rows.
 template <int G, bool HASW, int HWC, int TY>
-__global__ void __launch_bounds__(64 * TY)
-    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY>(), 64 * TY))))
+__global__ void __launch_bounds__(64 * (TY + 1))
+    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY>(), 64 * (TY + 1)))))
     k_var_filter(VarArgs a) {
-    extern __shared__ float ring[];
+    extern __shared__ float4 vring[];
+    constexpr int P = 2 * TY - 1, NS = var_ring_slots<TY>();
+    constexpr int SLOT = var_slot_float4s<G, HASW, HWC>();
+    constexpr int AHEAD_LOADS = var_loads_per_row<G, HASW, HWC>() * (kVarAhead - 1);
     const int tiles_x = (a.W + TX - 1) / TX;
-    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    const int t = xcd_tile(tiles_x * ((a.H + 2 * TY - 1) / (2 * TY)));
     if (t < 0) return;
-    var_tile<G, HASW, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
+    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * 2 * TY;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int last = 2 * a.r + 1;  // steps 0 .. last
+
+    // The loader meets the filtering wavefronts' __syncthreads() with a bare
+    // s_barrier: a workgroup fence there would wait for the rows still in
+    // flight too.  Its own counter wait covers the rows the next step reads.
+    if (wave == TY) {  // loader
+        for (int j = 0; j < P + kVarAhead - 1; ++j)
+            var_load_row<G, HASW, HWC>(a, vring + j * SLOT, y0 - a.r + j, x0, lane);
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // rows 0 .. P - 1 have landed
+        __builtin_amdgcn_s_barrier();
+        for (int k = 0; k <= last; ++k) {
+            const int j = k + P + kVarAhead - 1;
+            var_load_row<G, HASW, HWC>(a, vring + (j % NS) * SLOT, y0 - a.r + j, x0, lane);
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // row k + P has landed
+            __builtin_amdgcn_s_barrier();
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // nothing may land in LDS after the workgroup ends
+        return;
+    }
+
+    const int x = x0 + lane, yA = y0 + 2 * wave;
+    const bool actA = x < a.W && yA < a.H, actB = x < a.W && yA + 1 < a.H;
+    VarPix<G> A, B;
+    var_pix_init<G, HASW>(a, x, yA, actA, A);
+    var_pix_init<G, HASW>(a, x, yA + 1, actB, B);
+    __syncthreads();
+    for (int k = 0; k <= last; ++k) {
+        const float4 *slot = vring + ((2 * wave + k) % NS) * SLOT;
+        if (actA) var_row<G, HASW>(a, slot, lane, k, A, B);
+        __syncthreads();
+    }
+    var_store<G>(a, x, yA, actA, actB, A, B);
 }
 
 template <int G, bool HASW, int HWC, int TY>
 static int var_launch4(const VarArgs &a, hipStream_t st) {
+    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
+    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W * F4 + 255) / 256, a.H), dim3(256), 0, st, a,
+                       const_cast<float4 *>(a.zero));
+    STATER_HIP(hipGetLastError());
     const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY>();
     STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
-    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + 2 * TY - 1) / (2 * TY))));
+    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC, TY>), grid, dim3(64 * (TY + 1)), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
 
+// rows: output rows per tile (4 or 8; two per filtering wavefront).
 template <int G>
-static int var_launch(const VarArgs &a, bool hasw, int ty, hipStream_t st) {
-    if (a.r > 10) return hasw ? var_launch4<G, true, TX + 40, kTY>(a, st) : var_launch4<G, false, TX + 40, kTY>(a, st);
-    if (!hasw) return var_launch4<G, false, TX + 20, kTY>(a, st);
-    return ty == 2 ? var_launch4<G, true, TX + 20, 2>(a, st)
-                   : ty == 8 ? var_launch4<G, true, TX + 20, 8>(a, st) : var_launch4<G, true, TX + 20, kTY>(a, st);
+static int var_launch(const VarArgs &a, bool hasw, int rows, hipStream_t st) {
+    if (a.r > 10) return hasw ? var_launch4<G, true, TX + 40, 2>(a, st) : var_launch4<G, false, TX + 40, 2>(a, st);
+    if (!hasw) return var_launch4<G, false, TX + 20, 2>(a, st);
+    return rows == 8 ? var_launch4<G, true, TX + 20, 4>(a, st) : var_launch4<G, true, TX + 20, 2>(a, st);
 }
 
 VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0) {
@@ -329,6 +500,8 @@ VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const s
     a.set_stride = (size_t)N_STATS * 3 * bs->plane;
     a.z2 = p->var_z * p->var_z;
     a.sp_scale = p->ds_factor * kLog2e;
+    a.rec = (float4 *)bs->var_rec;
+    a.zero = (const float4 *)bs->var_zero;
     return a;
 }
 

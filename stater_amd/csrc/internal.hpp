@@ -52,6 +52,8 @@ struct stater_bufset {
     // Welch decisions of the forward half-window (radius <= kBitsMaxRadius):
     // [3 channels][kBitsWords][plane] words, bit i of word k = offset 32k + i
     uint32_t *welch_bits = nullptr;
+    float *var_rec = nullptr;   // packed variance-filter records of one set group, kVarRecFloats per pixel
+    float *var_zero = nullptr;  // one all-zero record (the filter's halo outside the image)
     // adaptive-sampling scratch (inside the set's memory, after the planes)
     void *as_scratch = nullptr;
     size_t as_scratch_bytes = 0;
@@ -72,6 +74,12 @@ constexpr int kBitsWords = (2 * kBitsMaxRadius * kBitsMaxRadius + 2 * kBitsMaxRa
 
 // Adaptive-sampling scratch of a set: control block, then q, h and inc
 // (4 B per pixel each), then one int64 per 256-pixel block.
+// Variance filter input records: ok, 7 features, then (lnS2, z^2 se2, s^2,
+// m2C/(n-1)) per set-channel of a group of up to 3 sets, padded to an odd
+// number of 16-byte items: 44 floats per pixel.
+constexpr int kVarRecFloats = 44;
+constexpr size_t kVarZeroBytes = 256;
+
 constexpr size_t kAsCtlBytes = 512;
 inline size_t as_scratch_bytes_for(size_t pixels) {
     const size_t nb = (pixels + 255) / 256;
@@ -95,11 +103,10 @@ const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
 // kernels (accumulate.hip, denoise.hip, synth.hip)
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
 int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st);
-// ty: tile rows of the filters (2, 4 or 8; 2 and 8 only for the r <= 10
-// feature-weighted mode-2 filters, otherwise 4).  The variance filter's
-// default is 8 rows: its ring of halo rows is restaged for every tile, and at
-// 8 rows (one 512-thread tile per CU) that costs 3.5 staged rows per output row
-// instead of 6.
+// ty: output rows per tile.  Variance filter: 4 or 8 (two rows per
+// filtering wavefront; 8 only for the r <= 10 feature-weighted filter,
+// otherwise 4).  Welch filter: 2, 4 or 8 (one row per wavefront; 2 and 8 only for the
+// r <= 10 feature-weighted mode-2 filter, otherwise 4).
 constexpr int kVarTileRows = 8;
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                       hipStream_t st, int ty = kVarTileRows);

@@ -53,6 +53,8 @@ struct VarArgs {
     int W, H, ld, r, g0;       // g0: first set of this group
     size_t plane, set_stride;  // distance between sets of the statistics block
     float z2, sp_scale;        // sp_scale = ds_factor * log2 e
+    float4 *rec;               // packed records of the set group (written by the pack pass)
+    const float4 *zero;        // one all-zero record
 };
 
 struct WelchArgs {

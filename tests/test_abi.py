@@ -82,12 +82,15 @@ def test_bufset_layout_size(lib):
     mt5 = (0.01, 0.25, 0.5, 0.75, 1.0, 3.0)
     S = 6                                  # untransformed + 5 distinct Box-Cox lambdas
     base = 1 + S * 7 * 3 + 3 + (S - 1) * 3  # n, moments, reference point, shifted means
-    derived = S * (1 + 6 * 3) + 9 + 1 + 21  # jb, 6 per-set buffers, meanVar/meanD/filmD, targetspp, Welch bits
+    # jb, 6 per-set buffers, meanVar/meanD/filmD, targetspp, Welch bits, variance-filter records
+    derived = S * (1 + 6 * 3) + 9 + 1 + 21 + 44
     ld = 1920                               # already a multiple of 64 floats
-    assert _bytes_for(lib, 1920, 1080, mt5) == (base + derived) * ld * 1080 * 4 + _as_scratch(1920 * 1080)
+    zero = 256                              # the variance filter's zero record, last
+    assert _bytes_for(lib, 1920, 1080, mt5) == (base + derived) * ld * 1080 * 4 + _as_scratch(1920 * 1080) + zero
     assert _bytes_for(lib, 1920, 1080, mt5, flags=sa.STATER_BUFSET_NO_DERIVED) == base * ld * 1080 * 4
     # rows padded to 256 bytes; repeated lambdas share one set; lambda 1 is the identity
-    assert _bytes_for(lib, 100, 10, (1.0,), mm=2) == (1 + 21 + 3 + 0 + 19 + 10 + 21) * 128 * 10 * 4 + _as_scratch(1000)
+    assert _bytes_for(lib, 100, 10, (1.0,), mm=2) == \
+        (1 + 21 + 3 + 0 + 19 + 10 + 21 + 44) * 128 * 10 * 4 + _as_scratch(1000) + zero
     assert _bytes_for(lib, 100, 10, (0.5, 0.5, 1.0)) == _bytes_for(lib, 100, 10, (0.5,))
 
 

@@ -123,6 +123,48 @@ def test_prepass_and_variance_filter(ctx):
     assert np.max(np.abs(gCD[ok] - vCD[ok]) / np.maximum(np.abs(vCD[ok]), 1e-30)) < FILTER_RTOL
 
 
+@pytest.mark.parametrize("lambdas,r,ragged,weighted", [
+    (MT5_LAMBDAS, 10, True, True),         # pixels with n < 4 keep their own variance; two 3-set groups
+    (MT5_LAMBDAS, 12, False, True),        # r > 10: the wide-halo build
+    ((0.5,), 3, True, True),               # one group of 2 sets
+    ((0.25, 0.5, 3.0), 5, False, False),   # groups of 3 and 1 sets, no features and ds_factor 0
+])
+def test_variance_filter_cases(ctx, lambdas, r, ragged, weighted):
+    W, H = 96, 64
+    counts = _ragged_counts(W, H) if ragged else None
+    sc, chunk, feats = generate(ctx, 2, 3, W, H, 64)
+    bs = sa.BufferSet(ctx, W, H, lambdas, max_moment=5)
+    cplane = None
+    if counts is not None:
+        cbuf = hipmem.DeviceBuffer(W * H * 4)
+        cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+        hipmem.write_plane(cplane, counts[None])
+    bs.accumulate(chunk.desc(counts=cplane))
+    p = sa.default_params(radius=r)
+    feats_list = None
+    if not weighted:
+        p.ds_factor = 0.0
+    bs.denoise_vars(p, feats.gbuffers() if weighted else None)
+    ctx.synchronize()
+    n, st = read_stats(bs, lambdas)
+    lnS2, se2, m2C = (read_set_buf(bs, lambdas, k) for k in ("lnS2", "se2", "m2C"))
+    if weighted:
+        f = feats.to_numpy()
+        feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    vD, vCD, amb = oracle.denoise_vars(n, st[:, 1], lnS2, se2, m2C, r, p.ds_factor, p.var_z, feats_list,
+                                       want_ambiguous=True)
+    assert amb.sum() / (float(amb.size) * (2 * r + 1) ** 2) <= DECISION_MISMATCH_MAX
+    gD, gCD = read_set_buf(bs, lambdas, "varD"), read_set_buf(bs, lambdas, "varCD")
+    if ragged:  # pixels with fewer than 4 samples: their own s^2 and m2C/(n-1), unfiltered
+        few = np.broadcast_to(n < 4, gD.shape)
+        own = np.broadcast_to(n >= 2, gD.shape) & few
+        assert np.count_nonzero(own) > 0
+        assert np.array_equal(gD[few & ~own], np.zeros(np.count_nonzero(few & ~own), np.float32))
+    for g, v in ((gD, vD), (gCD, vCD)):
+        ok = np.isfinite(v) & (np.abs(v) < 1e37) & (amb == 0)
+        assert np.max(np.abs(g[ok] - v[ok]) / np.maximum(np.abs(v[ok]), 1e-30)) < FILTER_RTOL
+
+
 @pytest.mark.parametrize("mt_mode,ragged", [(2, False), (3, False), (2, True), (3, True)])
 def test_welch_filter_mt5(ctx, mt_mode, ragged):
     W, H, r = 96, 64, 10
