@@ -32,7 +32,7 @@
 //    mean is K + mean(y - K); mean(y - K) itself persists in an internal
 //    plane so that a later call continues the recurrence bit for bit.  One
 //    log per sample-channel is shared by all sets; expm1 uses a range-reduced
-//    polynomial and the log the hardware log2.  The update is the reference recurrence in factored form,
+//    polynomial in base 2 and the log the hardware log2.  The update is the reference recurrence in factored form,
 //        m2 += d^2 (1 - 1/n)
 //        m3 += -3 (d/n) m2 + d^3 (1 - 1/n^2)
 //        m4 += -4 (d/n) m3 - 6 (d^2/n^2) m2 + d^4 (1 - 1/n^3)
@@ -82,25 +82,30 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float nf, float x
     st[6] += sqrtf(d * (d - dN));  // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
 }
 
-// expm1(z) with |relative error| of a few ulp: z = k ln2 + r, |r| <= ln2/2,
-// expm1(z) = 2^k expm1(r) + (2^k - 1), expm1(r) by its Taylor polynomial to
-// r^8/8! (truncation < 6e-9 relative).  z below -104 gives -1.
-__device__ __forceinline__ float expm1_fast(float z) {
-    z = fmaxf(z, -104.f);
-    const float k = __builtin_rintf(z * 1.44269504088896341f);
-    float r = __builtin_fmaf(k, -0.693145751953125f, z);   // ln2 split: hi has 12 trailing zero bits
-    r = __builtin_fmaf(k, -1.4286068203094173e-06f, r);
-    float p = 1.f / 40320.f;
-    p = __builtin_fmaf(p, r, 1.f / 5040.f);
-    p = __builtin_fmaf(p, r, 1.f / 720.f);
-    p = __builtin_fmaf(p, r, 1.f / 120.f);
-    p = __builtin_fmaf(p, r, 1.f / 24.f);
-    p = __builtin_fmaf(p, r, 1.f / 6.f);
-    p = __builtin_fmaf(p, r, 0.5f);
-    p = __builtin_fmaf(p * r, r, r);                       // expm1(r)
-    const float two_k = __builtin_ldexpf(1.f, (int)k);
-    return __builtin_fmaf(two_k, p, two_k - 1.f);
+// Wt (e^z - 1) for z = w ln 2, evaluated in base 2: w = k + f with k the
+// nearest integer (f = w - k is exact, |f| <= 1/2), 2^f - 1 = sum of
+// (f ln 2)^j / j! to j = 7 (truncation < 2e-8 relative), and
+//     Wt (2^w - 1) = (Wt 2^k)(2^f - 1) + (Wt 2^k - Wt),
+// so the scale Wt rides on the exponent and costs no multiply.  |error| is a
+// few ulp of the result at every w; w below -150 gives -Wt (Wt 2^k flushes to
+// zero), the limit x -> 0.
+__device__ __forceinline__ float wexpm1(float w, float Wt) {
+    w = fmaxf(w, -150.f);
+    const float k = __builtin_rintf(w);
+    const float f = w - k;
+    float p = 1.5252733804059841e-05f;             // ln2^7 / 7!
+    p = __builtin_fmaf(p, f, 1.5403530393381608e-04f);  // ln2^6 / 6!
+    p = __builtin_fmaf(p, f, 1.3333558146428443e-03f);  // ln2^5 / 5!
+    p = __builtin_fmaf(p, f, 9.6181291076284772e-03f);  // ln2^4 / 4!
+    p = __builtin_fmaf(p, f, 5.5504108664821580e-02f);  // ln2^3 / 3!
+    p = __builtin_fmaf(p, f, 2.4022650695910071e-01f);  // ln2^2 / 2!
+    p = __builtin_fmaf(p, f, 6.9314718055994531e-01f);  // ln2
+    p *= f;                                        // 2^f - 1
+    const float wk = __builtin_ldexpf(Wt, (int)k);
+    return __builtin_fmaf(wk, p, wk - Wt);
 }
+
+constexpr float kLog2eAcc = 1.4426950408889634f;
 
 // Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
 struct NFac {
@@ -224,7 +229,7 @@ __device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
             // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
             const float dl = __builtin_amdgcn_logf(v) * 0.69314718055994531f - L;
 #pragma unroll
-            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], Wt[s] * expm1_fast(a.lam[s] * dl), f, xD, yD);
+            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[s]), f, xD, yD);
         }
     }
     // The statistic addresses are rebuilt after the loop rather than held in
@@ -326,7 +331,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         const float dl = T > 0 ? __builtin_amdgcn_logf(v) * 0.69314718055994531f - L : 0.f;
 #pragma unroll
         for (int s = 0; s <= T; ++s) {
-            const float t = s == 0 ? v : Wt[s] * expm1_fast(a.lam[s] * dl);
+            const float t = s == 0 ? v : wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[s]);
             meng_factored<NP>(pm[s], t, fr, 0.f, 0.f);
             if (MM >= 5) {
                 const float dg = t - mg[s];
