@@ -273,11 +273,11 @@ __device__ __forceinline__ void var_col_load(const float4 *q, VarCol<G, HASW> &c
 }
 
 template <bool HASW>
-__device__ __forceinline__ float var_weight(const VarArgs &a, const float (&gp)[kNF], float okq,
-                                            const float (&fq)[kNF], float lw0, float m, float dx2) {
+__device__ __forceinline__ float var_weight(float sp_scale, const float (&gp)[kNF], float okq, const float (&fq)[kNF],
+                                            float lw0, float m, float dx2) {
     float w = okq;
     if (HASW) {
-        float lw = __builtin_fmaf(a.sp_scale, dx2, lw0);
+        float lw = __builtin_fmaf(sp_scale, dx2, lw0);
 #pragma unroll
         for (int f = 0; f < kNF; ++f) {
             const float d = gp[f] - fq[f];
@@ -312,8 +312,8 @@ __device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, H
     const float fq[kNF] = {c.s0.y, c.s0.z, c.s0.w, c.s1.x, c.s1.y, c.s1.z, c.s1.w};
     const float dx2 = (float)(dx * dx);
     const bool zA = offA || (selfA && dx == 0), zB = offB || (selfB && dx == 0);
-    const float wA = var_weight<HASW>(a, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : 1.f, dx2);
-    const float wB = var_weight<HASW>(a, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : 1.f, dx2);
+    const float wA = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : 1.f, dx2);
+    const float wB = var_weight<HASW>(a.sp_scale, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : 1.f, dx2);
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
@@ -386,26 +386,32 @@ __device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool 
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Loader wavefront: row yq of the tile's halo (columns x0 - r .. x0 + 63 + r)
-// into one ring slot with the same number of 64-item direct loads for every
-// row, so that a count of outstanding loads names a number of rows.  Items
-// of columns and rows outside the image, and every item past the halo, read
-// the zero record.
-template <int G, bool HASW, int HWC>
-__device__ __forceinline__ void var_load_row(const VarArgs &a, float4 *dst, int yq, int x0, int lane) {
-    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
-    const int nch = (TX + 2 * a.r) * F4;
-    const bool row_in = yq >= 0 && yq < a.H;
-    const float4 *row = a.rec + (size_t)(row_in ? yq : 0) * a.ld * F4;
+// Loader wavefront: row yq of a tile's halo (columns x0 - r .. x0 + 63 + r)
+// of F4-item records into one ring slot, with IPR direct 64-item loads for
+// every row, so that a count of outstanding loads names a number of rows.
+// Items of columns and rows outside the image, and every item past the halo,
+// read the zero record.
+template <int F4, int IPR>
+__device__ __forceinline__ void ring_load_row(const float4 *rec, const float4 *zero, int W, int H, int ld, int r,
+                                              float4 *dst, int yq, int x0, int lane) {
+    const int nch = (TX + 2 * r) * F4;
+    const bool row_in = yq >= 0 && yq < H;
+    const float4 *row = rec + (size_t)(row_in ? yq : 0) * ld * F4;
 #pragma unroll
-    for (int i = 0; i < var_loads_per_row<G, HASW, HWC>(); ++i) {
+    for (int i = 0; i < IPR; ++i) {
         const int ch = i * 64 + lane;
         const int col = ch / F4, item = ch - col * F4;
-        const int xq = x0 - a.r + col;
-        const bool real = ch < nch && row_in && xq >= 0 && xq < a.W;
-        const float4 *src = real ? row + (size_t)xq * F4 + item : a.zero + item;
+        const int xq = x0 - r + col;
+        const bool real = ch < nch && row_in && xq >= 0 && xq < W;
+        const float4 *src = real ? row + (size_t)xq * F4 + item : zero + item;
         __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(dst + i * 64), 16, 0, 0);
     }
+}
+
+template <int G, bool HASW, int HWC>
+__device__ __forceinline__ void var_load_row(const VarArgs &a, float4 *dst, int yq, int x0, int lane) {
+    ring_load_row<VarRec<G, HASW>::FLOAT4S, var_loads_per_row<G, HASW, HWC>()>(a.rec, a.zero, a.W, a.H, a.ld, a.r,
+                                                                                dst, yq, x0, lane);
 }
 
 // One tile of 64 x 2 TY pixels per workgroup: TY filtering wavefronts (two
@@ -1162,6 +1168,310 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
     accum_tile<HASW, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
+// ------------------------------------------------------------------------
+// Pass 2 from records (the path without a decision dump): film (3), features
+// (7, weighted filter), then pass 1's decision words grouped by word index,
+// (c0, c1, c2, 0) per group, so that one 16-byte read fetches a neighbour's
+// three channel words; staged by a loader wavefront as in the variance
+// filter.  Each lane weights two vertically adjacent pixels A and B.  A
+// forward offset's decision comes from the pixel's own words (in registers),
+// a backward offset's from the neighbour's record, at a word index that is
+// the same for the whole wavefront.
+template <bool HASW>
+struct AccRec {
+    static constexpr int HEAD4 = HASW ? 3 : 1;                  // float4 items of film (+ features)
+    static constexpr int RAW4 = HEAD4 + kBitsWords;             // + one item per word group
+    static constexpr int FLOAT4S = RAW4 % 2 ? RAW4 : RAW4 + 1;  // 11 or 9: odd
+    static_assert(FLOAT4S * 4 <= kVarRecFloats, "record larger than its planes");
+};
+template <bool HASW, int HWC>
+constexpr int acc_loads_per_row() { return ((HWC + 2) * AccRec<HASW>::FLOAT4S + 63) / 64; }
+template <bool HASW, int HWC, int TY>
+constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<HASW, HWC>() * 64 * 16; }
+
+template <bool HASW>
+__device__ __forceinline__ float acc_rec_float(const WelchArgs &a, int x, int y, int i) {
+    using R = AccRec<HASW>;
+    if (i < 3)
+        return *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + (int64_t)i * a.film_cs);
+    if (i < 4 * R::HEAD4) return HASW && i - 3 < a.f.nf ? feat_at(a.f, i - 3, x, y) : 0.f;
+    const int g = i / 4 - R::HEAD4, c = i % 4;  // word group, channel
+    if (g < kBitsWords && c < 3)
+        return __uint_as_float(a.bits[((size_t)c * kBitsWords + g) * a.plane + (size_t)y * a.ld + x]);
+    return 0.f;
+}
+
+template <bool HASW>
+__global__ void __launch_bounds__(256) k_accum_pack(WelchArgs a, float4 *zero) {
+    using R = AccRec<HASW>;
+    const int y = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j >= a.W * R::FLOAT4S) return;
+    const int x = j / R::FLOAT4S, i = 4 * (j - x * R::FLOAT4S);
+    a.rec[(size_t)y * a.ld * R::FLOAT4S + j] =
+        make_float4(acc_rec_float<HASW>(a, x, y, i), acc_rec_float<HASW>(a, x, y, i + 1),
+                    acc_rec_float<HASW>(a, x, y, i + 2), acc_rec_float<HASW>(a, x, y, i + 3));
+}
+
+template <bool HASW>
+struct AccPix {
+    float gp[kNF], den[3], num[3];
+    uint32_t own[3][kBitsWords];
+};
+
+template <bool HASW>
+__device__ __forceinline__ void acc_pix_init(const WelchArgs &a, int x, int y, bool active, AccPix<HASW> &P) {
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) P.gp[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        P.den[c] = 1.f;  // the pixel itself, weight 1
+        P.num[c] = 0.f;
+#pragma unroll
+        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = 0u;
+    }
+    if (!active) return;
+    const size_t pix = (size_t)y * a.ld + x;
+    if (HASW)
+#pragma unroll
+        for (int k = 0; k < kNF; ++k) P.gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        P.num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+#pragma unroll
+        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = a.bits[((size_t)c * kBitsWords + w) * a.plane + pix];
+    }
+}
+
+// How a pixel finds the decisions of one window row: all forward offsets
+// (dy > 0: its own words, turned into a per-row mask), all backward offsets
+// (dy < 0: the neighbour's word, at an index that falls by one per column),
+// the centre row (dy = 0, both kinds, per column), or outside its window.
+enum AccMode { kAccFwd, kAccBwd, kAccGen, kAccOff };
+
+template <bool HASW>
+struct AccCol {
+    float4 f0, f1, f2;  // film 0..2, features 0..6 (features only with HASW)
+    uint4 wa, wb;       // the neighbour's word group for A and for B
+    int sha, shb;       // bit of the neighbour's word (backward offsets)
+};
+
+// Row-constant part of a pixel's step.  The own words are consumed front
+// first: rows run in order of increasing forward index, so the pixel's words
+// are shifted down (base counts the shifts) until the row's first word is at
+// index 0; o0, o1 are then the row's two own words (the row's 2r + 1
+// consecutive forward indices span at most two).
+struct AccRowP {
+    int dy, k0, i0;  // i0: forward index of column dx = -r (own for dy > 0, the neighbour's for dy < 0)
+    float lw;
+    uint32_t o0[3], o1[3], m[3];  // m: dy > 0, bit dx + r = decision of column dx
+};
+
+template <bool HASW>
+__device__ __forceinline__ AccRowP acc_row_p(const WelchArgs &a, AccPix<HASW> &P, int &base, int dy, bool off) {
+    const int r = a.r;
+    AccRowP R;
+    R.dy = dy;
+    R.lw = a.sp_scale * (float)(dy * dy);
+    R.i0 = dy > 0 ? fwd_index(-r, dy, r) : (dy < 0 ? fwd_index(r, -dy, r) : 0);
+    R.k0 = dy > 0 && !off ? R.i0 >> 5 : (dy == 0 ? 0 : base);
+    for (; base < R.k0; ++base)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+            for (int j = 0; j + 1 < kBitsWords; ++j) P.own[ch][j] = P.own[ch][j + 1];
+            P.own[ch][kBitsWords - 1] = 0u;
+        }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        R.o0[ch] = P.own[ch][0];
+        R.o1[ch] = P.own[ch][1];
+        R.m[ch] = __builtin_amdgcn_alignbit(R.o1[ch], R.o0[ch], (uint32_t)(R.i0 & 31));
+    }
+    return R;
+}
+
+// Word group and bit of the neighbour's decision for pixel mode M at column dx.
+template <int M>
+__device__ __forceinline__ void acc_qsel(const AccRowP &R, int dx, int r, int &wq, int &sh) {
+    if (M == kAccBwd) {
+        const int idx = R.i0 - (dx + r);
+        wq = max(idx >> 5, 0);  // the prefetch past the padding column may go below index 0
+        sh = idx & 31;
+    } else if (M == kAccGen) {
+        const int idx = dx < 0 ? -dx - 1 : 0;  // centre row: backward offsets dx < 0
+        wq = idx >> 5;
+        sh = idx & 31;
+    } else {
+        wq = 0;
+        sh = 0;
+    }
+}
+
+template <bool HASW, int MA, int MB>
+__device__ __forceinline__ void acc_col_load(const float4 *q, const AccRowP &RA, const AccRowP &RB, int dx, int r,
+                                             AccCol<HASW> &c) {
+    c.f0 = q[0];
+    if (HASW) {
+        c.f1 = q[1];
+        c.f2 = q[2];
+    }
+    const uint4 *g = (const uint4 *)(q + AccRec<HASW>::HEAD4);
+    int wA, wB;
+    acc_qsel<MA>(RA, dx, r, wA, c.sha);
+    acc_qsel<MB>(RB, dx, r, wB, c.shb);
+    if (MA == kAccBwd || MA == kAccGen) c.wa = g[wA];
+    if (MB == kAccBwd || MB == kAccGen) c.wb = g[wB];
+}
+
+// The weight and the three channel decisions of column dx (j = dx + r) for
+// one pixel in mode M; the padding column and the pixel itself weigh 0.
+template <bool HASW, int M>
+__device__ __forceinline__ void acc_pix_col(const WelchArgs &a, const AccCol<HASW> &c, const float (&fq)[kNF],
+                                            const float (&film)[3], const uint4 &qw, int sh, int dx, const AccRowP &R,
+                                            AccPix<HASW> &P) {
+    if (M == kAccOff) return;
+    const bool z = dx > a.r || (M == kAccGen && dx == 0);
+    const float w = var_weight<HASW>(a.sp_scale, P.gp, 1.f, fq, z ? -INFINITY : R.lw, z ? 0.f : 1.f, (float)(dx * dx));
+    const uint32_t q3[3] = {qw.x, qw.y, qw.z};
+    const int j = dx + a.r;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        uint32_t bit;
+        if (M == kAccFwd) {
+            bit = (R.m[ch] >> j) & 1u;
+        } else if (M == kAccBwd) {
+            bit = (q3[ch] >> sh) & 1u;
+        } else {  // centre row: dx > 0 own index dx - 1 (word 0), dx < 0 the neighbour's
+            bit = dx > 0 ? (R.o0[ch] >> (dx - 1)) & 1u : (q3[ch] >> sh) & 1u;
+        }
+        const float wa = bit ? w : 0.f;
+        P.den[ch] += wa;
+        P.num[ch] = __builtin_fmaf(wa, film[ch], P.num[ch]);
+    }
+}
+
+template <bool HASW, int MA, int MB>
+__device__ __forceinline__ void acc_col_test(const WelchArgs &a, const AccCol<HASW> &c, int dx, const AccRowP &RA,
+                                             const AccRowP &RB, AccPix<HASW> &A, AccPix<HASW> &B) {
+    const float fq[kNF] = {c.f0.w, c.f1.x, c.f1.y, c.f1.z, c.f1.w, c.f2.x, c.f2.y};
+    const float film[3] = {c.f0.x, c.f0.y, c.f0.z};
+    acc_pix_col<HASW, MA>(a, c, fq, film, c.wa, c.sha, dx, RA, A);
+    acc_pix_col<HASW, MB>(a, c, fq, film, c.wb, c.shb, dx, RB, B);
+}
+
+// One window row for A and B in modes MA, MB: 2r + 2 columns (the last one
+// the zeroed padding column, weight 0), the reads of column dx + 1 issued
+// before the weights of column dx.
+template <bool HASW, int MA, int MB>
+__device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, const AccRowP &RA, const AccRowP &RB,
+                                         AccPix<HASW> &A, AccPix<HASW> &B) {
+    constexpr int F4 = AccRec<HASW>::FLOAT4S;
+    const int r = a.r;
+    AccCol<HASW> X, Y;
+    acc_col_load<HASW, MA, MB>(q, RA, RB, -r, r, X);
+    for (int dx = -r; dx <= r; dx += 2, q += 2 * F4) {
+        acc_col_load<HASW, MA, MB>(q + F4, RA, RB, dx + 1, r, Y);
+        __builtin_amdgcn_sched_barrier(0);
+        acc_col_test<HASW, MA, MB>(a, X, dx, RA, RB, A, B);
+        acc_col_load<HASW, MA, MB>(q + 2 * F4, RA, RB, dx + 2, r, X);
+        __builtin_amdgcn_sched_barrier(0);
+        acc_col_test<HASW, MA, MB>(a, Y, dx + 1, RA, RB, A, B);
+    }
+}
+
+// Step k: the staged row at dy = k - r from A and k - r - 1 from B.
+template <bool HASW>
+__device__ __forceinline__ void acc_row(const WelchArgs &a, const float4 *slot, int tx, int k, AccPix<HASW> &A,
+                                        AccPix<HASW> &B, int &baseA, int &baseB) {
+    const int r = a.r;
+    const AccRowP RA = acc_row_p<HASW>(a, A, baseA, k - r, k > 2 * r);
+    const AccRowP RB = acc_row_p<HASW>(a, B, baseB, k - r - 1, k < 1);
+    const float4 *q = slot + tx * AccRec<HASW>::FLOAT4S;  // column of dx = -r
+    if (k == 0)
+        acc_cols<HASW, kAccBwd, kAccOff>(a, q, RA, RB, A, B);
+    else if (k < r)
+        acc_cols<HASW, kAccBwd, kAccBwd>(a, q, RA, RB, A, B);
+    else if (k == r)
+        acc_cols<HASW, kAccGen, kAccBwd>(a, q, RA, RB, A, B);
+    else if (k == r + 1)
+        acc_cols<HASW, kAccFwd, kAccGen>(a, q, RA, RB, A, B);
+    else if (k <= 2 * r)
+        acc_cols<HASW, kAccFwd, kAccFwd>(a, q, RA, RB, A, B);
+    else
+        acc_cols<HASW, kAccOff, kAccFwd>(a, q, RA, RB, A, B);
+}
+
+template <bool HASW, int HWC, int TY>
+__global__ void __launch_bounds__(64 * (TY + 1))
+    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(acc_lds_bytes<HASW, HWC, TY>(), 64 * (TY + 1)))))
+    k_welch_accum2(WelchArgs a) {
+    extern __shared__ float4 aring[];
+    constexpr int F4 = AccRec<HASW>::FLOAT4S, IPR = acc_loads_per_row<HASW, HWC>();
+    constexpr int P = 2 * TY - 1, NS = var_ring_slots<TY>(), SLOT = IPR * 64;
+    constexpr int AHEAD_LOADS = IPR * (kVarAhead - 1);
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + 2 * TY - 1) / (2 * TY)));
+    if (t < 0) return;
+    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * 2 * TY;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int last = 2 * a.r + 1;
+
+    if (wave == TY) {  // loader, as in the variance filter
+        for (int j = 0; j < P + kVarAhead - 1; ++j)
+            ring_load_row<F4, IPR>(a.rec, a.zero, a.W, a.H, a.ld, a.r, aring + j * SLOT, y0 - a.r + j, x0, lane);
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));
+        __builtin_amdgcn_s_barrier();
+        for (int k = 0; k <= last; ++k) {
+            const int j = k + P + kVarAhead - 1;
+            ring_load_row<F4, IPR>(a.rec, a.zero, a.W, a.H, a.ld, a.r, aring + (j % NS) * SLOT, y0 - a.r + j, x0,
+                                   lane);
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));
+            __builtin_amdgcn_s_barrier();
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        return;
+    }
+
+    const int x = x0 + lane, yA = y0 + 2 * wave;
+    const bool actA = x < a.W && yA < a.H, actB = x < a.W && yA + 1 < a.H;
+    AccPix<HASW> A, B;
+    acc_pix_init<HASW>(a, x, yA, actA, A);
+    acc_pix_init<HASW>(a, x, yA + 1, actB, B);
+    int baseA = 0, baseB = 0;  // own words shifted out so far
+    __syncthreads();
+    for (int k = 0; k <= last; ++k) {
+        const float4 *slot = aring + ((2 * wave + k) % NS) * SLOT;
+        if (actA) acc_row<HASW>(a, slot, lane, k, A, B, baseA, baseB);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (actA)
+            *(float *)((char *)a.out + (int64_t)yA * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = A.num[c] / A.den[c];
+        if (actB)
+            *(float *)((char *)a.out + (int64_t)(yA + 1) * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) =
+                B.num[c] / B.den[c];
+    }
+}
+
+template <bool HASW>
+static int accum2_launch(const WelchArgs &a, hipStream_t st) {
+    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = 4;
+    constexpr int F4 = AccRec<HASW>::FLOAT4S;
+    hipLaunchKernelGGL((k_accum_pack<HASW>), dim3((a.W * F4 + 255) / 256, a.H), dim3(256), 0, st, a,
+                       const_cast<float4 *>(a.zero));
+    STATER_HIP(hipGetLastError());
+    const size_t lds = (size_t)acc_lds_bytes<HASW, HWC, TY>();
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum2<HASW, HWC, TY>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + 2 * TY - 1) / (2 * TY))));
+    hipLaunchKernelGGL((k_welch_accum2<HASW, HWC, TY>), grid, dim3(64 * (TY + 1)), lds, st, a);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
 template <int S>
 static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
     constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = kTY;
@@ -1171,6 +1481,7 @@ static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
     STATER_HIP(hipGetLastError());
+    if (!a.dec && a.rec) return hasw ? accum2_launch<true>(a, st) : accum2_launch<false>(a, st);
     if (hasw) {
         lds = (size_t)TY * AccumLayout<true>::FLOATS * HWC * 4;
         STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<true, HWC, TY>,
@@ -1279,6 +1590,8 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     }
     a.dec = dec;
     a.bits = bs->welch_bits;
+    a.rec = (float4 *)bs->var_rec;
+    a.zero = (const float4 *)bs->var_zero;
     const int S = plain ? 1 : bs->S;
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");

@@ -71,6 +71,8 @@ struct WelchArgs {
     float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
     uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
     uint32_t *bits;        // forward half-window decisions [3][kBitsWords][plane] (symmetric form)
+    float4 *rec;           // pass-2 records (the buffer set's variance-filter record planes)
+    const float4 *zero;    // one all-zero record
     FeatArgs f;
     int W, H, ld, r, mode3;
     size_t plane, set_stride;
