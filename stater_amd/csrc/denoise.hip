@@ -175,46 +175,62 @@ constexpr int waitcnt_vm(int v) { return (v & 15) | ((v >> 4) << 14) | (7 << 4) 
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
-// Item `item` of the record of pixel (x, y), inside the image.
+// Pack pass.  A workgroup packs 64 pixels of a row: it first stages their
+// inputs plane by plane (coalesced 256-byte rows; one wavefront per plane),
+// then writes the records item by item (consecutive threads, consecutive
+// 16-byte items).  Staged planes: n, the 7 features, then (lnS2, se2, m2,
+// m2C) per set-channel of the group.
+template <int G>
+constexpr int var_pack_planes() { return 8 + 12 * G; }
+
 template <int G, bool HASW>
-__device__ __forceinline__ float4 var_rec_item(const VarArgs &a, int x, int y, int item) {
-    using R = VarRec<G, HASW>;
+__device__ __forceinline__ float var_pack_input(const VarArgs &a, int pl, int x, int y) {
     const size_t pix = (size_t)y * a.ld + x;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (item < R::SINGLE4) {
-        auto f = [&](int k) { return k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f; };
-        if (item == 0) {
-            v.x = a.n[pix] >= 4 ? 1.f : 0.f;
-            if (HASW) { v.y = f(0); v.z = f(1); v.w = f(2); }
-        } else {
-            v = make_float4(f(3), f(4), f(5), f(6));
-        }
-    } else if (item < R::ITEMS) {
-        const int sc = item - R::SINGLE4;  // g*3 + c within the group
-        const int g = sc / 3, c = sc - g * 3;
-        const int n = a.n[pix];
-        const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-        if (n >= 4) { v.x = a.lnS2[o]; v.y = a.z2 * a.se2[o]; }
-        if (n >= 2) {
-            const float inv = 1.f / (float)(n - 1);
-            v.z = finite_or_max(a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv);
-            v.w = finite_or_max(a.m2C[o] * inv);
-        }
-    }
-    return v;
+    if (pl == 0) return __int_as_float(a.n[pix]);
+    if (pl < 8) return HASW && pl - 1 < a.f.nf ? feat_at(a.f, pl - 1, x, y) : 0.f;
+    const int sc = (pl - 8) >> 2, kind = (pl - 8) & 3;  // set-channel g*3 + c, statistic
+    const int g = sc / 3, c = sc - g * 3;
+    const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+    if (kind == 0) return a.lnS2[o];
+    if (kind == 1) return a.se2[o];
+    if (kind == 2) return a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix];
+    return a.m2C[o];
 }
 
-// Pack pass: one thread per 16-byte record item, rows of records at pitch
-// ld (consecutive threads write consecutive items).
 template <int G, bool HASW>
 __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
     using R = VarRec<G, HASW>;
-    const int y = blockIdx.y;
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    constexpr int NPL = var_pack_planes<G>();
+    __shared__ float in[NPL][64];
+    const int y = blockIdx.y, x0 = blockIdx.x * 64;
     if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (j >= a.W * R::FLOAT4S) return;
-    const int x = j / R::FLOAT4S, item = j - x * R::FLOAT4S;
-    a.rec[(size_t)y * a.ld * R::FLOAT4S + j] = var_rec_item<G, HASW>(a, x, y, item);
+    for (int i = threadIdx.x; i < NPL * 64; i += 256) {
+        const int pl = i >> 6, px = i & 63;
+        in[pl][px] = x0 + px < a.W ? var_pack_input<G, HASW>(a, pl, x0 + px, y) : 0.f;
+    }
+    __syncthreads();
+    const int npx = min(64, a.W - x0);
+    float4 *rec = a.rec + ((size_t)y * a.ld + x0) * R::FLOAT4S;
+    for (int j = threadIdx.x; j < npx * R::FLOAT4S; j += 256) {
+        const int px = j / R::FLOAT4S, item = j - px * R::FLOAT4S;
+        const int n = __float_as_int(in[0][px]);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (item == 0) {
+            v.x = n >= 4 ? 1.f : 0.f;
+            if (HASW) { v.y = in[1][px]; v.z = in[2][px]; v.w = in[3][px]; }
+        } else if (HASW && item == 1) {
+            v = make_float4(in[4][px], in[5][px], in[6][px], in[7][px]);
+        } else if (item < R::ITEMS) {
+            const int b = 8 + 4 * (item - R::SINGLE4);
+            if (n >= 4) { v.x = in[b][px]; v.y = a.z2 * in[b + 1][px]; }
+            if (n >= 2) {
+                const float inv = 1.f / (float)(n - 1);
+                v.z = finite_or_max(in[b + 2][px] * inv);
+                v.w = finite_or_max(in[b + 3][px] * inv);
+            }
+        }
+        rec[j] = v;
+    }
 }
 
 // One output pixel's state: its own statistics and the running sums.
@@ -473,8 +489,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 
 template <int G, bool HASW, int HWC, int TY>
 static int var_launch4(const VarArgs &a, hipStream_t st) {
-    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
-    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W * F4 + 255) / 256, a.H), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
                        const_cast<float4 *>(a.zero));
     STATER_HIP(hipGetLastError());
     const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY>();
@@ -1189,29 +1204,47 @@ constexpr int acc_loads_per_row() { return ((HWC + 2) * AccRec<HASW>::FLOAT4S + 
 template <bool HASW, int HWC, int TY>
 constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<HASW, HWC>() * 64 * 16; }
 
+// Pack pass, staged as for the variance filter.  Staged planes: film (3),
+// features (7), the 21 decision words (channel-major).
+constexpr int kAccPackPlanes = 3 + kNF + 3 * kBitsWords;
+
 template <bool HASW>
-__device__ __forceinline__ float acc_rec_float(const WelchArgs &a, int x, int y, int i) {
-    using R = AccRec<HASW>;
-    if (i < 3)
-        return *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + (int64_t)i * a.film_cs);
-    if (i < 4 * R::HEAD4) return HASW && i - 3 < a.f.nf ? feat_at(a.f, i - 3, x, y) : 0.f;
-    const int g = i / 4 - R::HEAD4, c = i % 4;  // word group, channel
-    if (g < kBitsWords && c < 3)
-        return __uint_as_float(a.bits[((size_t)c * kBitsWords + g) * a.plane + (size_t)y * a.ld + x]);
-    return 0.f;
+__device__ __forceinline__ float acc_pack_input(const WelchArgs &a, int pl, int x, int y) {
+    if (pl < 3)
+        return *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + (int64_t)pl * a.film_cs);
+    if (pl < 3 + kNF) return HASW && pl - 3 < a.f.nf ? feat_at(a.f, pl - 3, x, y) : 0.f;
+    return __uint_as_float(a.bits[(size_t)(pl - 3 - kNF) * a.plane + (size_t)y * a.ld + x]);
 }
 
 template <bool HASW>
 __global__ void __launch_bounds__(256) k_accum_pack(WelchArgs a, float4 *zero) {
     using R = AccRec<HASW>;
-    const int y = blockIdx.y;
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    __shared__ float in[kAccPackPlanes][64];
+    const int y = blockIdx.y, x0 = blockIdx.x * 64;
     if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (j >= a.W * R::FLOAT4S) return;
-    const int x = j / R::FLOAT4S, i = 4 * (j - x * R::FLOAT4S);
-    a.rec[(size_t)y * a.ld * R::FLOAT4S + j] =
-        make_float4(acc_rec_float<HASW>(a, x, y, i), acc_rec_float<HASW>(a, x, y, i + 1),
-                    acc_rec_float<HASW>(a, x, y, i + 2), acc_rec_float<HASW>(a, x, y, i + 3));
+    for (int i = threadIdx.x; i < kAccPackPlanes * 64; i += 256) {
+        const int pl = i >> 6, px = i & 63;
+        in[pl][px] = x0 + px < a.W ? acc_pack_input<HASW>(a, pl, x0 + px, y) : 0.f;
+    }
+    __syncthreads();
+    const int npx = min(64, a.W - x0);
+    float4 *rec = a.rec + ((size_t)y * a.ld + x0) * R::FLOAT4S;
+    for (int j = threadIdx.x; j < npx * R::FLOAT4S; j += 256) {
+        const int px = j / R::FLOAT4S, item = j - px * R::FLOAT4S;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (item == 0) {
+            v = make_float4(in[0][px], in[1][px], in[2][px], HASW ? in[3][px] : 0.f);
+        } else if (HASW && item < 3) {
+            const int b = 4 * item;  // features 1..4, then 5, 6
+            v = make_float4(in[b][px], in[b + 1][px], item == 1 ? in[b + 2][px] : 0.f,
+                            item == 1 ? in[b + 3][px] : 0.f);
+        } else if (item < R::HEAD4 + kBitsWords) {
+            const int g = item - R::HEAD4;  // word group: (c0, c1, c2, 0)
+            v = make_float4(in[3 + kNF + g][px], in[3 + kNF + kBitsWords + g][px],
+                            in[3 + kNF + 2 * kBitsWords + g][px], 0.f);
+        }
+        rec[j] = v;
+    }
 }
 
 template <bool HASW>
@@ -1459,8 +1492,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 template <bool HASW>
 static int accum2_launch(const WelchArgs &a, hipStream_t st) {
     constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = 4;
-    constexpr int F4 = AccRec<HASW>::FLOAT4S;
-    hipLaunchKernelGGL((k_accum_pack<HASW>), dim3((a.W * F4 + 255) / 256, a.H), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((k_accum_pack<HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
                        const_cast<float4 *>(a.zero));
     STATER_HIP(hipGetLastError());
     const size_t lds = (size_t)acc_lds_bytes<HASW, HWC, TY>();
