@@ -83,8 +83,9 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float nf, float x
 }
 
 // Wt (e^z - 1) for z = w ln 2, evaluated in base 2: w = k + f with k the
-// nearest integer (f = w - k is exact, |f| <= 1/2), 2^f - 1 = sum of
-// (f ln 2)^j / j! to j = 7 (truncation < 2e-8 relative), and
+// nearest integer (f = w - k is exact, |f| <= 1/2), 2^f - 1 = f p(f) with p a
+// degree-5 least-relative-error fit to (2^f - 1)/f on [-1/2, 1/2]
+// (|relative error| 1.1e-8 in exact arithmetic), and
 //     Wt (2^w - 1) = (Wt 2^k)(2^f - 1) + (Wt 2^k - Wt),
 // so the scale Wt rides on the exponent and costs no multiply.  |error| is a
 // few ulp of the result at every w; w below -150 gives -Wt (Wt 2^k flushes to
@@ -93,14 +94,13 @@ __device__ __forceinline__ float wexpm1(float w, float Wt) {
     w = fmaxf(w, -150.f);
     const float k = __builtin_rintf(w);
     const float f = w - k;
-    float p = 1.5252733804059841e-05f;             // ln2^7 / 7!
-    p = __builtin_fmaf(p, f, 1.5403530393381608e-04f);  // ln2^6 / 6!
-    p = __builtin_fmaf(p, f, 1.3333558146428443e-03f);  // ln2^5 / 5!
-    p = __builtin_fmaf(p, f, 9.6181291076284772e-03f);  // ln2^4 / 4!
-    p = __builtin_fmaf(p, f, 5.5504108664821580e-02f);  // ln2^3 / 3!
-    p = __builtin_fmaf(p, f, 2.4022650695910071e-01f);  // ln2^2 / 2!
-    p = __builtin_fmaf(p, f, 6.9314718055994531e-01f);  // ln2
-    p *= f;                                        // 2^f - 1
+    float p = 1.540351563395118e-04f;
+    p = __builtin_fmaf(p, f, 1.3390735495609548e-03f);
+    p = __builtin_fmaf(p, f, 9.618237484810723e-03f);
+    p = __builtin_fmaf(p, f, 5.550357433591522e-02f);
+    p = __builtin_fmaf(p, f, 2.4022649795021536e-01f);
+    p = __builtin_fmaf(p, f, 6.931471879267055e-01f);
+    p *= f;  // 2^f - 1
     const float wk = __builtin_ldexpf(Wt, (int)k);
     return __builtin_fmaf(wk, p, wk - Wt);
 }
@@ -109,7 +109,7 @@ constexpr float kLog2eAcc = 1.4426950408889634f;
 
 // Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
 struct NFac {
-    float b, a1, a2, a3, b4, b6, sa1;
+    float b, a1, a2, a3, b3, b4, b6, sa1;
 };
 
 __device__ __forceinline__ NFac nfac(float nf) {
@@ -119,6 +119,7 @@ __device__ __forceinline__ NFac nfac(float nf) {
     f.a1 = 1.f - f.b;
     f.a2 = 1.f - b2;
     f.a3 = 1.f - b2 * f.b;
+    f.b3 = -3.f * f.b;
     f.b4 = -4.f * f.b;
     f.b6 = -6.f * b2;
     f.sa1 = __builtin_amdgcn_sqrtf(f.a1);
@@ -126,7 +127,9 @@ __device__ __forceinline__ NFac nfac(float nf) {
 }
 
 // The reference recurrence in factored form on a shifted value t = y - K
-// (st[0] holds the shifted mean).
+// (st[0] holds the shifted mean), grouped by powers of d:
+//     m3 += d (d^2 a2 - 3 b m2),   m4 += d (-4 b m3 + d (d^2 a3 - 6 b^2 m2))
+// with the updated m2 and m3, as in the reference.
 template <int MM>
 __device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f, float xD, float yD) {
     float &mean = st[0];
@@ -137,12 +140,12 @@ __device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f,
     float &m2 = st[1];
     m2 = __builtin_fmaf(d2, f.a1, m2);
     if (MM == 2) return;
-    const float db = d * f.b;
     float &m3 = st[2];
-    m3 = __builtin_fmaf(-3.f * db, m2, __builtin_fmaf(d * d2, f.a2, m3));
+    m3 = __builtin_fmaf(d, __builtin_fmaf(f.b3, m2, d2 * f.a2), m3);
     if (MM == 3) return;
     float &m4 = st[3];
-    m4 = __builtin_fmaf(d * f.b4, m3, __builtin_fmaf(d2 * f.b6, m2, __builtin_fmaf(d2 * d2, f.a3, m4)));
+    const float u = __builtin_fmaf(d2, f.a3, f.b6 * m2);
+    m4 = __builtin_fmaf(d, __builtin_fmaf(d, u, f.b4 * m3), m4);
     if (MM == 4) return;
     const float r = t - mean;
     st[4] = __builtin_fmaf(r, xD, st[4]);
