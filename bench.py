@@ -53,7 +53,8 @@ def kernel_work(W, H, spp, S, r):
     T = S - 1
     acc_ops = P * spp * (S * 3 * ACC_OPS_PER_SET_CHANNEL + 4 * (3 + 3 * T + 3 * S))
     return {
-        "welch_filter": {"ops": tests * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
+        # mode 2 tests each unordered pair once and weights every ordered pair
+        "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) * 2, "bound": "valu"},
         "prepass": {"bytes": P * (4 + S * 5 * 3 * 4 + S * (9 * 4 + 4)), "bound": "hbm"},

@@ -26,8 +26,13 @@ N_SIMD = 256 * 4        # MI355X: 256 CUs x 4 SIMDs
 CLOCK_GHZ = 2.4
 CYCLES_PER_VALU = 4     # a wave64 FP32 VALU instruction on a 16-lane SIMD
 
-# kernel symbol -> the launch names the library's timers use
-TIMER_NAMES = [("k_welch", "welch_filter"), ("k_var_filter", "var_filter"),
+# kernel symbol -> the library timer whose launches it belongs to (a timer
+# can cover several kernels: the variance filter is a pack pass and a filter
+# pass per set group, the mode-2 Welch filter a decision pass, a pack pass
+# and a weighting pass)
+TIMER_NAMES = [("k_welch", "welch_filter"), ("k_welch_decide", "welch_filter"), ("k_accum_pack", "welch_filter"),
+               ("k_welch_accum2", "welch_filter"), ("k_welch_accum", "welch_filter"),
+               ("k_var_filter", "var_filter"), ("k_var_pack", "var_filter"),
                ("k_accumulate_merge", "accumulate_merge64"), ("k_accumulate", "accumulate"),
                ("k_prepass", "prepass")]
 
@@ -98,7 +103,11 @@ def _kib(path: str, counter: str) -> dict[str, list[float]]:
     return {k: v[counter] for k, v in vals.items() if counter in v}
 
 
-def traffic(fetch: str, write: str, cal_f: str, cal_w: str, cal_bytes: int, cmd: str, json_dir: str | None) -> str:
+def traffic(fetch: str, write: str, cal_f: str, cal_w: str, cal_bytes: int, cmd: str, json_dir: str | None,
+            calls: int = 1) -> str:
+    """Per kernel: mean HBM bytes per dispatch.  Per timer (json): all its
+    kernels' bytes over the profiled run divided by `calls` (the number of
+    times the program called each timed entry point)."""
     def cal(path, counter):
         for k, v in _kib(path, counter).items():
             if "copyBuffer" in k:
@@ -109,8 +118,9 @@ def traffic(fetch: str, write: str, cal_f: str, cal_w: str, cal_bytes: int, cmd:
     f, w = _kib(fetch, "FETCH_SIZE"), _kib(write, "WRITE_SIZE")
     out = [f"Command: `{cmd}`", "",
            f"Calibration (device-to-device copy of {cal_bytes} bytes): FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f}.",
-           "", "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB | HBM write MB | HBM MB/launch |",
+           "", "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB | HBM write MB | HBM MB/dispatch |",
            "|---|---|---|---|---|---|---|"]
+    per_timer = collections.defaultdict(lambda: [0.0, 0.0, []])
     for k in f:
         if k.startswith("__amd") or k not in w:
             continue
@@ -118,9 +128,20 @@ def traffic(fetch: str, write: str, cal_f: str, cal_w: str, cal_bytes: int, cmd:
         rd, wr = fk * 1024 * ff, wk * 1024 * fw
         out.append(f"| `{k}` | {len(f[k])} | {fk:.0f} | {wk:.0f} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | {(rd + wr) / 1e6:.1f} |")
         t = timer_name(k)
-        if json_dir and t:
+        if t:
+            acc = per_timer[t]
+            acc[0] += sum(f[k]) * 1024 * ff
+            acc[1] += sum(w[k]) * 1024 * fw
+            acc[2].append(k)
+    if per_timer:
+        out += ["", f"Per timed call ({calls} call(s) of each entry point profiled):", "",
+                "| timer | kernels | HBM read MB | HBM write MB | HBM MB/call |", "|---|---|---|---|---|"]
+    for t, (rd, wr, ks) in per_timer.items():
+        rd, wr = rd / calls, wr / calls
+        out.append(f"| {t} | {', '.join('`%s`' % k for k in ks)} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | {(rd + wr) / 1e6:.1f} |")
+        if json_dir:
             Path(json_dir, f"traffic_{t}.json").write_text(json.dumps(
-                {"kernel": k, "bytes_per_launch": round(rd + wr), "read_bytes": round(rd), "write_bytes": round(wr),
+                {"kernels": ks, "bytes_per_launch": round(rd + wr), "read_bytes": round(rd), "write_bytes": round(wr),
                  "command": cmd}) + "\n")
     return "\n".join(out) + "\n"
 
@@ -132,6 +153,7 @@ def main() -> int:
     ap.add_argument("--cmd", required=True)
     ap.add_argument("--cal-bytes", type=int, default=2**30)
     ap.add_argument("--json-dir")
+    ap.add_argument("--calls", type=int, default=1, help="traffic: calls of each timed entry point in the run")
     a = ap.parse_args()
     if a.mode == "stats":
         sys.stdout.write(stats(a.files[0], a.cmd))
@@ -140,7 +162,7 @@ def main() -> int:
     else:
         if len(a.files) != 4:
             raise SystemExit("traffic needs FETCH WRITE CAL_FETCH CAL_WRITE csv files")
-        sys.stdout.write(traffic(*a.files, a.cal_bytes, a.cmd, a.json_dir))
+        sys.stdout.write(traffic(*a.files, a.cal_bytes, a.cmd, a.json_dir, a.calls))
     return 0
 
 
