@@ -152,6 +152,33 @@ int64_t stater_bufset_bytes(const stater_bufset *bs);
 int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas,
                                 const float *lambdas, int max_moment, unsigned flags);
 
+/* ---- on-disk buffer interchange (PFM) ------------------------------------
+ * The reference's file format: OpenCV imwrite/imread of each buffer as
+ * <stem>-<spp>-<name>.pfm (src/statistics/buffer.cpp:44-57 writes,
+ * src/statistics/statpath.cpp:479-547 reads them in --denoise mode).  "PF"
+ * (RGB) or "Pf" (1 channel), rows bottom to top, little-endian floats;
+ * integer planes are stored as floats and rounded to nearest when read.
+ * Host arrays are top row first, channels interleaved.  Plane copies are
+ * synchronous with the stream. */
+int stater_pfm_info(const char *path, int *width, int *height, int *channels);
+int stater_pfm_read_host(const char *path, float *data, int width, int height, int channels);
+int stater_pfm_write_host(const char *path, const float *data, int width, int height, int channels);
+/* Read a file into / write a device plane (f32, i32 or bf16; 1 or 3 channels). */
+int stater_pfm_read(const char *path, const stater_plane *dst, void *stream);
+int stater_pfm_write(const char *path, const stater_plane *src, void *stream);
+/* OutputBufferSelection::Write (buffer.cpp:44-57) for a set: every buffer
+ * whose reference name matches the ECMAScript regex (whole name; NULL = all)
+ * goes to <stem>-<spp>-<name>.pfm.  Names: film, filmD, t0-b<j>-film-<stat>,
+ * t0-b<j>-<stat> (see stater_bufset_view); "film" is the set-0 sample mean. */
+int stater_bufset_write_pfm(const stater_bufset *bs, const char *stem, uint64_t spp, const char *regex,
+                            void *stream, int *written);
+/* The loader of StatPathIntegrator::Denoise (statpath.cpp:505-545) for a set:
+ * every <stem>-<spp>-<name>.pfm whose name is one of the set's buffers is
+ * read into it (in name order).  film.pfm is left to the caller (the
+ * reference reads it into its Film, the input of denoise_film), as are
+ * other types' buffers (t4/t5/t6 G-buffers). */
+int stater_bufset_read_pfm(stater_bufset *bs, const char *stem, uint64_t spp, void *stream, int *nread);
+
 /* ---- accumulation -----------------------------------------------------
  * A sample chunk: for sample index j (0..spp-1) and component k (R,G,B,u,v)
  * the plane starts at data + j*sample_stride_bytes + k*comp_stride_bytes, with

@@ -114,6 +114,13 @@ SIGNATURES = [
     ("stater_bufset_view", C.c_int, [_P, C.c_char_p, C.POINTER(Plane)]),
     ("stater_bufset_bytes", C.c_int64, [_P]),
     ("stater_bufset_bytes_for", C.c_int64, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_uint]),
+    ("stater_pfm_info", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("stater_pfm_read_host", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int]),
+    ("stater_pfm_write_host", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int]),
+    ("stater_pfm_read", C.c_int, [C.c_char_p, C.POINTER(Plane), _P]),
+    ("stater_pfm_write", C.c_int, [C.c_char_p, C.POINTER(Plane), _P]),
+    ("stater_bufset_write_pfm", C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_char_p, _P, C.POINTER(C.c_int)]),
+    ("stater_bufset_read_pfm", C.c_int, [_P, C.c_char_p, C.c_uint64, _P, C.POINTER(C.c_int)]),
     ("stater_accumulate", C.c_int, [_P, C.POINTER(Samples), C.c_int, _P]),
     ("stater_default_params", None, [C.POINTER(DenoiseParams)]),
     ("stater_denoise_vars", C.c_int, [_P, C.POINTER(DenoiseParams), C.POINTER(GBuffers), _P]),

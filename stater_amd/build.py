@@ -18,7 +18,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libstater.so"
-SOURCES = ["api.cpp", "batch.cpp", "server.cpp", "accumulate.hip", "denoise.hip", "synth.hip", "convert.hip"]
+SOURCES = ["api.cpp", "batch.cpp", "server.cpp", "pfm.cpp", "accumulate.hip", "denoise.hip", "synth.hip", "convert.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STATER_ARCH", "gfx950")
