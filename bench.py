@@ -81,12 +81,13 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
     log(f"cpu baseline inputs generated in {time.perf_counter() - t0:.1f}s")
     feats = [(f["albedo"], 0.02), (f["normal"], 0.1), (f["depth"], 0.5)]
     t0 = time.perf_counter()
-    n, st = oracle.accumulate(smp, lambdas, 5, nthreads=nthreads)
+    n, st = oracle.accumulate(smp, lambdas, 5, sanitize=True, nthreads=nthreads)
+    film = oracle.film_resolve(oracle.film_add(smp, sanitize=True))
     t1 = time.perf_counter()
     jb, lnS2, se2, m2C = oracle.prepass(n, st, p.var_z)
     vD, vCD = oracle.denoise_vars(n, st[:, 1], lnS2.astype(np.float32), se2.astype(np.float32),
                                   m2C.astype(np.float32), r, p.ds_factor, p.var_z, feats, nthreads=nthreads)
-    oracle.denoise_film(n, st[:, 0], vCD.astype(np.float32), st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
+    oracle.denoise_film(n, st[:, 0], vCD.astype(np.float32), film, r, p.ds_factor, p.alpha, mt_mode=2,
                         jb=jb.astype(np.float32), feats=feats, nthreads=nthreads)
     t2 = time.perf_counter()
     total = t2 - t0
@@ -96,7 +97,7 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
         "cores": nthreads,
         "kind": "port",
         "sample": f"{w}x{h} crop at ({x0},{y0}) of the same 1080p frame, {spp} spp mt5; oracle accumulate "
-                  f"(reference FP32 recurrence, 16x16 tiles) {t1 - t0:.2f}s + FP64 prepass/variance/Welch "
+                  f"(reference FP32 recurrence, 16x16 tiles, sanitised) and Film {t1 - t0:.2f}s + FP64 prepass/variance/Welch "
                   f"filters {t2 - t1:.2f}s on {nthreads} threads of {cpu_model()}",
         "accumulate_gsample_s": round(w * h * spp / (t1 - t0) / 1e9, 4),
     }
@@ -272,17 +273,21 @@ def main() -> int:
     log(f"generating {P * spp / 1e6:.0f}M samples ({chunk.buf.nbytes / 2**30:.1f} GiB) on the device")
     synth_samples(ctx, sc, chunk, 0, feats)
     ctx.synchronize()
-    bs = sa.BufferSet(ctx, W, H, lambdas, 5)
+    # the reference's data flow: sanitised samples feed the statistics and the
+    # Film (statpath.cpp:313-349); denoiseFilm filters the resolved Film
+    bs = sa.BufferSet(ctx, W, H, lambdas, 5, flags=sa.STATER_BUFSET_FILM)
     S = len(set(float(x) for x in lambdas) | {1.0})
     p = sa.default_params(radius=r)
     g = feats.gbuffers()
-    desc = chunk.desc()
+    desc = chunk.desc(sanitize=True)
+    film = bs.view("film")
 
     def step():
         bs.reset()
         bs.accumulate(desc)
+        bs.film_resolve()
         bs.denoise_vars(p, g)
-        bs.denoise_film(p, g)
+        bs.denoise_film(p, g, film=film)
 
     for i in range(args.warmup):
         step()
@@ -350,14 +355,14 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Philox sample model, device-generated before timing)",
-        "config": {"workload": f"{W}x{H} RGB, {spp} spp, mt5 (6 lambda slots, 5 Box-Cox), M4C, "
-                               f"prepass + variance filter + Welch filter, r={r}, alpha=0.005, "
+        "config": {"workload": f"{W}x{H} RGB, {spp} spp sanitised, mt5 (6 lambda slots, 5 Box-Cox), M4C + "
+                               f"Film, prepass + variance filter + Welch filter of the Film, r={r}, alpha=0.005, "
                                f"albedo/normal/depth weights", "width": W, "height": H, "spp": spp,
                    "radius": r, "sets": S},
         "gsample_s": gsample,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "what": "one 1080p job (reset + accumulate + denoise_vars + denoise_film) alone"},
+                       "what": "one 1080p job (reset + accumulate + film + denoise_vars + denoise_film) alone"},
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

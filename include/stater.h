@@ -122,6 +122,7 @@ int  stater_timing_get(stater_ctx *ctx, int index, char *name, int name_len,
  * lambdas are the reference's "lambdas" list (scenes/render-denoise-mt5.pbrt:26);
  * n_lambdas = 0 means no transform (the reference's transform = false). */
 #define STATER_BUFSET_NO_DERIVED 1u  /* accumulate-only set: skip denoise planes */
+#define STATER_BUFSET_FILM       2u  /* also keep the Film's pixel sums (film = Film::UpdateImage) */
 
 int stater_bufset_create(stater_ctx *ctx, int width, int height,
                          int n_lambdas, const float *lambdas, int max_moment,
@@ -140,7 +141,8 @@ int stater_bufset_reset(stater_bufset *bs, void *stream);
 /* Look up a buffer by the reference's name: "n", "film", "filmD", "targetspp",
  * "t0-b<j>-film-<stat>" (untransformed statistics of lambda slot j) or
  * "t0-b<j>-<stat>" (Box-Cox statistics of slot j), stat in
- * mean m2 m3 m4 c2X c2Y m1 jb lnS2 se2 m2C varD varCD discr meanVar meanD n. */
+ * mean m2 m3 m4 c2X c2Y m1 jb lnS2 se2 m2C varD varCD discr meanVar meanD n.
+ * "film" is the resolved Film with STATER_BUFSET_FILM, else the set-0 mean. */
 int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out);
 /* Copy a plane into another of the same shape, converting the layout
  * (PLANAR <-> INTERLEAVED, e.g. a caller's CV_32FC3 film) and the storage type
@@ -151,6 +153,18 @@ int64_t stater_bufset_bytes(const stater_bufset *bs);
 /* Bytes a set of this shape would hold, without allocating. */
 int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas,
                                 const float *lambdas, int max_moment, unsigned flags);
+
+/* ---- film (pbrt Film fed the same samples) ------------------------------
+ * With STATER_BUFSET_FILM every stater_accumulate call also adds its samples
+ * to the Film's pixel sums as the reference's render loop does: the tile sums
+ * the (sanitised) RGB of the pixel's samples in order (FilmTile::AddSample,
+ * box filter of radius 1/2: weight 1), and the merge adds their XYZ and the
+ * weight count (Film::MergeFilmTile), one merge per call.  stater_film_resolve
+ * is Film::UpdateImage (src/core/film.cpp:188-220, no splats): RGB =
+ * max(0, XYZToRGB(xyz) / weight sum) * scale, into out (NULL: the set's
+ * "film" plane, which is then what denoise_film's film argument should be).
+ * stater_bufset_reset clears the sums. */
+int stater_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, void *stream);
 
 /* ---- on-disk buffer interchange (PFM) ------------------------------------
  * The reference's file format: OpenCV imwrite/imread of each buffer as

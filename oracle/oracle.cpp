@@ -506,6 +506,55 @@ ORC_API void orc_accumulate_f32(int W, int H, int S, const float *lambdas, int m
     accumulate<float, true>(W, H, S, lambdas, mm, samples, spp, counts, san, n, stats, nthreads);
 }
 
+// ---- Film: the image denoiseFilm filters --------------------------------
+// pbrt-v3's Film fed one call's samples with the box filter of radius 1/2:
+// FilmTile::AddSample (src/core/film.h:152-190) adds L * 1 * 1 to the pixel's
+// contribSum in sample order and 1 to its weight sum; Film::MergeFilmTile
+// (src/core/film.cpp:136-149) adds RGBToXYZ(contribSum) and the weight sum
+// (RGBToXYZ as written at src/core/spectrum.h:72-76).  Samples exactly on a
+// pixel's left or top edge count in their own pixel only (the box filter
+// would add them to the neighbour across that edge too).  xyzw: [4][P].
+ORC_API void orc_film_add(int W, int H, const float *samples, int spp, const int32_t *counts, int san,
+                          float *xyzw) {
+    const size_t P = (size_t)W * H;
+    for (size_t p = 0; p < P; ++p) {
+        const int cnt = counts ? std::min(counts[p], spp) : spp;
+        if (cnt <= 0) continue;
+        float sum[3] = {0.f, 0.f, 0.f};
+        for (int j = 0; j < cnt; ++j) {
+            const float *sj = samples + (size_t)j * 5 * P + p;
+            float rgb[3] = {sj[0], sj[P], sj[2 * P]};
+            if (san) sanitize(rgb[0], rgb[1], rgb[2]);
+            for (int c = 0; c < 3; ++c) sum[c] += rgb[c];
+        }
+        const float X = 0.412453f * sum[0] + 0.357580f * sum[1] + 0.180423f * sum[2];
+        const float Y = 0.212671f * sum[0] + 0.715160f * sum[1] + 0.072169f * sum[2];
+        const float Z = 0.019334f * sum[0] + 0.119193f * sum[1] + 0.950227f * sum[2];
+        xyzw[p] += X;
+        xyzw[P + p] += Y;
+        xyzw[2 * P + p] += Z;
+        xyzw[3 * P + p] += (float)cnt;
+    }
+}
+
+// Film::UpdateImage (src/core/film.cpp:188-220) without splats: XYZToRGB
+// (src/core/spectrum.h:66-70), times 1/weight and clamped at 0 when the
+// weight sum is not 0, times scale.  rgb: [3][P].
+ORC_API void orc_film_resolve(int P, const float *xyzw, float scale, float *rgb) {
+    for (int p = 0; p < P; ++p) {
+        const float X = xyzw[p], Y = xyzw[P + p], Z = xyzw[2 * (size_t)P + p], w = xyzw[3 * (size_t)P + p];
+        float o[3];
+        o[0] = 3.240479f * X - 1.537150f * Y - 0.498535f * Z;
+        o[1] = -0.969256f * X + 1.875991f * Y + 0.041556f * Z;
+        o[2] = 0.055648f * X - 0.204043f * Y + 1.057311f * Z;
+        if (w != 0.f) {
+            const float inv = 1.f / w;
+            for (int c = 0; c < 3; ++c) o[c] = std::max(0.f, o[c] * inv);
+        }
+        for (int c = 0; c < 3; ++c) rgb[(size_t)c * P + p] = o[c] * scale;
+    }
+}
+
 // The same recurrence in FP64 with the FP64 Box-Cox.
 ORC_API void orc_accumulate_f64(int W, int H, int S, const float *lambdas, int mm, const float *samples,
                                 int spp, const int32_t *counts, int san, int32_t *n, double *stats,

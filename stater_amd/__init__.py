@@ -27,8 +27,8 @@ STATER_OK = 0
  STATER_E_NOTFOUND) = (-1, -2, -3, -4, -5, -6)
 STATER_F32, STATER_I32, STATER_BF16 = 0, 1, 2
 STATER_PLANAR, STATER_INTERLEAVED = 0, 1
+STATER_BUFSET_NO_DERIVED, STATER_BUFSET_FILM = 1, 2
 STATER_CTX_TIMING = 1
-STATER_BUFSET_NO_DERIVED = 1
 STATER_ACC_SEQUENTIAL, STATER_ACC_MERGE64 = 0, 1
 MAX_LAMBDAS, MAX_RADIUS, MAX_GBUFFERS, MAX_EDGES = 8, 20, 4, 16
 
@@ -114,6 +114,7 @@ SIGNATURES = [
     ("stater_bufset_view", C.c_int, [_P, C.c_char_p, C.POINTER(Plane)]),
     ("stater_bufset_bytes", C.c_int64, [_P]),
     ("stater_bufset_bytes_for", C.c_int64, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_uint]),
+    ("stater_film_resolve", C.c_int, [_P, C.c_float, C.POINTER(Plane), _P]),
     ("stater_pfm_info", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("stater_pfm_read_host", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int]),
     ("stater_pfm_write_host", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int]),
@@ -325,6 +326,10 @@ class BufferSet:
 
     def calc_mean_vars(self, stream=None) -> None:
         check(self.lib.stater_calc_mean_vars(self.h, stream))
+
+    def film_resolve(self, scale: float = 1.0, out: Plane | None = None, stream=None) -> None:
+        """Film::UpdateImage into out (None: the set's "film"); sets created with STATER_BUFSET_FILM."""
+        check(self.lib.stater_film_resolve(self.h, scale, C.byref(out) if out else None, stream))
 
 
 class Batch:

@@ -156,11 +156,14 @@ __device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f,
 constexpr int nstats(int MM) { return MM >= 5 ? 7 : MM; }
 
 // statpath.cpp:313-332: NaN channel, luminance < -1e-5 or infinite -> black.
+// The reference compares the float luminance with the double -1e-5; for every
+// float y that is y < (float)-1e-5 (the float nearest -1e-5 lies just above
+// it, and no float lies between it and -1e-5), so no FP64 is needed.
 __device__ __forceinline__ bool keep_sample(float r, float g, float b) {
 #pragma clang fp contract(off)
     const bool nan = __builtin_isnan(r) || __builtin_isnan(g) || __builtin_isnan(b);
     const float y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
-    return !(nan || (double)y < -1e-5 || __builtin_isinf(y));
+    return !(nan || y < -1e-5f || __builtin_isinf(y));
 }
 
 // The own channel is read by one wavefront only (streamed, non-temporal);
@@ -173,6 +176,16 @@ __device__ __forceinline__ float load_sample(const char *base, int64_t off, int6
     const float b = *(const float *)(base + off + 2 * comp);
     const float v = c == 0 ? r : (c == 1 ? g : b);
     return keep_sample(r, g, b) ? v : 0.f;
+}
+
+// This call's film contribution of a pixel-channel: the FilmTile sum of its
+// samples (FilmTile::AddSample, src/core/film.h:152-190: box filter of radius
+// 1/2, weight 1, so contribSum += L in sample order) and, from the R lane,
+// the weight count.  Merged into the Film's XYZ sums by k_film_merge.
+__device__ __forceinline__ void film_store(const AccArgs &a, int c, uint32_t pix, float sum, int cnt) {
+    if (!a.film_chunk) return;
+    a.film_chunk[(size_t)c * a.plane + pix] = sum;
+    if (c == 0) a.film_chunk[3 * a.plane + pix] = (float)cnt;
 }
 
 template <int T, int MM>
@@ -188,7 +201,10 @@ __device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
     if (a.counts)
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
                            a.count_offset);
-    if (cnt <= 0) return;
+    if (cnt <= 0) {
+        film_store(a, c, pix, 0.f, 0);
+        return;
+    }
     uint32_t n = (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
 
@@ -216,9 +232,11 @@ __device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
         }
     }
 
+    float fsum = 0.f;  // the Film tile's sum of this pixel-channel's samples
     for (int j = 0; j < cnt; ++j) {
         const int64_t off = (int64_t)j * a.sstride;
         const float v = load_sample(base, off, a.comp, c, a.sanitize);
+        fsum += v;
         float xD = 0.f, yD = 0.f;
         if (MM >= 5) {  // shared by the three channel wavefronts: keep in L2
             xD = *(const float *)(base + off + 3 * a.comp) - 0.5f;
@@ -248,6 +266,7 @@ __device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
     if (c == 0) a.n[pix] = (int32_t)n;
+    film_store(a, c, pix, fsum, cnt);
 }
 
 // FP64 merge of two partial moment sets (Chan, Golub & LeVeque 1979; Pebay
@@ -286,7 +305,10 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
     if (a.counts)
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
                            a.count_offset);
-    if (cnt <= 0) return;
+    if (cnt <= 0) {
+        film_store(a, c, pix, 0.f, 0);
+        return;
+    }
     const uint32_t n0 = (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
 
@@ -320,9 +342,11 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         mg[s] = st[s][0];
     }
 
+    float fsum = 0.f;  // the Film tile's sum of this pixel-channel's samples
     for (int j = 0; j < cnt; ++j) {
         const int64_t off = (int64_t)j * a.sstride;
         const float v = load_sample(base, off, a.comp, c, a.sanitize);
+        fsum += v;
         float xD = 0.f, yD = 0.f;
         if (MM >= 5) {
             xD = *(const float *)(base + off + 3 * a.comp) - 0.5f;
@@ -367,6 +391,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
     if (c == 0) a.n[pix] = (int32_t)(n0 + cnt);
+    film_store(a, c, pix, fsum, cnt);
 }
 
 // At least three waves per SIMD (at most 168 VGPRs); the six-set body with
@@ -382,6 +407,67 @@ __global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(
 __global__ void k_zero(uint32_t *p, size_t words) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+}
+
+// Film::MergeFilmTile (src/core/film.cpp:136-149): the Film's XYZ sums take
+// this call's tile sum through RGBToXYZ (src/core/spectrum.h:72-76, evaluated
+// as written) and its weight sum the sample count.
+__global__ void __launch_bounds__(256) k_film_merge(const float *chunk, float *xyzw, int W, int ld, size_t plane) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    const size_t pix = (size_t)y * ld + x;
+    const float cnt = chunk[3 * plane + pix];
+    if (!(cnt > 0.f)) return;
+    const float r = chunk[pix], g = chunk[plane + pix], b = chunk[2 * plane + pix];
+    const float X = 0.412453f * r + 0.357580f * g + 0.180423f * b;
+    const float Y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
+    const float Z = 0.019334f * r + 0.119193f * g + 0.950227f * b;
+    xyzw[pix] += X;
+    xyzw[plane + pix] += Y;
+    xyzw[2 * plane + pix] += Z;
+    xyzw[3 * plane + pix] += cnt;
+}
+
+// Film::UpdateImage (src/core/film.cpp:188-220) without splats: XYZToRGB
+// (src/core/spectrum.h:66-70), divided by the weight sum and clamped at 0,
+// times scale.
+__global__ void __launch_bounds__(256) k_film_resolve(const float *xyzw, int W, int ld, size_t plane, float scale,
+                                                      float *out, int64_t out_pitch, int64_t out_cs, int out_il) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    const size_t pix = (size_t)y * ld + x;
+    const float X = xyzw[pix], Y = xyzw[plane + pix], Z = xyzw[2 * plane + pix], w = xyzw[3 * plane + pix];
+    float rgb[3];
+    rgb[0] = 3.240479f * X - 1.537150f * Y - 0.498535f * Z;
+    rgb[1] = -0.969256f * X + 1.875991f * Y + 0.041556f * Z;
+    rgb[2] = 0.055648f * X - 0.204043f * Y + 1.057311f * Z;
+    if (w != 0.f) {
+        const float inv = 1.f / w;
+        for (int c = 0; c < 3; ++c) rgb[c] = fmaxf(0.f, rgb[c] * inv);
+    }
+    char *row = (char *)out + (int64_t)y * out_pitch;
+    for (int c = 0; c < 3; ++c) {
+        float *dst = out_il ? (float *)(row + ((int64_t)x * 3 + c) * 4) : (float *)(row + (int64_t)x * 4 + c * out_cs);
+        *dst = rgb[c] * scale;
+    }
+}
+
+int launch_film_merge(stater_bufset *bs, hipStream_t st) {
+    hipLaunchKernelGGL(k_film_merge, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_chunk,
+                       bs->film_xyzw, bs->W, bs->ld, bs->plane);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+int launch_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, hipStream_t st) {
+    LaunchTimer t(bs->ctx, st, "film_resolve");
+    hipLaunchKernelGGL(k_film_resolve, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_xyzw, bs->W,
+                       bs->ld, bs->plane, scale, (float *)out->ptr, out->pitch_bytes, out->channel_stride_bytes,
+                       out->layout == STATER_INTERLEAVED ? 1 : 0);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
 }
 
 int launch_zero(void *p, size_t bytes, hipStream_t st) {
@@ -432,6 +518,7 @@ AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s) {
         a.lam[k] = k < bs->S ? bs->set_lambda[k] : 1.f;
         a.inv_lam[k] = 1.f / a.lam[k];
     }
+    a.film_chunk = bs->film_chunk;
     return a;
 }
 
@@ -452,7 +539,7 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
-    return STATER_OK;
+    return bs->film_xyzw ? launch_film_merge(bs, st) : STATER_OK;
 }
 
 }  // namespace stater

@@ -284,6 +284,7 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     L.plane = (size_t)L.ld * H;
     // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
     size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
+    if (flags & STATER_BUFSET_FILM) np += 3 + 4 + 4;  // film, XYZ + weight sums, per-call sums
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
         np += (size_t)L.S * (1 + 6 * 3) + 9 + 1 + 3 * kBitsWords + kVarRecFloats;
         L.as_bytes = as_scratch_bytes_for((size_t)W * H);
@@ -319,6 +320,11 @@ static stater_bufset *bufset_over(stater_ctx *ctx, const Layout &L, int width, i
     bs->stats = p; p += (size_t)L.S * N_STATS * 3 * L.plane;
     bs->lref = p; p += 3 * L.plane;
     bs->mshift = p; p += (size_t)(L.S - 1) * 3 * L.plane;
+    if (flags & STATER_BUFSET_FILM) {
+        bs->film_rgb = p; p += 3 * L.plane;
+        bs->film_xyzw = p; p += 4 * L.plane;
+        bs->film_chunk = p; p += 4 * L.plane;
+    }
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
         bs->jb = p; p += (size_t)L.S * L.plane;
         float **d[6] = {&bs->lnS2, &bs->se2, &bs->m2C, &bs->varD, &bs->varCD, &bs->discr};
@@ -391,7 +397,9 @@ int64_t stater_bufset_bytes(const stater_bufset *bs) { return bs ? (int64_t)bs->
 int stater_bufset_reset(stater_bufset *bs, void *stream) {
     if (!bs) return fail(STATER_E_ARG, "stater_bufset_reset: NULL set");
     const size_t bytes = (1 + (size_t)bs->S * N_STATS * 3) * bs->plane * 4;
-    return launch_zero(bs->base, bytes, pick_stream(bs, stream));
+    int rc = launch_zero(bs->base, bytes, pick_stream(bs, stream));
+    if (!rc && bs->film_xyzw) rc = launch_zero(bs->film_xyzw, 4 * bs->plane * 4, pick_stream(bs, stream));
+    return rc;
 }
 
 static void fill_plane(const stater_bufset *bs, void *ptr, int channels, stater_dtype dt, stater_plane *out) {
@@ -412,7 +420,10 @@ int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *
     const bool derived = !(bs->flags & STATER_BUFSET_NO_DERIVED);
     auto need_derived = [&]() { return fail(STATER_E_NOTFOUND, "buffer '" + nm + "' not allocated (NO_DERIVED)"); };
     if (nm == "n") { fill_plane(bs, bs->n, 1, STATER_I32, out); return STATER_OK; }
-    if (nm == "film") { fill_plane(bs, bs->stat(0, ST_MEAN, 0), 3, STATER_F32, out); return STATER_OK; }
+    if (nm == "film") {
+        fill_plane(bs, bs->film_rgb ? bs->film_rgb : bs->stat(0, ST_MEAN, 0), 3, STATER_F32, out);
+        return STATER_OK;
+    }
     if (nm == "filmD") { if (!derived) return need_derived(); fill_plane(bs, bs->filmD, 3, STATER_F32, out); return STATER_OK; }
     if (nm == "targetspp") { if (!derived) return need_derived(); fill_plane(bs, bs->targetspp, 1, STATER_I32, out); return STATER_OK; }
     // t<type>-b<slot>-[film-]<stat>; the set holds type 0 (Radiance).
@@ -448,6 +459,19 @@ int stater_accumulate(stater_bufset *bs, const stater_samples *samples, int mode
         return fail(STATER_E_ARG, "stater_accumulate: unknown mode");
     if (samples->spp == 0) return STATER_OK;
     return launch_accumulate(bs, samples, mode, pick_stream(bs, stream));
+}
+
+int stater_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, void *stream) {
+    if (!bs) return fail(STATER_E_ARG, "stater_film_resolve: NULL set");
+    if (!bs->film_xyzw) return fail(STATER_E_STATE, "stater_film_resolve: set was created without STATER_BUFSET_FILM");
+    stater_plane own;
+    if (!out) {
+        fill_plane(bs, bs->film_rgb, 3, STATER_F32, &own);
+        out = &own;
+    }
+    if (!out->ptr || out->width != bs->W || out->height != bs->H || out->channels != 3 || out->dtype != STATER_F32)
+        return fail(STATER_E_ARG, "stater_film_resolve: out must be a W x H 3-channel f32 plane");
+    return launch_film_resolve(bs, scale, out, pick_stream(bs, stream));
 }
 
 void stater_default_params(stater_denoise_params *p) {

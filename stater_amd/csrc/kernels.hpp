@@ -25,6 +25,7 @@ struct AccArgs {
     int W, H;
     float lam[kMaxSets];
     float inv_lam[kMaxSets];
+    float *film_chunk;  // [4][plane] this call's film RGB sums and sample count, or null
 };
 
 struct FeatArgs {

@@ -1,0 +1,63 @@
+"""The Film fed by stater_accumulate (STATER_BUFSET_FILM) against the
+oracle's pbrt Film, bit for bit: one call, several calls with ragged counts,
+the FP64-merge accumulation mode, and resolving into an interleaved plane."""
+import numpy as np
+import pytest
+
+import oracle
+import stater_amd as sa
+from gpu_helpers import generate
+from stater_amd import hipmem
+from stater_amd.workload import MT5_LAMBDAS
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("mode", [sa.STATER_ACC_SEQUENTIAL, sa.STATER_ACC_MERGE64])
+def test_film_matches_pbrt_film(ctx, mode):
+    W, H = 96, 64
+    sc, chunk, feats = generate(ctx, 2, 7, W, H, 48)
+    smp = chunk.to_numpy()
+    rng = np.random.default_rng(4)
+    counts = rng.integers(0, 30, (H, W)).astype(np.int32)
+    cbuf = hipmem.DeviceBuffer(W * H * 4)
+    cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+    hipmem.write_plane(cplane, counts[None])
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5, flags=sa.STATER_BUFSET_FILM)
+    # two calls: ragged counts over the first 30 samples, then the next 18 for every pixel
+    bs.accumulate(chunk.desc(spp=30, sanitize=True, counts=cplane), mode)
+    first = chunk.desc(spp=18, sanitize=True)
+    first.data = chunk.desc().data + 30 * chunk.desc().sample_stride_bytes
+    bs.accumulate(first, mode)
+    bs.film_resolve()
+    ctx.synchronize()
+    ref = oracle.film_add(smp[:30], counts=counts, sanitize=True)
+    ref = oracle.film_add(smp[30:48], ref, sanitize=True)
+    film = hipmem.read_plane(bs.view("film"))
+    assert np.array_equal(_bits(film), _bits(oracle.film_resolve(ref)))
+    assert np.array_equal(hipmem.read_plane(bs.view("t0-b0-film-n"))[0], counts + 18)
+
+
+def test_resolve_scale_interleaved_and_reset(ctx):
+    W, H = 40, 24
+    sc, chunk, feats = generate(ctx, 2, 8, W, H, 16)
+    bs = sa.BufferSet(ctx, W, H, (), 4, flags=sa.STATER_BUFSET_FILM)
+    bs.accumulate(chunk.desc(sanitize=True))
+    buf = hipmem.DeviceBuffer(3 * 64 * 4 * H)
+    inter = sa.plane(buf.ptr, W, H, 3, 3 * 64 * 4, 4, layout=sa.STATER_INTERLEAVED)
+    bs.film_resolve(2.0, inter)
+    ctx.synchronize()
+    ref = oracle.film_resolve(oracle.film_add(chunk.to_numpy(), sanitize=True), 2.0)
+    got = buf.download(np.float32, 3 * 64 * H).reshape(H, 3 * 64)[:, : 3 * W].reshape(H, W, 3).transpose(2, 0, 1)
+    assert np.array_equal(_bits(got), _bits(ref))
+    bs.reset()
+    bs.film_resolve()
+    ctx.synchronize()
+    assert not np.any(hipmem.read_plane(bs.view("film")))
+    # a set without the flag has no Film: "film" is the set-0 mean and resolving is refused
+    plain = sa.BufferSet(ctx, W, H, (), 2)
+    assert sa.load_library().stater_film_resolve(plain.h, 1.0, None, None) == sa.STATER_E_STATE
