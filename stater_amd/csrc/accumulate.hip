@@ -7,13 +7,14 @@
 // uploaded to the GPU every iteration (estimator.cpp:346-353).  Here the
 // statistics never leave device memory.
 //
-// Mapping: one thread per (pixel, channel).  A 192-thread workgroup is three
-// wavefronts, one per RGB channel, over 64 consecutive pixels of a row, so
-// every sample-plane load of a wavefront is one coalesced 256-byte segment and
-// the three channel wavefronts share the (u, v) lines through L1.  Each thread
-// keeps its pixel-channel's statistics of every set in registers for the
-// whole chunk and walks the samples in order, so the update stays the
-// reference's sequential recurrence.
+// Mapping (sequential mode): one thread per pixel, 64 consecutive pixels of a
+// row per wavefront, so every sample-plane load of a wavefront is one
+// coalesced 256-byte segment.  Each thread keeps the three channels'
+// statistics of every set in registers for the whole chunk and walks the
+// samples in order, so the update stays the reference's sequential
+// recurrence; the guard, the count and the per-sample factors are computed
+// once per pixel-sample for all channels and sets.  The FP64-merge mode runs
+// one thread per (pixel, channel): three wavefronts per 64 pixels.
 //
 // Numerics:
 //  * set 0 (untransformed) runs the reference update in its exact FP32
@@ -31,8 +32,9 @@
 //    Central moments, c2X, c2Y and m1 do not depend on the shift; the stored
 //    mean is K + mean(y - K); mean(y - K) itself persists in an internal
 //    plane so that a later call continues the recurrence bit for bit.  One
-//    log per sample-channel is shared by all sets; expm1 uses a range-reduced
-//    polynomial in base 2 and the log the hardware log2.  The update is the reference recurrence in factored form,
+//    log per sample-channel is shared by all sets; expm1 is evaluated in base
+//    2 (wexpm1 below) and the log is the hardware log2.  The update is the
+//    reference recurrence in factored form,
 //        m2 += d^2 (1 - 1/n)
 //        m3 += -3 (d/n) m2 + d^3 (1 - 1/n^2)
 //        m4 += -4 (d/n) m3 - 6 (d^2/n^2) m2 + d^4 (1 - 1/n^3)
@@ -188,85 +190,116 @@ __device__ __forceinline__ void film_store(const AccArgs &a, int c, uint32_t pix
     if (c == 0) a.film_chunk[3 * a.plane + pix] = (float)cnt;
 }
 
+// One thread per pixel (sequential mode): the three channels' statistics of
+// every set live in one lane, so the sanitising guard, the sample count, the
+// per-sample factors of the factored recurrence and the (u, v) loads are
+// done once per pixel-sample instead of once per channel, and each sample
+// costs five loads per pixel instead of fifteen.
 template <int T, int MM>
-__device__ __forceinline__ void acc_tile(const AccArgs a, int bx, int by) {
+__device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
     constexpr int NS = nstats(MM);
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront-uniform channel
-    const int x = bx * 64 + (threadIdx.x & 63);
-    const int y = by;
     if (x >= a.W) return;
-    // 32-bit pixel offset on uniform plane bases (width, height <= 16384)
     uint32_t pix = (uint32_t)y * (uint32_t)a.ld + (uint32_t)x;
     int cnt = a.spp;
     if (a.counts)
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
                            a.count_offset);
     if (cnt <= 0) {
-        film_store(a, c, pix, 0.f, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) film_store(a, c, pix, 0.f, 0);
         return;
     }
     uint32_t n = (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
 
-    float st[T + 1][NS];
+    float st[3][T + 1][NS];
 #pragma unroll
-    for (int s = 0; s <= T; ++s)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
+        for (int s = 0; s <= T; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                st[c][s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
 
-    // Box-Cox reference point: the first sample the pixel-channel ever saw.
-    float L = 0.f, Wt[T + 1];
+    float L[3] = {0.f, 0.f, 0.f}, Wt[3][T + 1];
     if (T > 0) {
-        const size_t lp = (size_t)c * a.plane + pix;
+        float v0[3] = {0.f, 0.f, 0.f};
         if (n == 0) {
-            const float v0 = load_sample(base, 0, a.comp, c, a.sanitize);
-            L = (v0 > 0.f && __builtin_isfinite(v0)) ? __builtin_amdgcn_logf(v0) * 0.69314718055994531f : 0.f;
-            a.lref[lp] = L;
-        } else {
-            L = a.lref[lp];
+            v0[0] = *(const float *)base;
+            v0[1] = *(const float *)(base + a.comp);
+            v0[2] = *(const float *)(base + 2 * a.comp);
+            if (a.sanitize && !keep_sample(v0[0], v0[1], v0[2])) v0[0] = v0[1] = v0[2] = 0.f;
         }
 #pragma unroll
-        for (int s = 1; s <= T; ++s) {
-            Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
-            st[s][0] = n ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
+        for (int c = 0; c < 3; ++c) {
+            const size_t lp = (size_t)c * a.plane + pix;
+            if (n == 0) {
+                L[c] = (v0[c] > 0.f && __builtin_isfinite(v0[c])) ? __builtin_amdgcn_logf(v0[c]) * 0.69314718055994531f
+                                                                  : 0.f;
+                a.lref[lp] = L[c];
+            } else {
+                L[c] = a.lref[lp];
+            }
+#pragma unroll
+            for (int s = 1; s <= T; ++s) {
+                Wt[c][s] = expf(a.lam[s] * L[c]) * a.inv_lam[s];
+                st[c][s][0] = n ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
+            }
         }
     }
 
-    float fsum = 0.f;  // the Film tile's sum of this pixel-channel's samples
+    float fsum[3] = {0.f, 0.f, 0.f};
+    // The next sample's loads are issued before this sample's arithmetic.
+    auto load = [&](int j, float (&r)[5]) {
+        const char *sj = base + (int64_t)j * a.sstride;
+#pragma unroll
+        for (int k = 0; k < (MM >= 5 ? 5 : 3); ++k) r[k] = __builtin_nontemporal_load((const float *)(sj + k * a.comp));
+    };
+    float nx[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    load(0, nx);
     for (int j = 0; j < cnt; ++j) {
-        const int64_t off = (int64_t)j * a.sstride;
-        const float v = load_sample(base, off, a.comp, c, a.sanitize);
-        fsum += v;
+        float v[3] = {nx[0], nx[1], nx[2]};
         float xD = 0.f, yD = 0.f;
-        if (MM >= 5) {  // shared by the three channel wavefronts: keep in L2
-            xD = *(const float *)(base + off + 3 * a.comp) - 0.5f;
-            yD = *(const float *)(base + off + 4 * a.comp) - 0.5f;
+        if (MM >= 5) {
+            xD = nx[3] - 0.5f;
+            yD = nx[4] - 0.5f;
         }
+        if (j + 1 < cnt) load(j + 1, nx);
+        if (a.sanitize && !keep_sample(v[0], v[1], v[2])) v[0] = v[1] = v[2] = 0.f;
         ++n;
         const float nf = (float)n;
-        meng_exact<MM>(st[0], v, nf, xD, yD);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            fsum[c] += v[c];
+            meng_exact<MM>(st[c][0], v[c], nf, xD, yD);
+        }
         if (T > 0) {
             const NFac f = nfac(nf);
-            // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
-            const float dl = __builtin_amdgcn_logf(v) * 0.69314718055994531f - L;
 #pragma unroll
-            for (int s = 1; s <= T; ++s) meng_factored<MM>(st[s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[s]), f, xD, yD);
+            for (int c = 0; c < 3; ++c) {
+                // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
+                const float dl = __builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c];
+#pragma unroll
+                for (int s = 1; s <= T; ++s)
+                    meng_factored<MM>(st[c][s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[c][s]), f, xD, yD);
+            }
         }
     }
-    // The statistic addresses are rebuilt after the loop rather than held in
-    // 2 x (T+1) x NS registers across it.
     asm volatile("" : "+v"(pix));
 #pragma unroll
-    for (int s = 1; s <= T; ++s) {
-        a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[s][0];
-        st[s][0] += expm1f(a.lam[s] * L) * a.inv_lam[s];  // K: shift of the reference point
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] = st[c][s][0];
+            st[c][s][0] += expm1f(a.lam[s] * L[c]) * a.inv_lam[s];  // K: shift of the reference point
+        }
+#pragma unroll
+        for (int s = 0; s <= T; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[c][s][k];
+        film_store(a, c, pix, fsum[c], cnt);
     }
-#pragma unroll
-    for (int s = 0; s <= T; ++s)
-#pragma unroll
-        for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
-    if (c == 0) a.n[pix] = (int32_t)n;
-    film_store(a, c, pix, fsum, cnt);
+    a.n[pix] = (int32_t)n;
 }
 
 // FP64 merge of two partial moment sets (Chan, Golub & LeVeque 1979; Pebay
@@ -399,7 +432,9 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
 #define STATER_ACC_OCC __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3)))
 
 template <int T, int MM>
-__global__ void STATER_ACC_OCC k_accumulate(AccArgs a) { acc_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(64) k_accumulate_px(AccArgs a) {
+    accp_tile<T, MM>(a, blockIdx.x * 64 + threadIdx.x, blockIdx.y);
+}
 
 template <int T, int MM>
 __global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
@@ -481,7 +516,7 @@ int launch_zero(void *p, size_t bytes, hipStream_t st) {
 template <int T, int MM>
 static void launch_tm(const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
     if (merge) hipLaunchKernelGGL((k_accumulate_merge<T, MM>), grid, dim3(192), 0, st, a);
-    else hipLaunchKernelGGL((k_accumulate<T, MM>), grid, dim3(192), 0, st, a);
+    else hipLaunchKernelGGL((k_accumulate_px<T, MM>), grid, dim3(64), 0, st, a);
 }
 
 template <int T>
