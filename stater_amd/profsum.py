@@ -34,6 +34,7 @@ TIMER_NAMES = [("k_welch", "welch_filter"), ("k_welch_decide", "welch_filter"), 
                ("k_welch_accum2", "welch_filter"), ("k_welch_accum", "welch_filter"),
                ("k_var_filter", "var_filter"), ("k_var_pack", "var_filter"),
                ("k_accumulate_merge", "accumulate_merge64"), ("k_accumulate", "accumulate"),
+               ("k_accumulate_px", "accumulate"), ("k_film_merge", "accumulate"), ("k_film_resolve", "film_resolve"),
                ("k_prepass", "prepass")]
 
 
