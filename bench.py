@@ -122,8 +122,9 @@ def bench_c4(args) -> int:
     """48 uneven images: the plain path (one image per launch, images one after
     another on one stream) against the batch (the same launches spread over 4
     streams, tile heights tuned per image, one hipGraph).  Both generate their
-    samples on the device in chunks inside the timed region and end with
-    bit-identical buffers."""
+    samples on the device in chunks inside the timed region, run the
+    renderer's loop (guarded samples, pbrt Film resolved and filtered) and end
+    with bit-identical buffers."""
     import stater_amd as sa
     from stater_amd.serve import job_lambdas
     from stater_amd.workload import FEATURE_SDS, Features, SampleChunk, make_scene, synth_features, synth_samples
@@ -136,7 +137,7 @@ def bench_c4(args) -> int:
         sc = make_scene(2, 1000 + i, W, H, image_id=i)
         feats = Features(W, H)
         synth_features(ctx, sc, feats)
-        bs = sa.BufferSet(ctx, W, H, job_lambdas(T), 5)
+        bs = sa.BufferSet(ctx, W, H, job_lambdas(T), 5, flags=sa.STATER_BUFSET_FILM)
         imgs.append((bs, sc, feats, feats.gbuffers(FEATURE_SDS), spp, T))
     P = sum(im[0].width * im[0].height for im in imgs)
     samples = sum(im[0].width * im[0].height * im[4] for im in imgs)
@@ -152,9 +153,10 @@ def bench_c4(args) -> int:
             for off in range(0, spp, chunk_spp):
                 k = min(chunk_spp, spp - off)
                 synth_samples(ctx, sc, ch, off, feats, spp=k)
-                bs.accumulate(ch.desc(spp=k))
+                bs.accumulate(ch.desc(spp=k, sanitize=True))
             bs.denoise_vars(p, g)
-            bs.denoise_film(p, g)
+            bs.film_resolve()
+            bs.denoise_film(p, g, film=bs.view("film"))
 
     def timed(fn, reps):
         fn()
@@ -182,7 +184,7 @@ def bench_c4(args) -> int:
         "gsample_s_batched": round(samples / t_batch / 1e9, 3), "mpixel": round(P / 1e6, 3),
         "gsamples": round(samples / 1e9, 3), "dtype": "f32", "data": "synthetic, device-generated in the timed region",
         "config": {"workload": "48 images W~U[256,2048] H~U[256,1152] spp~LogU[16,1024] T~U{1..5}, r=%d, "
-                               "chunk %d spp" % (args.radius, chunk_spp)},
+                               "chunk %d spp, sanitised samples + Film" % (args.radius, chunk_spp)},
         "tile_rows": [(f["tile_rows_var"], f["tile_rows_welch"]) for f in batch.images()],
     }
     print(json.dumps(out), flush=True)
@@ -224,7 +226,8 @@ def bench_c5(args) -> int:
         "speedup_vs_one_at_a_time": round(sat["mpixel_s"] / plain["mpixel_s"], 3),
         "saturated": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items()},
         "sweep": sweep, "dtype": "f32", "data": "synthetic, device-generated per job",
-        "config": {"workload": "Poisson arrivals, jobs %d per rate, admission budget %d GiB" % (args.jobs, budget >> 30)},
+        "config": {"workload": "Poisson arrivals, jobs %d per rate, admission budget %d GiB, sanitised samples + Film"
+                               % (args.jobs, budget >> 30)},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -323,7 +323,10 @@ int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
  * launches fill the device beside large films' tails.  use_graph captures the
  * whole fan-out/fan-in once into a hipGraph and replays it.  The buffers each
  * film ends with are bit-identical to running it on its own
- * (stater_accumulate + stater_denoise_vars + stater_denoise_film).
+ * (stater_accumulate + stater_denoise_vars + stater_denoise_film).  A set
+ * created with STATER_BUFSET_FILM runs the renderer's loop: guarded samples
+ * (sanitize = 1), stater_film_resolve(scale 1) after the last pass, and the
+ * resolved Film as the Welch filter's film (statpath.cpp:313-336, 411, 428-432).
  * Replaces nothing in the reference (it renders one film per process). */
 typedef struct stater_batch stater_batch;
 typedef struct {
@@ -381,7 +384,10 @@ typedef struct {
     uint32_t seed, image_id;
     int32_t  feature_dtype;   /* STATER_F32 or STATER_BF16 storage of the filters' features */
     int32_t  keep;            /* 1: keep the buffers after completion (stater_server_view) */
-    int32_t  reserved[4];
+    int32_t  film;            /* 1: the renderer's loop: guarded samples, the set's pbrt Film
+                                 (STATER_BUFSET_FILM) resolved and used as the Welch
+                                 filter's film; 0: the radiance mean is the film */
+    int32_t  reserved[3];
 } stater_job;
 
 typedef struct {

@@ -71,7 +71,8 @@ class Job(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("n_lambdas", C.c_int32),
                 ("lambdas", C.c_float * MAX_LAMBDAS), ("spp", C.c_int32), ("rounds", C.c_int32),
                 ("cap_spp", C.c_uint32), ("scene_model", C.c_int32), ("seed", C.c_uint32), ("image_id", C.c_uint32),
-                ("feature_dtype", C.c_int32), ("keep", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("feature_dtype", C.c_int32), ("keep", C.c_int32), ("film", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class JobResult(C.Structure):

@@ -10,7 +10,11 @@
 // (2, 4 and 8 rows) on every film and keeps the fastest for it.
 //
 // Each film runs the same kernels in the same order as on its own, so a batch
-// leaves bit-identical buffers.
+// leaves bit-identical buffers.  A set created with STATER_BUFSET_FILM runs
+// the renderer's loop (src/statistics/statpath.cpp:313-336, 388, 411,
+// 428-432): samples pass the NaN / negative / infinity guard, the pbrt Film is
+// built beside the statistics, resolved after the last pass, and is the Welch
+// filter's film.
 
 #include <algorithm>
 #include <cstring>
@@ -39,6 +43,8 @@ struct stater_batch {
         bool has_alb;
         stater_gbuffers g;
         bool has_g;
+        stater_plane film;  // resolved pbrt Film (STATER_BUFSET_FILM sets)
+        bool has_film;
         int spp;
         int branch;
         int ty_var, ty_welch;
@@ -63,10 +69,16 @@ stater_samples chunk_desc(const stater_batch *b, const stater_batch::Img &I, int
     s.comp_stride_bytes = (int64_t)I.bs->plane * 4;
     s.sample_stride_bytes = 5 * s.comp_stride_bytes;
     s.spp = spp;
+    s.sanitize = I.has_film ? 1 : 0;
     return s;
 }
 
 const stater_gbuffers *gb(const stater_batch::Img &I) { return I.has_g ? &I.g : nullptr; }
+const stater_plane *film(const stater_batch::Img &I) { return I.has_film ? &I.film : nullptr; }
+
+int welch(const stater_batch *b, const stater_batch::Img &I, hipStream_t st, int ty) {
+    return launch_welch_filter(I.bs, &b->p, gb(I), film(I), nullptr, false, st, ty);
+}
 
 // One film's chain on stream st.
 int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t st) {
@@ -78,7 +90,8 @@ int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t 
     }
     if (!rc) rc = launch_prepass(I.bs, b->p.var_z, st);
     if (!rc) rc = launch_var_filter(I.bs, &b->p, gb(I), st, I.ty_var);
-    if (!rc) rc = launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, I.ty_welch);
+    if (!rc && I.has_film) rc = launch_film_resolve(I.bs, 1.f, &I.film, st);
+    if (!rc) rc = welch(b, I, st, I.ty_welch);
     return rc;
 }
 
@@ -150,6 +163,8 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
         I.has_g = im.gbufs != nullptr;
         if (I.has_g) I.g = *im.gbufs;
         I.spp = im.spp;
+        I.has_film = im.set->film_xyzw != nullptr;
+        if (I.has_film) (void)stater_bufset_view(im.set, "film", &I.film);
         I.ty_var = kVarTileRows;
         I.ty_welch = 4;
         // work estimate: samples x sets (generation + accumulation) + window tests
@@ -255,8 +270,7 @@ int stater_batch_tune(stater_batch *b, void *stream) {
         for (int ty : kWelchRowOptions) {
             if (rc || b->p.mt_mode != 2) break;
             float ms = 0.f;
-            rc = time_one([&] { return launch_welch_filter(I.bs, &b->p, gb(I), nullptr, nullptr, false, st, ty); },
-                          &ms);
+            rc = time_one([&] { return welch(b, I, st, ty); }, &ms);
             if (rc) break;
             if (ms < best_w) { best_w = ms; I.ty_welch = ty; }
         }

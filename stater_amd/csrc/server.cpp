@@ -93,7 +93,8 @@ struct JobLayout {
 };
 
 int job_layout(const stater_job &j, JobLayout &L) {
-    const int64_t sb = stater_bufset_bytes_for(j.width, j.height, j.n_lambdas, j.lambdas, 5, 0);
+    const int64_t sb =
+        stater_bufset_bytes_for(j.width, j.height, j.n_lambdas, j.lambdas, 5, j.film ? STATER_BUFSET_FILM : 0u);
     if (sb < 0) return (int)STATER_E_ARG;
     L.ld = (j.width + 63) / 64 * 64;
     const size_t plane = (size_t)L.ld * j.height;
@@ -163,7 +164,8 @@ int run_job(stater_server *s, JobRec *j, stater_server::Slot &slot) {
     JobLayout L;
     JOB_CALL(job_layout(J, L));
     char *m = s->mem + j->off;
-    JOB_CALL(stater_bufset_create_in(s->ctx, W, H, J.n_lambdas, J.lambdas, 5, 0, m, L.set_bytes, st, &j->bs));
+    JOB_CALL(stater_bufset_create_in(s->ctx, W, H, J.n_lambdas, J.lambdas, 5, J.film ? STATER_BUFSET_FILM : 0u, m,
+                                     L.set_bytes, st, &j->bs));
     stater_bufset *bs = j->bs;
     const size_t plane = (size_t)L.ld * H;
     float *feat = (float *)(m + L.feat_off);
@@ -200,6 +202,7 @@ int run_job(stater_server *s, JobRec *j, stater_server::Slot &slot) {
     d.pitch_bytes = (int64_t)L.ld * 4;
     d.comp_stride_bytes = (int64_t)plane * 4;
     d.sample_stride_bytes = 5 * d.comp_stride_bytes;
+    d.sanitize = J.film ? 1 : 0;
     const int chunk = s->cfg.chunk_spp;
     const stater_denoise_params &p = s->cfg.params;
     for (int r = 0; r < J.rounds; ++r) {
@@ -234,7 +237,12 @@ int run_job(stater_server *s, JobRec *j, stater_server::Slot &slot) {
         }
     }
     JOB_CALL(stater_denoise_vars(bs, &p, gp, st));
-    JOB_CALL(stater_denoise_film(bs, &p, gp, nullptr, nullptr, st));
+    stater_plane film;
+    if (J.film) {
+        JOB_CALL(stater_film_resolve(bs, 1.f, nullptr, st));
+        JOB_CALL(stater_bufset_view(bs, "film", &film));
+    }
+    JOB_CALL(stater_denoise_film(bs, &p, gp, J.film ? &film : nullptr, nullptr, st));
     STATER_HIP(hipStreamSynchronize(st));
     return STATER_OK;
 }
@@ -357,8 +365,8 @@ int64_t stater_server_job_bytes(const stater_server *s, const stater_job *job) {
     if (job->width > s->cfg.max_width || job->height > s->cfg.max_height)
         return fail(STATER_E_UNSUPPORTED, "stater_server: film larger than the server's maximum");
     if (job->spp < 1 || job->rounds < 1 || job->rounds > 64 ||
-        (job->feature_dtype != STATER_F32 && job->feature_dtype != STATER_BF16))
-        return fail(STATER_E_ARG, "stater_server: job needs spp >= 1, 1..64 rounds and f32/bf16 features");
+        (job->feature_dtype != STATER_F32 && job->feature_dtype != STATER_BF16) || (job->film & ~1))
+        return fail(STATER_E_ARG, "stater_server: job needs spp >= 1, 1..64 rounds, f32/bf16 features, film 0/1");
     JobLayout L;
     if (job_layout(*job, L)) return STATER_E_ARG;
     return (int64_t)round_up(L.total, kBlock);

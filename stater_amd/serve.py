@@ -47,7 +47,8 @@ class Server:
 
     @staticmethod
     def job(width: int, height: int, lambdas=MT5_LAMBDAS, spp: int = 64, rounds: int = 1, cap: int = 4096,
-            model: int = 2, seed: int = 5, image_id: int = 0, bf16: bool = False, keep: bool = False) -> Job:
+            model: int = 2, seed: int = 5, image_id: int = 0, bf16: bool = False, keep: bool = False,
+            film: bool = False) -> Job:
         j = Job()
         j.width, j.height, j.n_lambdas = width, height, len(lambdas)
         assert len(lambdas) <= MAX_LAMBDAS
@@ -57,6 +58,7 @@ class Server:
         j.seed, j.image_id = seed, image_id
         j.feature_dtype = STATER_BF16 if bf16 else STATER_F32
         j.keep = 1 if keep else 0
+        j.film = 1 if film else 0
         return j
 
     def job_bytes(self, job: Job) -> int:
@@ -110,6 +112,7 @@ class Mix:
     spps: tuple = (64, 256, 1024)
     rounds: int = 1
     bf16: bool = False
+    film: bool = True  # the renderer's loop: guarded samples, pbrt Film into the Welch filter
 
     def draw(self, rng: np.random.Generator, image_id: int, seed: int) -> Job:
         w = np.array([s[2] for s in self.sizes])
@@ -117,7 +120,7 @@ class Mix:
         T = int(rng.choice(self.ts))
         spp = int(rng.choice(self.spps))
         return Server.job(W, H, job_lambdas(T), spp=spp, rounds=self.rounds, seed=seed, image_id=image_id,
-                          bf16=self.bf16)
+                          bf16=self.bf16, film=self.film)
 
 
 def run_poisson(server: Server, rate: float, n_jobs: int, mix: Mix, seed: int = 5,

@@ -72,3 +72,44 @@ def test_batch_matches_single_images(ctx):
             _same(got[i], _outputs(bs, _image_lambdas(T)))
     print(batch.images())
     batch.close()
+
+
+def test_batch_of_film_sets_runs_the_renderer_loop(ctx):
+    """STATER_BUFSET_FILM sets: guarded samples, the resolved pbrt Film as the
+    Welch filter's film; the same buffers as the single-image calls."""
+    p = sa.default_params(radius=5)
+    shapes = SHAPES[:3]
+    imgs, keep = [], []
+    for i, (W, H, T, spp) in enumerate(shapes):
+        sc = make_scene(2, 2000 + i, W, H, image_id=i)
+        feats = Features(W, H)
+        synth_features(ctx, sc, feats)
+        bs = sa.BufferSet(ctx, W, H, _image_lambdas(T), 5, flags=sa.STATER_BUFSET_FILM)
+        imgs.append((bs, sc, feats.albedo, feats.gbuffers(FEATURE_SDS), spp))
+        keep.append(feats)
+    batch = sa.Batch(ctx, imgs, p, chunk_spp=16)
+    batch.run(use_graph=True)
+    ctx.synchronize()
+    for (bs, sc, alb, g, spp), feats, (W, H, T, _) in zip(imgs, keep, shapes):
+        lam = _image_lambdas(T)
+        got = _outputs(bs, lam)
+        got["film"] = hipmem.read_plane(bs.view("film"))
+        solo = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM)
+        # the Film merges each accumulate call's sums (one pbrt tile merge per
+        # call), so the single-image calls take the batch's 16-sample passes
+        chunk = SampleChunk(W, H, 16)
+        for off in range(0, spp, 16):
+            k = min(16, spp - off)
+            synth_samples(ctx, sc, chunk, off, feats, spp=k)
+            solo.accumulate(chunk.desc(spp=k, sanitize=True))
+        solo.denoise_vars(p, g)
+        solo.film_resolve()
+        solo.denoise_film(p, g, film=solo.view("film"))
+        ctx.synchronize()
+        want = _outputs(solo, lam)
+        want["film"] = hipmem.read_plane(solo.view("film"))
+        _same(got, want)
+        # the Welch filter saw the Film, not the radiance mean
+        assert not np.array_equal(want["film"], want["stats"][0, 0])
+        solo.close()
+    batch.close()

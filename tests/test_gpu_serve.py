@@ -77,6 +77,37 @@ def test_job_alone_coscheduled_and_single_image_path_agree(ctx):
     bs.close()
 
 
+def test_film_job_runs_the_renderer_loop(ctx):
+    """film=1: guarded samples, the set's pbrt Film resolved and used as the
+    Welch filter's film, as the single-image calls do."""
+    srv = _server(ctx, slots=2)
+    jid = srv.submit(Server.job(**JOB_X, keep=True, film=True))
+    other = srv.submit(Server.job(96, 64, job_lambdas(1), spp=40, rounds=2, cap=200, seed=5, image_id=2, film=True))
+    r = srv.wait(jid)
+    assert r["status"] == 0 and srv.wait(other)["status"] == 0
+    got = _read(srv, jid, OUT[:-1] + ("film",))
+    srv.release(jid)
+    srv.close()
+
+    W, H, lam = JOB_X["width"], JOB_X["height"], JOB_X["lambdas"]
+    sc = make_scene(2, 5, W, H, image_id=7)
+    feats = Features(W, H)
+    synth_features(ctx, sc, feats)
+    bs = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM)
+    chunk = SampleChunk(W, H, 16)
+    for off in range(0, 48, 16):  # the server's 16-sample passes: one Film merge each
+        synth_samples(ctx, sc, chunk, off, feats)
+        bs.accumulate(chunk.desc(sanitize=True))
+    p = sa.default_params(radius=6)
+    g = feats.gbuffers(FEATURE_SDS)
+    bs.denoise_vars(p, g)
+    bs.film_resolve()
+    bs.denoise_film(p, g, film=bs.view("film"))
+    ctx.synchronize()
+    _same(got, {n: hipmem.read_plane(bs.view(n)) for n in got})
+    bs.close()
+
+
 def test_adaptive_rounds_match_the_round_driver(ctx):
     W, H = 96, 64
     srv = _server(ctx, slots=2)
