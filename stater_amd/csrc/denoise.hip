@@ -240,32 +240,36 @@ struct VarPix {
     float gp[kNF];
 };
 
+// The pixel's own statistics come from its record (all items issued at
+// once, one memory latency per tile instead of one per dependent load); an
+// inactive lane reads the zero record.  The record clamps a non-finite s^2 to
+// FLT_MAX for its neighbours' sums; the pixel's own sum starts from the exact
+// value, reloaded in that (rare) case.
 template <int G, bool HASW>
 __device__ __forceinline__ void var_pix_init(const VarArgs &a, int x, int y, bool active, VarPix<G> &P) {
-#pragma unroll
-    for (int k = 0; k < kNF; ++k) P.gp[k] = 0.f;
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            P.Lp[g][c] = P.Zp[g][c] = P.num[g][c] = P.numC[g][c] = 0.f;
-            P.den[g][c] = 1.f;  // the pixel itself, weight 1
-        }
-    if (!active) return;
+    using R = VarRec<G, HASW>;
     const size_t pix = (size_t)y * a.ld + x;
-    const int n = a.n[pix];
-    if (HASW)
+    const float4 *q = active ? a.rec + pix * R::FLOAT4S : a.zero;
+    float4 v[R::ITEMS];
 #pragma unroll
-        for (int k = 0; k < kNF; ++k) P.gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+    for (int i = 0; i < R::ITEMS; ++i) v[i] = q[i];
+    const float gp[kNF] = {v[0].y, v[0].z, v[0].w, v[HASW ? 1 : 0].x, v[HASW ? 1 : 0].y, v[HASW ? 1 : 0].z,
+                           v[HASW ? 1 : 0].w};
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) P.gp[k] = HASW ? gp[k] : 0.f;
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-            P.Lp[g][c] = n >= 4 ? a.lnS2[o] : 0.f;
-            P.Zp[g][c] = n >= 4 ? a.z2 * a.se2[o] : 0.f;
-            if (n >= 2) {
-                const float inv = 1.f / (float)(n - 1);
+            const float4 it = v[R::SINGLE4 + g * 3 + c];
+            P.Lp[g][c] = it.x;
+            P.Zp[g][c] = it.y;
+            P.num[g][c] = it.z;
+            P.numC[g][c] = it.w;
+            P.den[g][c] = 1.f;  // the pixel itself, weight 1
+            if (it.z == FLT_MAX || it.w == FLT_MAX) {
+                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+                const float inv = 1.f / (float)(a.n[pix] - 1);
                 P.num[g][c] = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
                 P.numC[g][c] = a.m2C[o] * inv;
             }
@@ -1253,27 +1257,29 @@ struct AccPix {
     uint32_t own[3][kBitsWords];
 };
 
+// The pixel's own film, features and decision words come from its record in
+// one batch of loads; an inactive lane reads the zero record.
 template <bool HASW>
 __device__ __forceinline__ void acc_pix_init(const WelchArgs &a, int x, int y, bool active, AccPix<HASW> &P) {
+    using R = AccRec<HASW>;
+    const float4 *q = active ? a.rec + ((size_t)y * a.ld + x) * R::FLOAT4S : a.zero;
+    float4 h[R::HEAD4];
+    uint4 wd[kBitsWords];
 #pragma unroll
-    for (int k = 0; k < kNF; ++k) P.gp[k] = 0.f;
+    for (int i = 0; i < R::HEAD4; ++i) h[i] = q[i];
+#pragma unroll
+    for (int w = 0; w < kBitsWords; ++w) wd[w] = ((const uint4 *)q)[R::HEAD4 + w];
+    const float gp[kNF] = {h[0].w, h[HASW ? 1 : 0].x, h[HASW ? 1 : 0].y, h[HASW ? 1 : 0].z,
+                           h[HASW ? 1 : 0].w, h[HASW ? 2 : 0].x, h[HASW ? 2 : 0].y};
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) P.gp[k] = HASW ? gp[k] : 0.f;
+    const float film[3] = {h[0].x, h[0].y, h[0].z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         P.den[c] = 1.f;  // the pixel itself, weight 1
-        P.num[c] = 0.f;
+        P.num[c] = film[c];
 #pragma unroll
-        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = 0u;
-    }
-    if (!active) return;
-    const size_t pix = (size_t)y * a.ld + x;
-    if (HASW)
-#pragma unroll
-        for (int k = 0; k < kNF; ++k) P.gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        P.num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
-#pragma unroll
-        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = a.bits[((size_t)c * kBitsWords + w) * a.plane + pix];
+        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = c == 0 ? wd[w].x : c == 1 ? wd[w].y : wd[w].z;
     }
 }
 
