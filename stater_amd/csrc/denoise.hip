@@ -156,13 +156,12 @@ struct VarRec {
     static_assert(FLOAT4S * 4 <= kVarRecFloats, "record larger than its planes");
 };
 
-// Ring slot: the halo row's HWC records plus two padding columns for the
-// pair loop's even column count, rounded up to whole 64-item loads (the
-// loader fills everything past the halo with zeros).  The ring holds the
+// Ring slot: the halo row's HWC records, rounded up to whole 64-item loads
+// (the loader fills everything past the halo with zeros).  The ring holds the
 // 2 TY - 1 live rows and kVarAhead rows in flight.
 constexpr int kVarAhead = 2;
 template <int G, bool HASW, int HWC>
-constexpr int var_loads_per_row() { return ((HWC + 2) * VarRec<G, HASW>::FLOAT4S + 63) / 64; }
+constexpr int var_loads_per_row() { return (HWC * VarRec<G, HASW>::FLOAT4S + 63) / 64; }
 template <int G, bool HASW, int HWC>
 constexpr int var_slot_float4s() { return var_loads_per_row<G, HASW, HWC>() * 64; }
 template <int TY>
@@ -320,20 +319,19 @@ __device__ __forceinline__ void var_test(float Lp, float Zp, float4 q, float w, 
 }
 
 // The pairs of neighbour column dx with A and B: both weights, then every
-// set-channel's log-variance test for A and for B side by side.  A pixel
-// whose window does not reach this row (offA/offB), and the pixel itself
-// (dx = 0 on its own row), get weight exactly 0, which leaves every sum as it
-// is.  lw0 = sp_scale dy^2, or -inf for weight 0; m (unweighted filter) is 1
-// or 0.
+// set-channel's log-variance test for A and for B side by side.  A row
+// outside a pixel's window comes with lw = -inf (weighted) or m = 0
+// (unweighted), so its weight is exactly 0 and every sum stays as it is; the
+// pixel itself (dx = 0 on its own row) gets weight 0 the same way.
 template <int G, bool HASW>
-__device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, HASW> &c, int dx, bool offA, bool offB,
-                                             bool selfA, bool selfB, float lwA, float lwB, VarPix<G> &A,
+__device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, HASW> &c, int dx, bool selfA,
+                                             bool selfB, float lwA, float lwB, float mA, float mB, VarPix<G> &A,
                                              VarPix<G> &B) {
     const float fq[kNF] = {c.s0.y, c.s0.z, c.s0.w, c.s1.x, c.s1.y, c.s1.z, c.s1.w};
     const float dx2 = (float)(dx * dx);
-    const bool zA = offA || (selfA && dx == 0), zB = offB || (selfB && dx == 0);
-    const float wA = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : 1.f, dx2);
-    const float wB = var_weight<HASW>(a.sp_scale, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : 1.f, dx2);
+    const bool zA = selfA && dx == 0, zB = selfB && dx == 0;
+    const float wA = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : mA, dx2);
+    const float wB = var_weight<HASW>(a.sp_scale, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : mB, dx2);
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
@@ -343,32 +341,39 @@ __device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, H
         }
 }
 
-// Step k of the tile: the staged row at dy = k - r from A (k <= 2r) and
-// k - r - 1 from B (k >= 1).  The LDS reads of column dx + 1 are issued
-// before the tests of column dx, in two register sets that alternate.  The
-// loop runs an even number of columns: the 2r + 2nd (dx = r + 1) is the
-// slot's zeroed padding column, tested with weight 0; the reads of the last
-// iteration's next column land in the second padding column and are unused.
+// The 2r + 1 columns of one staged row: the LDS reads of column dx + 1 are
+// issued before the tests of column dx, in two register sets that
+// alternate; pairs of columns from dx = -r, then the last column alone.
 template <int G, bool HASW>
-__device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, int tx, int k, VarPix<G> &A,
-                                        VarPix<G> &B) {
+__device__ __forceinline__ void var_cols(const VarArgs &a, const float4 *slot, int tx, bool selfA, bool selfB,
+                                         float lwA, float lwB, float mA, float mB, VarPix<G> &A, VarPix<G> &B) {
     constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
-    const int dyA = k - a.r, dyB = k - a.r - 1;
-    const bool offA = k > 2 * a.r, offB = k < 1;  // row outside the pixel's window
-    const float lwA = a.sp_scale * (float)(dyA * dyA), lwB = a.sp_scale * (float)(dyB * dyB);
-    const bool selfA = dyA == 0, selfB = dyB == 0;
     const float4 *q = slot + tx * F4;  // column of dx = -r
     VarCol<G, HASW> X, Y;
     var_col_load<G, HASW>(q, X);
-    for (int dx = -a.r; dx <= a.r; dx += 2, q += 2 * F4) {
+    for (int dx = -a.r; dx < a.r; dx += 2, q += 2 * F4) {
         var_col_load<G, HASW>(q + F4, Y);
         __builtin_amdgcn_sched_barrier(0);
-        var_col_test<G, HASW>(a, X, dx, offA, offB, selfA, selfB, lwA, lwB, A, B);
+        var_col_test<G, HASW>(a, X, dx, selfA, selfB, lwA, lwB, mA, mB, A, B);
         var_col_load<G, HASW>(q + 2 * F4, X);
         __builtin_amdgcn_sched_barrier(0);
-        const bool pad = dx + 1 > a.r;
-        var_col_test<G, HASW>(a, Y, dx + 1, offA || pad, offB || pad, selfA, selfB, lwA, lwB, A, B);
+        var_col_test<G, HASW>(a, Y, dx + 1, selfA, selfB, lwA, lwB, mA, mB, A, B);
     }
+    var_col_test<G, HASW>(a, X, a.r, selfA, selfB, lwA, lwB, mA, mB, A, B);
+}
+
+// Step k of the tile: the staged row at dy = k - r from A (k <= 2r) and
+// k - r - 1 from B (k >= 1).
+template <int G, bool HASW>
+__device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, int tx, int k, VarPix<G> &A,
+                                        VarPix<G> &B) {
+    const int dyA = k - a.r, dyB = k - a.r - 1;
+    const bool offA = k > 2 * a.r, offB = k < 1;  // row outside the pixel's window
+    const float lwA = offA ? -INFINITY : a.sp_scale * (float)(dyA * dyA);
+    const float lwB = offB ? -INFINITY : a.sp_scale * (float)(dyB * dyB);
+    const float mA = offA ? 0.f : 1.f, mB = offB ? 0.f : 1.f;
+    const bool selfA = dyA == 0, selfB = dyB == 0;
+    var_cols<G, HASW>(a, slot, tx, selfA, selfB, lwA, lwB, mA, mB, A, B);
 }
 
 // A pixel with fewer than 4 samples is not filtered: it keeps its own
