@@ -1209,7 +1209,7 @@ struct AccRec {
     static_assert(FLOAT4S * 4 <= kVarRecFloats, "record larger than its planes");
 };
 template <bool HASW, int HWC>
-constexpr int acc_loads_per_row() { return ((HWC + 2) * AccRec<HASW>::FLOAT4S + 63) / 64; }
+constexpr int acc_loads_per_row() { return (HWC * AccRec<HASW>::FLOAT4S + 63) / 64; }
 template <bool HASW, int HWC, int TY>
 constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<HASW, HWC>() * 64 * 16; }
 
@@ -1341,7 +1341,7 @@ template <int M>
 __device__ __forceinline__ void acc_qsel(const AccRowP &R, int dx, int r, int &wq, int &sh) {
     if (M == kAccBwd) {
         const int idx = R.i0 - (dx + r);
-        wq = max(idx >> 5, 0);  // the prefetch past the padding column may go below index 0
+        wq = idx >> 5;  // idx >= i0 - 2r >= r
         sh = idx & 31;
     } else if (M == kAccGen) {
         const int idx = dx < 0 ? -dx - 1 : 0;  // centre row: backward offsets dx < 0
@@ -1370,13 +1370,13 @@ __device__ __forceinline__ void acc_col_load(const float4 *q, const AccRowP &RA,
 }
 
 // The weight and the three channel decisions of column dx (j = dx + r) for
-// one pixel in mode M; the padding column and the pixel itself weigh 0.
+// one pixel in mode M; the pixel itself weighs 0.
 template <bool HASW, int M>
 __device__ __forceinline__ void acc_pix_col(const WelchArgs &a, const AccCol<HASW> &c, const float (&fq)[kNF],
                                             const float (&film)[3], const uint4 &qw, int sh, int dx, const AccRowP &R,
                                             AccPix<HASW> &P) {
     if (M == kAccOff) return;
-    const bool z = dx > a.r || (M == kAccGen && dx == 0);
+    const bool z = M == kAccGen && dx == 0;
     const float w = var_weight<HASW>(a.sp_scale, P.gp, 1.f, fq, z ? -INFINITY : R.lw, z ? 0.f : 1.f, (float)(dx * dx));
     const uint32_t q3[3] = {qw.x, qw.y, qw.z};
     const int j = dx + a.r;
@@ -1405,9 +1405,9 @@ __device__ __forceinline__ void acc_col_test(const WelchArgs &a, const AccCol<HA
     acc_pix_col<HASW, MB>(a, c, fq, film, c.wb, c.shb, dx, RB, B);
 }
 
-// One window row for A and B in modes MA, MB: 2r + 2 columns (the last one
-// the zeroed padding column, weight 0), the reads of column dx + 1 issued
-// before the weights of column dx.
+// One window row for A and B in modes MA, MB: the reads of column dx + 1
+// issued before the weights of column dx; pairs of columns from dx = -r, then
+// the last column alone.
 template <bool HASW, int MA, int MB>
 __device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, const AccRowP &RA, const AccRowP &RB,
                                          AccPix<HASW> &A, AccPix<HASW> &B) {
@@ -1415,7 +1415,7 @@ __device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, co
     const int r = a.r;
     AccCol<HASW> X, Y;
     acc_col_load<HASW, MA, MB>(q, RA, RB, -r, r, X);
-    for (int dx = -r; dx <= r; dx += 2, q += 2 * F4) {
+    for (int dx = -r; dx < r; dx += 2, q += 2 * F4) {
         acc_col_load<HASW, MA, MB>(q + F4, RA, RB, dx + 1, r, Y);
         __builtin_amdgcn_sched_barrier(0);
         acc_col_test<HASW, MA, MB>(a, X, dx, RA, RB, A, B);
@@ -1423,6 +1423,7 @@ __device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, co
         __builtin_amdgcn_sched_barrier(0);
         acc_col_test<HASW, MA, MB>(a, Y, dx + 1, RA, RB, A, B);
     }
+    acc_col_test<HASW, MA, MB>(a, X, r, RA, RB, A, B);
 }
 
 // Step k: the staged row at dy = k - r from A and k - r - 1 from B.
