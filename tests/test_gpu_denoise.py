@@ -197,6 +197,64 @@ def test_welch_filter_mt5(ctx, mt_mode, ragged):
     np.testing.assert_array_equal(discr, want)
 
 
+@pytest.mark.parametrize("W,H,r,weighted", [
+    (77, 45, 1, True),     # smallest window of the one-test-per-pair path, ragged tiles
+    (77, 45, 3, False),    # unweighted (no features, ds_factor 0)
+    (130, 37, 7, True),    # three 64-column tiles, the last one 2 wide
+    (8, 5, 10, True),      # image smaller than the window and than one tile
+    (33, 20, 12, True),    # r > 10: every ordered pair tested on its own
+    (70, 9, 0, True),      # r = 0: the film itself
+])
+def test_welch_filter_shapes(ctx, W, H, r, weighted):
+    """Window radii, ragged and tiny films, weighted and unweighted: the
+    decisions and filmD against the oracle."""
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=32, seed=5)
+    p = sa.default_params(radius=r, mt_mode=2)
+    g, feats_list = None, None
+    if weighted:
+        g = feats.gbuffers()
+        f = feats.to_numpy()
+        feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    else:
+        p.ds_factor = 0.0
+    bs.denoise_vars(p, g)
+    bs.denoise_film(p, g)
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    varCD = read_set_buf(bs, MT5_LAMBDAS, "varCD")
+    ref = oracle.denoise_film(n, st[:, 0].copy(), varCD, st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
+                              feats=feats_list, want_decisions=True)
+    dec = _decisions(ctx, bs, p, g, False, W, H, r)
+    diff = dec != ref["decisions"]
+    assert np.count_nonzero(diff) / dec.size <= DECISION_MISMATCH_MAX
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
+    assert np.count_nonzero(same) > 0.9 * same.size
+    filmD = hipmem.read_plane(bs.view("filmD"))
+    np.testing.assert_allclose(filmD[same], ref["filmD"][same], rtol=FILTER_RTOL)
+    if r == 0:
+        assert np.array_equal(filmD, st[0, 0])
+
+
+@pytest.mark.parametrize("W,H,r", [(77, 45, 1), (8, 5, 10), (130, 37, 20)])
+def test_variance_filter_shapes(ctx, W, H, r):
+    """Ragged and tiny films and the largest radius: varD and varCD against the oracle."""
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=32, seed=6)
+    p = sa.default_params(radius=r)
+    bs.denoise_vars(p, feats.gbuffers())
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    lnS2, se2, m2C = (read_set_buf(bs, MT5_LAMBDAS, k) for k in ("lnS2", "se2", "m2C"))
+    f = feats.to_numpy()
+    feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    vD, vCD, amb = oracle.denoise_vars(n, st[:, 1], lnS2, se2, m2C, r, p.ds_factor, p.var_z, feats_list,
+                                       want_ambiguous=True)
+    gD, gCD = read_set_buf(bs, MT5_LAMBDAS, "varD"), read_set_buf(bs, MT5_LAMBDAS, "varCD")
+    for g, v in ((gD, vD), (gCD, vCD)):
+        ok = np.isfinite(v) & (np.abs(v) < 1e37) & (amb == 0)
+        assert np.count_nonzero(ok) > 0.9 * ok.size
+        assert np.max(np.abs(g[ok] - v[ok]) / np.maximum(np.abs(v[ok]), 1e-30)) < FILTER_RTOL
+
+
 def test_mean_vars_bitexact(ctx):
     W, H = 50, 30
     sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=16)
