@@ -57,7 +57,6 @@ def kernel_work(W, H, spp, S, r):
         "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) * 2, "bound": "valu"},
-        "prepass": {"bytes": P * (4 + S * 5 * 3 * 4 + S * (9 * 4 + 4)), "bound": "hbm"},
     }
 
 

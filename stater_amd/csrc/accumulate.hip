@@ -131,9 +131,11 @@ __device__ __forceinline__ NFac nfac(float nf) {
 // The reference recurrence in factored form on a shifted value t = y - K
 // (st[0] holds the shifted mean), grouped by powers of d:
 //     m3 += d (d^2 a2 - 3 b m2),   m4 += d (-4 b m3 + d (d^2 a3 - 6 b^2 m2))
-// with the updated m2 and m3, as in the reference.
+// with the updated m2 and m3, as in the reference.  The c2X, c2Y terms
+// (t - mean') xD use t - mean' = d (1 - b): xa = xD (1 - b) and ya = yD (1 - b)
+// are formed once per sample for all sets.
 template <int MM>
-__device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f, float xD, float yD) {
+__device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f, float xa, float ya) {
     float &mean = st[0];
     const float d = t - mean;
     mean = __builtin_fmaf(d, f.b, mean);
@@ -149,9 +151,8 @@ __device__ __forceinline__ void meng_factored(float *st, float t, const NFac &f,
     const float u = __builtin_fmaf(d2, f.a3, f.b6 * m2);
     m4 = __builtin_fmaf(d, __builtin_fmaf(d, u, f.b4 * m3), m4);
     if (MM == 4) return;
-    const float r = t - mean;
-    st[4] = __builtin_fmaf(r, xD, st[4]);
-    st[5] = __builtin_fmaf(r, yD, st[5]);
+    st[4] = __builtin_fmaf(d, xa, st[4]);
+    st[5] = __builtin_fmaf(d, ya, st[5]);
     st[6] = __builtin_fmaf(fabsf(d), f.sa1, st[6]);
 }
 
@@ -275,13 +276,14 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         }
         if (T > 0) {
             const NFac f = nfac(nf);
+            const float xa = xD * f.a1, ya = yD * f.a1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
                 const float dl = __builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c];
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
-                    meng_factored<MM>(st[c][s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[c][s]), f, xD, yD);
+                    meng_factored<MM>(st[c][s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[c][s]), f, xa, ya);
             }
         }
     }

@@ -357,8 +357,12 @@ int stater_bufset_create(stater_ctx *ctx, int width, int height, int n_lambdas, 
         (void)hipGetLastError();
         return fail(STATER_E_NOMEM, "stater_bufset_create: device allocation failed");
     }
-    e = hipMemset(mem, 0, L.bytes());
-    if (e != hipSuccess) { (void)hipFree(mem); return hip_fail(e, "hipMemset"); }
+    // cleared on the context's stream and waited for: that stream does not
+    // synchronise with the null stream, so a plain hipMemset could still be
+    // running when the first kernel of the set starts
+    e = hipMemsetAsync(mem, 0, L.bytes(), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { (void)hipFree(mem); return hip_fail(e, "hipMemsetAsync"); }
     *out = bufset_over(ctx, L, width, height, n_lambdas, lambdas, max_moment, flags, mem, true);
     return STATER_OK;
 }
@@ -518,10 +522,7 @@ int stater_denoise_vars(stater_bufset *bs, const stater_denoise_params *p, const
     int rc = check_params(bs, p, gbufs);
     if (rc) return rc;
     if (bs->mm < 4) return fail(STATER_E_STATE, "stater_denoise_vars: needs max_moment >= 4");
-    hipStream_t st = pick_stream(bs, stream);
-    rc = launch_prepass(bs, p->var_z, st);
-    if (rc) return rc;
-    return launch_var_filter(bs, p, gbufs, st);
+    return launch_var_filter(bs, p, gbufs, pick_stream(bs, stream));
 }
 
 static int check_film_plane(const stater_bufset *bs, const stater_plane *f, const char *what) {

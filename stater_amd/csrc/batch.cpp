@@ -1,7 +1,7 @@
 // batch.cpp — films of uneven size processed together.
 //
 // Every film's chain (reset; passes of chunk_spp generated samples, each
-// followed by accumulation; prepass; variance filter; Welch filter) runs as
+// followed by accumulation; prepass and variance filter; Welch filter) runs as
 // the single-film launches, and the chains are spread over a few HIP streams
 // (longest chain first, onto the least loaded stream), so the launches of
 // small films fill the device beside the large films' tails and nothing waits
@@ -88,7 +88,6 @@ int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t 
         rc = launch_synth_samples(b->ctx, &I.sc, I.has_alb ? &I.albedo : nullptr, (uint32_t)off, nullptr, &s, st);
         if (!rc) rc = launch_accumulate(I.bs, &s, STATER_ACC_SEQUENTIAL, st);
     }
-    if (!rc) rc = launch_prepass(I.bs, b->p.var_z, st);
     if (!rc) rc = launch_var_filter(I.bs, &b->p, gb(I), st, I.ty_var);
     if (!rc && I.has_film) rc = launch_film_resolve(I.bs, 1.f, &I.film, st);
     if (!rc) rc = welch(b, I, st, I.ty_welch);

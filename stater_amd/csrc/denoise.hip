@@ -61,63 +61,38 @@ __device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y)
 }
 
 // ================================================================ prepass
+// One pixel-channel of one set: lnS2, se2 and m2C, and the channel's
+// Jarque-Bera term (S^2 + (K - 3)^2 / 4; +inf when m2 or m4 is not finite,
+// 0 without a term).  A pixel with n < 4 gets zeros; its jb is 0.
+struct PreOut {
+    float lnS2, se2, m2C, jbt;
+};
 
-__device__ __forceinline__ void prepass_tile(const PreArgs a, int bx, int by) {
-    const int x = bx * 256 + threadIdx.x;
-    const int y = by;
-    if (x >= a.W) return;
-    const size_t pix = (size_t)y * a.ld + x;
-    const int n = a.n[pix];
+__device__ __forceinline__ PreOut prepass_one(int n, float m2, float m3, float m4, float c2x, float c2y, float z,
+                                              bool lev) {
+    PreOut o{0.f, 0.f, 0.f, 0.f};
+    if (n < 4) return o;
     const float nf = (float)n;
-    for (int s = 0; s < a.S; ++s) {
-        float jbs = 0.f;
-        for (int c = 0; c < 3; ++c) {
-            const size_t o = ((size_t)s * 3 + c) * a.plane + pix;
-            auto at = [&](int st) { return a.stats[(((size_t)s * N_STATS + st) * 3 + c) * a.plane + pix]; };
-            if (n < 4) { a.lnS2[o] = 0.f; a.se2[o] = 0.f; a.m2C[o] = 0.f; continue; }
-            const float m2 = at(ST_M2), m3 = at(ST_M3), m4 = at(ST_M4);
-            if (!(__builtin_isfinite(m2) && __builtin_isfinite(m4))) {
-                a.lnS2[o] = INFINITY; a.se2[o] = INFINITY; a.m2C[o] = m2; jbs = INFINITY;
-                continue;
-            }
-            if (m2 > 0.f) {
-                const float sm = sqrtf(m2);
-                const float S3 = sqrtf(nf) * (m3 / m2) / sm;       // (m3/n)/(m2/n)^1.5
-                const float K = nf * (m4 / m2) / m2;               // (m4/n)/(m2/n)^2
-                jbs += nf / 6.f * (S3 * S3 + 0.25f * (K - 3.f) * (K - 3.f));
-                const float cb = nf / (nf - a.z);
-                const float s2 = m2 / (nf - 1.f);
-                a.lnS2[o] = logf(fmaxf(cb * s2, 1e-30f));
-                a.se2[o] = cb * cb * (K - (nf - 3.f) / nf) / (nf - 1.f);
-                float m2c = m2;
-                if (a.lev) {
-                    const float cx = at(ST_C2X) / sm, cy = at(ST_C2Y) / sm;
-                    m2c = m2 * fmaxf(0.f, 1.f - 12.f * (cx * cx + cy * cy) / nf);
-                }
-                a.m2C[o] = m2c;
-            } else {
-                a.lnS2[o] = logf(1e-30f);
-                a.se2[o] = 0.f;
-                a.m2C[o] = 0.f;
-            }
+    if (!(__builtin_isfinite(m2) && __builtin_isfinite(m4))) return PreOut{INFINITY, INFINITY, m2, INFINITY};
+    if (m2 > 0.f) {
+        const float sm = sqrtf(m2);
+        const float S3 = sqrtf(nf) * (m3 / m2) / sm;  // (m3/n)/(m2/n)^1.5
+        const float K = nf * (m4 / m2) / m2;          // (m4/n)/(m2/n)^2
+        o.jbt = S3 * S3 + 0.25f * (K - 3.f) * (K - 3.f);
+        const float cb = nf / (nf - z);
+        const float s2 = m2 / (nf - 1.f);
+        o.lnS2 = logf(fmaxf(cb * s2, 1e-30f));
+        o.se2 = cb * cb * (K - (nf - 3.f) / nf) / (nf - 1.f);
+        float m2c = m2;
+        if (lev) {
+            const float cx = c2x / sm, cy = c2y / sm;
+            m2c = m2 * fmaxf(0.f, 1.f - 12.f * (cx * cx + cy * cy) / nf);
         }
-        a.jb[(size_t)s * a.plane + pix] = n < 4 ? 0.f : jbs / 3.f;
+        o.m2C = m2c;
+    } else {
+        o.lnS2 = logf(1e-30f);
     }
-}
-
-__global__ void __launch_bounds__(256) k_prepass(PreArgs a) { prepass_tile(a, blockIdx.x, blockIdx.y); }
-
-PreArgs make_pre_args(stater_bufset *bs, float var_z) {
-    return PreArgs{bs->n, bs->stats, bs->jb, bs->lnS2, bs->se2, bs->m2C, bs->S, bs->W, bs->H, bs->ld,
-                   bs->mm >= 5 ? 1 : 0, bs->plane, var_z};
-}
-
-int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st) {
-    const PreArgs a = make_pre_args(bs, var_z);
-    LaunchTimer t(bs->ctx, st, "prepass");
-    hipLaunchKernelGGL(k_prepass, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, a);
-    STATER_HIP(hipGetLastError());
-    return STATER_OK;
+    return o;
 }
 
 // ======================================================= variance filter
@@ -174,26 +149,27 @@ constexpr int waitcnt_vm(int v) { return (v & 15) | ((v >> 4) << 14) | (7 << 4) 
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
-// Pack pass.  A workgroup packs 64 pixels of a row: it first stages their
-// inputs plane by plane (coalesced 256-byte rows; one wavefront per plane),
-// then writes the records item by item (consecutive threads, consecutive
-// 16-byte items).  Staged planes: n, the 7 features, then (lnS2, se2, m2,
-// m2C) per set-channel of the group.
+// Pack pass, with the prepass of the group's sets.  A workgroup handles 64
+// pixels of a row: it stages their inputs plane by plane (coalesced 256-byte
+// rows; one wavefront per plane): n, the 7 features, then (m2, m3, m4, c2X,
+// c2Y) per set-channel of the group; computes the prepass per pixel and
+// set-channel (lnS2, se2, m2C and the Jarque-Bera term, written to their
+// planes), then jb per pixel and set (the terms of R, G, B in order), and
+// writes the records item by item (consecutive threads, consecutive 16-byte
+// items).
 template <int G>
-constexpr int var_pack_planes() { return 8 + 12 * G; }
+constexpr int var_pack_planes() { return 8 + 15 * G; }
 
 template <int G, bool HASW>
 __device__ __forceinline__ float var_pack_input(const VarArgs &a, int pl, int x, int y) {
     const size_t pix = (size_t)y * a.ld + x;
     if (pl == 0) return __int_as_float(a.n[pix]);
     if (pl < 8) return HASW && pl - 1 < a.f.nf ? feat_at(a.f, pl - 1, x, y) : 0.f;
-    const int sc = (pl - 8) >> 2, kind = (pl - 8) & 3;  // set-channel g*3 + c, statistic
+    const int sc = (pl - 8) / 5, kind = (pl - 8) - sc * 5;  // set-channel g*3 + c; m2, m3, m4, c2X, c2Y
     const int g = sc / 3, c = sc - g * 3;
-    const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-    if (kind == 0) return a.lnS2[o];
-    if (kind == 1) return a.se2[o];
-    if (kind == 2) return a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix];
-    return a.m2C[o];
+    if (kind >= 3 && !a.lev) return 0.f;
+    const int st = kind == 0 ? ST_M2 : kind == 1 ? ST_M3 : kind == 2 ? ST_M4 : kind == 3 ? ST_C2X : ST_C2Y;
+    return a.stats[(((size_t)(a.g0 + g) * N_STATS + st) * 3 + c) * a.plane + pix];
 }
 
 template <int G, bool HASW>
@@ -201,6 +177,7 @@ __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
     using R = VarRec<G, HASW>;
     constexpr int NPL = var_pack_planes<G>();
     __shared__ float in[NPL][64];
+    __shared__ float4 pre[3 * G][64];  // lnS2, se2, m2C, Jarque-Bera term
     const int y = blockIdx.y, x0 = blockIdx.x * 64;
     if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int i = threadIdx.x; i < NPL * 64; i += 256) {
@@ -209,7 +186,30 @@ __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
     }
     __syncthreads();
     const int npx = min(64, a.W - x0);
-    float4 *rec = a.rec + ((size_t)y * a.ld + x0) * R::FLOAT4S;
+    const size_t row = (size_t)y * a.ld + x0;
+    for (int i = threadIdx.x; i < 3 * G * 64; i += 256) {
+        const int sc = i >> 6, px = i & 63;
+        const float *v = &in[8 + 5 * sc][0];
+        const PreOut o = prepass_one(__float_as_int(in[0][px]), v[px], v[64 + px], v[128 + px], v[192 + px],
+                                     v[256 + px], a.z, a.lev != 0);
+        pre[sc][px] = make_float4(o.lnS2, o.se2, o.m2C, o.jbt);
+        if (px < npx) {
+            const size_t off = ((size_t)a.g0 * 3 + sc) * a.plane + row + px;
+            a.lnS2[off] = o.lnS2;
+            a.se2[off] = o.se2;
+            a.m2C[off] = o.m2C;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < G * 64; i += 256) {
+        const int g = i >> 6, px = i & 63;
+        const int n = __float_as_int(in[0][px]);
+        float jbs = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) jbs += (float)n / 6.f * pre[g * 3 + c][px].w;
+        if (px < npx) a.jb[(size_t)(a.g0 + g) * a.plane + row + px] = n < 4 ? 0.f : jbs / 3.f;
+    }
+    float4 *rec = a.rec + row * R::FLOAT4S;
     for (int j = threadIdx.x; j < npx * R::FLOAT4S; j += 256) {
         const int px = j / R::FLOAT4S, item = j - px * R::FLOAT4S;
         const int n = __float_as_int(in[0][px]);
@@ -220,12 +220,13 @@ __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
         } else if (HASW && item == 1) {
             v = make_float4(in[4][px], in[5][px], in[6][px], in[7][px]);
         } else if (item < R::ITEMS) {
-            const int b = 8 + 4 * (item - R::SINGLE4);
-            if (n >= 4) { v.x = in[b][px]; v.y = a.z2 * in[b + 1][px]; }
+            const int sc = item - R::SINGLE4;
+            const float4 o = pre[sc][px];
+            if (n >= 4) { v.x = o.x; v.y = a.z2 * o.y; }
             if (n >= 2) {
                 const float inv = 1.f / (float)(n - 1);
-                v.z = finite_or_max(in[b + 2][px] * inv);
-                v.w = finite_or_max(in[b + 3][px] * inv);
+                v.z = finite_or_max(in[8 + 5 * sc][px] * inv);
+                v.w = finite_or_max(o.z * inv);
             }
         }
         rec[j] = v;
@@ -522,13 +523,16 @@ VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const s
     VarArgs a{};
     a.n = bs->n;
     a.m2 = bs->stat(0, ST_M2, 0);
-    a.lnS2 = bs->lnS2; a.se2 = bs->se2; a.m2C = bs->m2C;
+    a.lnS2 = bs->lnS2; a.se2 = bs->se2; a.m2C = bs->m2C; a.jb = bs->jb;
+    a.stats = bs->stats;
     a.varD = bs->varD; a.varCD = bs->varCD;
     a.f = make_feats(g);
     a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.r = p->radius; a.g0 = g0;
     a.plane = bs->plane;
     a.set_stride = (size_t)N_STATS * 3 * bs->plane;
+    a.z = p->var_z;
     a.z2 = p->var_z * p->var_z;
+    a.lev = bs->mm >= 5 ? 1 : 0;
     a.sp_scale = p->ds_factor * kLog2e;
     a.rec = (float4 *)bs->var_rec;
     a.zero = (const float4 *)bs->var_zero;

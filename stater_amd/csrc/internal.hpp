@@ -105,7 +105,7 @@ const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
 
 // kernels (accumulate.hip, denoise.hip, synth.hip)
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
-int launch_prepass(stater_bufset *bs, float var_z, hipStream_t st);
+// The variance filter's pack pass also runs the prepass (lnS2, se2, m2C, jb).
 // ty: output rows per tile.  Variance filter: 4 or 8 (two rows per
 // filtering wavefront; 8 only for the r <= 10 feature-weighted filter,
 // otherwise 4).  Welch filter: 2, 4 or 8 (one row per wavefront; 2 and 8 only for the

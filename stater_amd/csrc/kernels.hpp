@@ -36,24 +36,17 @@ struct FeatArgs {
     unsigned bf16;          // bit k: feature k is stored as bfloat16
 };
 
-struct PreArgs {
-    const int32_t *n;
-    const float *stats;
-    float *jb, *lnS2, *se2, *m2C;
-    int S, W, H, ld, lev;
-    size_t plane;
-    float z;
-};
-
 struct VarArgs {
     const int32_t *n;
-    const float *m2;  // set-0 base of the m2 statistic
-    const float *lnS2, *se2, *m2C;
+    const float *m2;     // set-0 base of the m2 statistic
+    const float *stats;  // statistics block (the pack pass reads m2, m3, m4, c2X, c2Y)
+    float *lnS2, *se2, *m2C, *jb;  // prepass outputs, written by the pack pass
     float *varD, *varCD;
     FeatArgs f;
     int W, H, ld, r, g0;       // g0: first set of this group
     size_t plane, set_stride;  // distance between sets of the statistics block
-    float z2, sp_scale;        // sp_scale = ds_factor * log2 e
+    float z, z2, sp_scale;     // sp_scale = ds_factor * log2 e
+    int lev;                   // c2X, c2Y present (max_moment 5): m2C by LEV
     float4 *rec;               // packed records of the set group (written by the pack pass)
     const float4 *zero;        // one all-zero record
 };
@@ -96,7 +89,6 @@ struct SynthOut {
 
 // Argument builders of the launches.
 AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s);
-PreArgs make_pre_args(stater_bufset *bs, float var_z);
 FeatArgs make_feats(const stater_gbuffers *g);
 VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0);
 WelchArgs make_welch_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g);

@@ -1,9 +1,10 @@
 """Minimal HIP runtime plumbing over ctypes: device buffers and copies.
 
 Used by the Python layer, the tests and the bench to hand device memory to
-the C-ABI and to read planes back.  Copies use the synchronous hipMemcpy on
-the null stream; callers synchronise the library's stream before reading
-results.
+the C-ABI and to read planes back.  The library's streams do not synchronise
+with the null stream, so every copy and clear here first waits for the whole
+device and then runs the synchronous hipMemcpy / hipMemset (followed by a
+device wait): plumbing for tests and set-up, not for timed paths.
 """
 from __future__ import annotations
 
@@ -37,6 +38,10 @@ def hip() -> C.CDLL:
         _hip.hipStreamDestroy.argtypes = [C.c_void_p]
         _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
     return _hip
+
+
+def _sync() -> None:
+    _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
 def _ok(rc: int, what: str) -> None:
@@ -100,16 +105,20 @@ class DeviceBuffer:
     def upload(self, arr: np.ndarray, offset: int = 0) -> None:
         a = np.ascontiguousarray(arr)
         assert offset + a.nbytes <= self.nbytes
+        _sync()
         _ok(hip().hipMemcpy(C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D), "H2D")
 
     def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
         out = np.empty(count, dtype=dtype)
         assert offset + out.nbytes <= self.nbytes
+        _sync()
         _ok(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), out.nbytes, D2H), "D2H")
         return out
 
     def zero(self) -> None:
+        _sync()
         _ok(hip().hipMemset(C.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
+        _sync()
 
 
 def read_plane(pl, dtype=None) -> np.ndarray:
@@ -120,6 +129,7 @@ def read_plane(pl, dtype=None) -> np.ndarray:
     for c in range(pl.channels):
         nbytes = pl.pitch_bytes * (pl.height - 1) + pl.width * 4
         tmp = np.empty(nbytes // 4, dtype=dt)
+        _sync()
         _ok(hip().hipMemcpy(tmp.ctypes.data_as(C.c_void_p), C.c_void_p(pl.ptr + c * pl.channel_stride_bytes),
                             nbytes, D2H), "D2H plane")
         full = np.empty(pitch_elems * pl.height, dtype=dt)
@@ -138,5 +148,6 @@ def write_plane(pl, arr: np.ndarray) -> None:
         full[:, : pl.width] = a[c]
         nbytes = pl.pitch_bytes * (pl.height - 1) + pl.width * 4
         flat = np.ascontiguousarray(full.reshape(-1)[: nbytes // 4])
+        _sync()
         _ok(hip().hipMemcpy(C.c_void_p(pl.ptr + c * pl.channel_stride_bytes), flat.ctypes.data_as(C.c_void_p),
                             nbytes, H2D), "H2D plane")

@@ -251,7 +251,9 @@ def test_variance_filter_shapes(ctx, W, H, r):
     gD, gCD = read_set_buf(bs, MT5_LAMBDAS, "varD"), read_set_buf(bs, MT5_LAMBDAS, "varCD")
     for g, v in ((gD, vD), (gCD, vCD)):
         ok = np.isfinite(v) & (np.abs(v) < 1e37) & (amb == 0)
-        assert np.count_nonzero(ok) > 0.9 * ok.size
+        why = [(s, c, int(np.count_nonzero(~np.isfinite(v[s, c]))), int(np.count_nonzero(amb[s, c])))
+               for s in range(v.shape[0]) for c in range(3) if not ok[s, c].all()]
+        assert np.count_nonzero(ok) > 0.9 * ok.size, why
         assert np.max(np.abs(g[ok] - v[ok]) / np.maximum(np.abs(v[ok]), 1e-30)) < FILTER_RTOL
 
 
