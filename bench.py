@@ -7,7 +7,8 @@ per-pixel prepass (Jarque-Bera, Bonett, LEV), variance filter and
 multi-transform Welch filter, radius 10, alpha 0.005, z 2.80703, spatial
 sd 10, albedo/normal/depth range weights.  Synthetic seeded samples and
 features (seed 2) are generated on the device before the timed region.
-A step = reset + accumulate + denoise_vars + denoise_film.
+A step = accumulate from an empty set (STATER_ACC_FRESH: the result of a
+reset followed by the call) + Film resolve + denoise_vars + denoise_film.
 
 Prints ONE JSON line.  Per-kernel durations come from HIP events recorded
 around every launch on the library's stream.
@@ -148,11 +149,10 @@ def bench_c4(args) -> int:
     def plain():
         for bs, sc, feats, g, spp, _ in imgs:
             ch = SampleChunk(bs.width, bs.height, chunk_spp, buf=big.buf)  # the largest chunk's memory
-            bs.reset()
             for off in range(0, spp, chunk_spp):
                 k = min(chunk_spp, spp - off)
                 synth_samples(ctx, sc, ch, off, feats, spp=k)
-                bs.accumulate(ch.desc(spp=k, sanitize=True))
+                bs.accumulate(ch.desc(spp=k, sanitize=True), sa.STATER_ACC_FRESH if off == 0 else 0)
             bs.denoise_vars(p, g)
             bs.film_resolve()
             bs.denoise_film(p, g, film=bs.view("film"))
@@ -285,8 +285,7 @@ def main() -> int:
     film = bs.view("film")
 
     def step():
-        bs.reset()
-        bs.accumulate(desc)
+        bs.accumulate(desc, sa.STATER_ACC_SEQUENTIAL | sa.STATER_ACC_FRESH)  # = reset, then accumulate
         bs.film_resolve()
         bs.denoise_vars(p, g)
         bs.denoise_film(p, g, film=film)
@@ -364,7 +363,7 @@ def main() -> int:
         "gsample_s": gsample,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "what": "one 1080p job (reset + accumulate + film + denoise_vars + denoise_film) alone"},
+                       "what": "one 1080p job (accumulate from empty + film + denoise_vars + denoise_film) alone"},
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,

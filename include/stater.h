@@ -219,6 +219,11 @@ typedef struct {
 /* Per-call partial moments merged into the resident ones in FP64
  * (Chan-Golub-LeVeque / Pebay); c2X, c2Y, m1 continue sequentially. */
 #define STATER_ACC_MERGE64    1
+/* OR'ed into the mode: the call starts from an empty set, with the result of
+ * stater_bufset_reset on the same stream followed by the call (for every
+ * pixel of the film, and the Film sums with STATER_BUFSET_FILM), without the
+ * separate clearing pass or the read of the old statistics. */
+#define STATER_ACC_FRESH      16
 
 int stater_accumulate(stater_bufset *bs, const stater_samples *samples,
                       int mode, void *stream);

@@ -29,7 +29,7 @@ STATER_F32, STATER_I32, STATER_BF16 = 0, 1, 2
 STATER_PLANAR, STATER_INTERLEAVED = 0, 1
 STATER_BUFSET_NO_DERIVED, STATER_BUFSET_FILM = 1, 2
 STATER_CTX_TIMING = 1
-STATER_ACC_SEQUENTIAL, STATER_ACC_MERGE64 = 0, 1
+STATER_ACC_SEQUENTIAL, STATER_ACC_MERGE64, STATER_ACC_FRESH = 0, 1, 16
 MAX_LAMBDAS, MAX_RADIUS, MAX_GBUFFERS, MAX_EDGES = 8, 20, 4, 16
 
 

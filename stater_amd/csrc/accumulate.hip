@@ -191,6 +191,13 @@ __device__ __forceinline__ void film_store(const AccArgs &a, int c, uint32_t pix
     if (c == 0) a.film_chunk[3 * a.plane + pix] = (float)cnt;
 }
 
+// A fresh call leaves what stater_bufset_reset leaves in the statistic slots
+// a pixel-channel does not accumulate (all of them when it takes no sample).
+__device__ __forceinline__ void clear_slots(const AccArgs &a, int S, int c, int k0, uint32_t pix) {
+    for (int s = 0; s < S; ++s)
+        for (int k = k0; k < N_STATS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = 0.f;
+}
+
 // One thread per pixel (sequential mode): the three channels' statistics of
 // every set live in one lane, so the sanitising guard, the sample count, the
 // per-sample factors of the factored recurrence and the (u, v) loads are
@@ -208,9 +215,13 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
     if (cnt <= 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) film_store(a, c, pix, 0.f, 0);
+        if (a.fresh) {
+            a.n[pix] = 0;
+            for (int c = 0; c < 3; ++c) clear_slots(a, T + 1, c, 0, pix);
+        }
         return;
     }
-    uint32_t n = (uint32_t)a.n[pix];
+    uint32_t n = a.fresh ? 0u : (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
 
     float st[3][T + 1][NS];
@@ -220,7 +231,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         for (int s = 0; s <= T; ++s)
 #pragma unroll
             for (int k = 0; k < NS; ++k)
-                st[c][s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
+                st[c][s][k] = a.fresh ? 0.f : a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
 
     float L[3] = {0.f, 0.f, 0.f}, Wt[3][T + 1];
     if (T > 0) {
@@ -299,6 +310,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         for (int s = 0; s <= T; ++s)
 #pragma unroll
             for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[c][s][k];
+        if (NS < N_STATS && a.fresh) clear_slots(a, T + 1, c, NS, pix);
         film_store(a, c, pix, fsum[c], cnt);
     }
     a.n[pix] = (int32_t)n;
@@ -342,16 +354,21 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
                            a.count_offset);
     if (cnt <= 0) {
         film_store(a, c, pix, 0.f, 0);
+        if (a.fresh) {
+            if (c == 0) a.n[pix] = 0;
+            clear_slots(a, T + 1, c, 0, pix);
+        }
         return;
     }
-    const uint32_t n0 = (uint32_t)a.n[pix];
+    const uint32_t n0 = a.fresh ? 0u : (uint32_t)a.n[pix];
     const char *base = (const char *)a.data + (int64_t)y * a.pitch + (int64_t)x * 4;
 
     float st[T + 1][NS];
 #pragma unroll
     for (int s = 0; s <= T; ++s)
 #pragma unroll
-        for (int k = 0; k < NS; ++k) st[s][k] = a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
+        for (int k = 0; k < NS; ++k)
+            st[s][k] = a.fresh ? 0.f : a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
     float L = 0.f, Wt[T + 1];
     if (T > 0) {
         const size_t lp = (size_t)c * a.plane + pix;
@@ -425,6 +442,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
     for (int s = 0; s <= T; ++s)
 #pragma unroll
         for (int k = 0; k < NS; ++k) a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix] = st[s][k];
+    if (NS < N_STATS && a.fresh) clear_slots(a, T + 1, c, NS, pix);
     if (c == 0) a.n[pix] = (int32_t)(n0 + cnt);
     film_store(a, c, pix, fsum, cnt);
 }
@@ -448,22 +466,29 @@ __global__ void k_zero(uint32_t *p, size_t words) {
 
 // Film::MergeFilmTile (src/core/film.cpp:136-149): the Film's XYZ sums take
 // this call's tile sum through RGBToXYZ (src/core/spectrum.h:72-76, evaluated
-// as written) and its weight sum the sample count.
-__global__ void __launch_bounds__(256) k_film_merge(const float *chunk, float *xyzw, int W, int ld, size_t plane) {
+// as written) and its weight sum the sample count.  fresh: the sums start
+// from zero (as after a reset).
+__global__ void __launch_bounds__(256) k_film_merge(const float *chunk, float *xyzw, int W, int ld, size_t plane,
+                                                    int fresh) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
     const size_t pix = (size_t)y * ld + x;
     const float cnt = chunk[3 * plane + pix];
-    if (!(cnt > 0.f)) return;
+    if (!(cnt > 0.f)) {
+        if (fresh)
+            for (int k = 0; k < 4; ++k) xyzw[k * plane + pix] = 0.f;
+        return;
+    }
     const float r = chunk[pix], g = chunk[plane + pix], b = chunk[2 * plane + pix];
     const float X = 0.412453f * r + 0.357580f * g + 0.180423f * b;
     const float Y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
     const float Z = 0.019334f * r + 0.119193f * g + 0.950227f * b;
-    xyzw[pix] += X;
-    xyzw[plane + pix] += Y;
-    xyzw[2 * plane + pix] += Z;
-    xyzw[3 * plane + pix] += cnt;
+    const float *old = fresh ? nullptr : xyzw;
+    xyzw[pix] = (old ? old[pix] : 0.f) + X;
+    xyzw[plane + pix] = (old ? old[plane + pix] : 0.f) + Y;
+    xyzw[2 * plane + pix] = (old ? old[2 * plane + pix] : 0.f) + Z;
+    xyzw[3 * plane + pix] = (old ? old[3 * plane + pix] : 0.f) + cnt;
 }
 
 // Film::UpdateImage (src/core/film.cpp:188-220) without splats: XYZToRGB
@@ -491,9 +516,9 @@ __global__ void __launch_bounds__(256) k_film_resolve(const float *xyzw, int W, 
     }
 }
 
-int launch_film_merge(stater_bufset *bs, hipStream_t st) {
+int launch_film_merge(stater_bufset *bs, bool fresh, hipStream_t st) {
     hipLaunchKernelGGL(k_film_merge, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_chunk,
-                       bs->film_xyzw, bs->W, bs->ld, bs->plane);
+                       bs->film_xyzw, bs->W, bs->ld, bs->plane, fresh ? 1 : 0);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
@@ -563,9 +588,10 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
     if (s->pitch_bytes % 4 || s->comp_stride_bytes % 4 || s->sample_stride_bytes % 4 ||
         s->pitch_bytes < (int64_t)bs->W * 4)
         return fail(STATER_E_ARG, "stater_accumulate: strides must be float multiples covering a row");
-    const AccArgs a = make_acc_args(bs, s);
+    AccArgs a = make_acc_args(bs, s);
+    a.fresh = (mode & STATER_ACC_FRESH) ? 1 : 0;
     dim3 grid((bs->W + 63) / 64, bs->H);
-    const bool merge = mode == STATER_ACC_MERGE64;
+    const bool merge = (mode & ~STATER_ACC_FRESH) == STATER_ACC_MERGE64;
     LaunchTimer t(bs->ctx, st, merge ? "accumulate_merge64" : "accumulate");
     switch (bs->S - 1) {
         case 0: launch_t<0>(bs->mm, a, merge, grid, st); break;
@@ -576,7 +602,7 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
-    return bs->film_xyzw ? launch_film_merge(bs, st) : STATER_OK;
+    return bs->film_xyzw ? launch_film_merge(bs, a.fresh != 0, st) : STATER_OK;
 }
 
 }  // namespace stater

@@ -459,9 +459,10 @@ int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *
 int stater_accumulate(stater_bufset *bs, const stater_samples *samples, int mode, void *stream) {
     if (!bs || !samples || !samples->data) return fail(STATER_E_ARG, "stater_accumulate: NULL argument");
     if (samples->spp < 0) return fail(STATER_E_ARG, "stater_accumulate: spp < 0");
-    if (mode != STATER_ACC_SEQUENTIAL && mode != STATER_ACC_MERGE64)
+    const int base = mode & ~STATER_ACC_FRESH;
+    if (base != STATER_ACC_SEQUENTIAL && base != STATER_ACC_MERGE64)
         return fail(STATER_E_ARG, "stater_accumulate: unknown mode");
-    if (samples->spp == 0) return STATER_OK;
+    if (samples->spp == 0) return (mode & STATER_ACC_FRESH) ? stater_bufset_reset(bs, stream) : STATER_OK;
     return launch_accumulate(bs, samples, mode, pick_stream(bs, stream));
 }
 

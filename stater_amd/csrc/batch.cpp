@@ -1,13 +1,14 @@
 // batch.cpp — films of uneven size processed together.
 //
-// Every film's chain (reset; passes of chunk_spp generated samples, each
-// followed by accumulation; prepass and variance filter; Welch filter) runs as
-// the single-film launches, and the chains are spread over a few HIP streams
-// (longest chain first, onto the least loaded stream), so the launches of
-// small films fill the device beside the large films' tails and nothing waits
-// for the host.  With use_graph the fan-out and fan-in are captured once into
-// a hipGraph and replayed.  stater_batch_tune times the filters' tile heights
-// (2, 4 and 8 rows) on every film and keeps the fastest for it.
+// Every film's chain (passes of chunk_spp generated samples, each followed by
+// accumulation, the first from an empty set; prepass and variance filter;
+// Welch filter) runs as the single-film launches, and the chains are spread
+// over a few HIP streams (longest chain first, onto the least loaded
+// stream), so the launches of small films fill the device beside the large
+// films' tails and nothing waits for the host.  With use_graph the fan-out
+// and fan-in are captured once into a hipGraph and replayed.
+// stater_batch_tune times the filters' tile heights (2, 4 and 8 rows) on
+// every film and keeps the fastest for it.
 //
 // Each film runs the same kernels in the same order as on its own, so a batch
 // leaves bit-identical buffers.  A set created with STATER_BUFSET_FILM runs
@@ -82,11 +83,12 @@ int welch(const stater_batch *b, const stater_batch::Img &I, hipStream_t st, int
 
 // One film's chain on stream st.
 int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t st) {
-    int rc = stater_bufset_reset(I.bs, st);
+    int rc = STATER_OK;
     for (int off = 0; off < I.spp && !rc; off += b->chunk_spp) {
         const stater_samples s = chunk_desc(b, I, std::min(b->chunk_spp, I.spp - off));
         rc = launch_synth_samples(b->ctx, &I.sc, I.has_alb ? &I.albedo : nullptr, (uint32_t)off, nullptr, &s, st);
-        if (!rc) rc = launch_accumulate(I.bs, &s, STATER_ACC_SEQUENTIAL, st);
+        // the first pass starts the film from an empty set (no clearing pass)
+        if (!rc) rc = launch_accumulate(I.bs, &s, STATER_ACC_SEQUENTIAL | (off == 0 ? STATER_ACC_FRESH : 0), st);
     }
     if (!rc) rc = launch_var_filter(I.bs, &b->p, gb(I), st, I.ty_var);
     if (!rc && I.has_film) rc = launch_film_resolve(I.bs, 1.f, &I.film, st);

@@ -120,7 +120,7 @@ int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const st
                       hipStream_t st);
 int launch_mean_vars(stater_bufset *bs, hipStream_t st);
 int launch_zero(void *p, size_t bytes, hipStream_t st);
-int launch_film_merge(stater_bufset *bs, hipStream_t st);
+int launch_film_merge(stater_bufset *bs, bool fresh, hipStream_t st);
 int launch_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, hipStream_t st);
 int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                            const stater_plane *film, int plain, uint8_t *dec, hipStream_t st);

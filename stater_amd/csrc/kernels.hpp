@@ -26,6 +26,7 @@ struct AccArgs {
     float lam[kMaxSets];
     float inv_lam[kMaxSets];
     float *film_chunk;  // [4][plane] this call's film RGB sums and sample count, or null
+    int fresh;          // start from an empty set: old n and statistics are neither read nor kept
 };
 
 struct FeatArgs {

@@ -137,3 +137,43 @@ def test_sanitize_and_counts(ctx):
     n_ref, st_ref = oracle.accumulate(smp, MT5_LAMBDAS, 5, counts=counts, sanitize=True)
     assert np.array_equal(n, n_ref)
     assert np.array_equal(st[0], st_ref[0], equal_nan=True)
+
+
+@pytest.mark.parametrize("mode,mm", [(sa.STATER_ACC_SEQUENTIAL, 5), (sa.STATER_ACC_MERGE64, 5),
+                                     (sa.STATER_ACC_SEQUENTIAL, 3)])
+def test_fresh_call_equals_reset_then_call(ctx, mode, mm):
+    """STATER_ACC_FRESH on a set holding other statistics and Film sums gives
+    bit for bit what a cleared set gives, pixels without samples included."""
+    W, H, spp = 70, 20, 24
+    _, chunk, _ = generate(ctx, 2, 9, W, H, spp)
+    _, other, _ = generate(ctx, 2, 10, W, H, spp)
+    counts = np.random.default_rng(3).integers(-2, spp + 1, (H, W)).astype(np.int32)
+    counts[:, :5] = 0
+    cbuf = hipmem.DeviceBuffer(W * H * 4)
+    cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+    hipmem.write_plane(cplane, counts[None])
+    lam = MT5_LAMBDAS if mm == 5 else (0.5,)
+    a = sa.BufferSet(ctx, W, H, lam, mm, flags=sa.STATER_BUFSET_FILM)
+    b = sa.BufferSet(ctx, W, H, lam, mm, flags=sa.STATER_BUFSET_FILM)
+    a.accumulate(other.desc(sanitize=True))
+    a.accumulate(chunk.desc(sanitize=True, counts=cplane), mode | sa.STATER_ACC_FRESH)
+    b.accumulate(chunk.desc(sanitize=True, counts=cplane), mode)
+    a.film_resolve()
+    b.film_resolve()
+    ctx.synchronize()
+    na, sta = read_stats(a, lam)
+    nb, stb = read_stats(b, lam)
+    assert np.array_equal(na, nb) and not na[:, :5].any()
+    assert np.array_equal(sta.view(np.uint32), stb.view(np.uint32))
+    fa, fb = hipmem.read_plane(a.view("film")), hipmem.read_plane(b.view("film"))
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+    # a later plain call continues both the same way
+    a.accumulate(other.desc(spp=7, sanitize=True), mode)
+    b.accumulate(other.desc(spp=7, sanitize=True), mode)
+    ctx.synchronize()
+    assert np.array_equal(read_stats(a, lam)[1].view(np.uint32), read_stats(b, lam)[1].view(np.uint32))
+    # no samples at all: the set is cleared
+    a.accumulate(chunk.desc(spp=0), mode | sa.STATER_ACC_FRESH)
+    ctx.synchronize()
+    n0, st0 = read_stats(a, lam)
+    assert not n0.any() and not st0.any()
