@@ -981,13 +981,36 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
 #pragma unroll
         for (int c = 0; c < 3; ++c) word[c] |= (okpq > 0.f && !(E[c] > 0.f) ? 1u : 0u) << (idx & 31);
         if ((idx & 31) == 31 || idx == nfwd - 1) {
+            if (a.rec_f4) {  // straight into the weighting pass's record, as one (c0, c1, c2, 0) group
+                ((uint4 *)(a.rec + pix * a.rec_f4 + a.rec_head))[idx >> 5] = make_uint4(word[0], word[1], word[2], 0u);
+            } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a.bits[((size_t)c * kBitsWords + (idx >> 5)) * a.plane + pix] = word[c];
-                word[c] = 0u;
+                for (int c = 0; c < 3; ++c) a.bits[((size_t)c * kBitsWords + (idx >> 5)) * a.plane + pix] = word[c];
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) word[c] = 0u;
         }
     }
+}
+
+// The head of a pixel's weighting-pass record: film (3) and, with weights,
+// the 7 features: (film0, film1, film2, f0) (f1 .. f4) (f5, f6, 0, 0), or
+// (film0, film1, film2, 0) without.
+__device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int y, float4 *rec) {
+    float film[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        film[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+    if (a.rec_head == 1) {
+        rec[0] = make_float4(film[0], film[1], film[2], 0.f);
+        return;
+    }
+    float f[kNF];
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) f[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+    rec[0] = make_float4(film[0], film[1], film[2], f[0]);
+    rec[1] = make_float4(f[1], f[2], f[3], f[4]);
+    rec[2] = make_float4(f[5], f[6], 0.f, 0.f);
 }
 
 template <int S, int HWC, int TY>
@@ -1017,6 +1040,7 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
                 welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
                 a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
             }
+        if (a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4);
     }
     const bool large_n = welch_window_large_n(a, x0, y0, TY);
 
@@ -1200,8 +1224,9 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
 // Pass 2 from records (the path without a decision dump): film (3), features
 // (7, weighted filter), then pass 1's decision words grouped by word index,
 // (c0, c1, c2, 0) per group, so that one 16-byte read fetches a neighbour's
-// three channel words; staged by a loader wavefront as in the variance
-// filter.  Each lane weights two vertically adjacent pixels A and B.  A
+// three channel words.  Pass 1 writes the records (the head when it loads the
+// pixel's own operands, each word group when it is complete); pass 2 stages
+// rows of them with a loader wavefront as in the variance filter.  Each lane weights two vertically adjacent pixels A and B.  A
 // forward offset's decision comes from the pixel's own words (in registers),
 // a backward offset's from the neighbour's record, at a word index that is
 // the same for the whole wavefront.
@@ -1216,49 +1241,6 @@ template <bool HASW, int HWC>
 constexpr int acc_loads_per_row() { return (HWC * AccRec<HASW>::FLOAT4S + 63) / 64; }
 template <bool HASW, int HWC, int TY>
 constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<HASW, HWC>() * 64 * 16; }
-
-// Pack pass, staged as for the variance filter.  Staged planes: film (3),
-// features (7), the 21 decision words (channel-major).
-constexpr int kAccPackPlanes = 3 + kNF + 3 * kBitsWords;
-
-template <bool HASW>
-__device__ __forceinline__ float acc_pack_input(const WelchArgs &a, int pl, int x, int y) {
-    if (pl < 3)
-        return *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + (int64_t)pl * a.film_cs);
-    if (pl < 3 + kNF) return HASW && pl - 3 < a.f.nf ? feat_at(a.f, pl - 3, x, y) : 0.f;
-    return __uint_as_float(a.bits[(size_t)(pl - 3 - kNF) * a.plane + (size_t)y * a.ld + x]);
-}
-
-template <bool HASW>
-__global__ void __launch_bounds__(256) k_accum_pack(WelchArgs a, float4 *zero) {
-    using R = AccRec<HASW>;
-    __shared__ float in[kAccPackPlanes][64];
-    const int y = blockIdx.y, x0 = blockIdx.x * 64;
-    if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int i = threadIdx.x; i < kAccPackPlanes * 64; i += 256) {
-        const int pl = i >> 6, px = i & 63;
-        in[pl][px] = x0 + px < a.W ? acc_pack_input<HASW>(a, pl, x0 + px, y) : 0.f;
-    }
-    __syncthreads();
-    const int npx = min(64, a.W - x0);
-    float4 *rec = a.rec + ((size_t)y * a.ld + x0) * R::FLOAT4S;
-    for (int j = threadIdx.x; j < npx * R::FLOAT4S; j += 256) {
-        const int px = j / R::FLOAT4S, item = j - px * R::FLOAT4S;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (item == 0) {
-            v = make_float4(in[0][px], in[1][px], in[2][px], HASW ? in[3][px] : 0.f);
-        } else if (HASW && item < 3) {
-            const int b = 4 * item;  // features 1..4, then 5, 6
-            v = make_float4(in[b][px], in[b + 1][px], item == 1 ? in[b + 2][px] : 0.f,
-                            item == 1 ? in[b + 3][px] : 0.f);
-        } else if (item < R::HEAD4 + kBitsWords) {
-            const int g = item - R::HEAD4;  // word group: (c0, c1, c2, 0)
-            v = make_float4(in[3 + kNF + g][px], in[3 + kNF + kBitsWords + g][px],
-                            in[3 + kNF + 2 * kBitsWords + g][px], 0.f);
-        }
-        rec[j] = v;
-    }
-}
 
 template <bool HASW>
 struct AccPix {
@@ -1508,9 +1490,6 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 template <bool HASW>
 static int accum2_launch(const WelchArgs &a, hipStream_t st) {
     constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = 4;
-    hipLaunchKernelGGL((k_accum_pack<HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
-                       const_cast<float4 *>(a.zero));
-    STATER_HIP(hipGetLastError());
     const size_t lds = (size_t)acc_lds_bytes<HASW, HWC, TY>();
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum2<HASW, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1527,9 +1506,16 @@ static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
     size_t lds = (size_t)TY * DecideLayout<S>::FLOATS * HWC * 4;
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (!a.dec && a.rec) {  // the decision pass writes the weighting pass's records itself
+        WelchArgs b = a;
+        b.rec_f4 = hasw ? AccRec<true>::FLOAT4S : AccRec<false>::FLOAT4S;
+        b.rec_head = hasw ? AccRec<true>::HEAD4 : AccRec<false>::HEAD4;
+        hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, b);
+        STATER_HIP(hipGetLastError());
+        return hasw ? accum2_launch<true>(b, st) : accum2_launch<false>(b, st);
+    }
     hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
     STATER_HIP(hipGetLastError());
-    if (!a.dec && a.rec) return hasw ? accum2_launch<true>(a, st) : accum2_launch<false>(a, st);
     if (hasw) {
         lds = (size_t)TY * AccumLayout<true>::FLOATS * HWC * 4;
         STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<true, HWC, TY>,

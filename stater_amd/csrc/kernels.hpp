@@ -68,6 +68,7 @@ struct WelchArgs {
     uint32_t *bits;        // forward half-window decisions [3][kBitsWords][plane] (symmetric form)
     float4 *rec;           // pass-2 records (the buffer set's variance-filter record planes)
     const float4 *zero;    // one all-zero record
+    int rec_f4, rec_head;  // > 0: pass 1 writes the records (float4 items per record, head items)
     FeatArgs f;
     int W, H, ld, r, mode3;
     size_t plane, set_stride;

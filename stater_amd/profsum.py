@@ -29,8 +29,8 @@ CYCLES_PER_VALU = 4     # a wave64 FP32 VALU instruction on a 16-lane SIMD
 # kernel symbol -> the library timer whose launches it belongs to (a timer
 # can cover several kernels: the variance filter is a pack pass (with the
 # prepass) and a filter pass per set group, the mode-2 Welch filter a
-# decision pass, a pack pass and a weighting pass)
-TIMER_NAMES = [("k_welch", "welch_filter"), ("k_welch_decide", "welch_filter"), ("k_accum_pack", "welch_filter"),
+# decision pass and a weighting pass)
+TIMER_NAMES = [("k_welch", "welch_filter"), ("k_welch_decide", "welch_filter"),
                ("k_welch_accum2", "welch_filter"), ("k_welch_accum", "welch_filter"),
                ("k_var_filter", "var_filter"), ("k_var_pack", "var_filter"),
                ("k_accumulate_merge", "accumulate_merge64"), ("k_accumulate", "accumulate"),
