@@ -57,7 +57,10 @@ def kernel_work(W, H, spp, S, r):
         # mode 2 tests each unordered pair once and weights every ordered pair
         "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
-        "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) * 2, "bound": "valu"},
+        # samples read once; a fresh call writes the statistics (n + 7 per set-channel), the
+        # reference logs and Box-Cox shifted means (3 + 3T planes) and the Film chunk (4 planes)
+        "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) + P * 4 * (3 + 3 * T + 4),
+                       "bound": "valu"},
     }
 
 
