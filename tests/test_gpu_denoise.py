@@ -62,6 +62,30 @@ def test_plain_method_128(ctx):
     assert frac_cross < 0.25 * frac_same, (frac_cross, frac_same)
 
 
+@pytest.mark.parametrize("W,H,r", [(77, 45, 3), (130, 20, 12)])
+def test_plain_method_shapes(ctx, W, H, r):
+    """The plain method on ragged films and a radius above 10."""
+    alpha = 0.005
+    sc, chunk, _ = generate(ctx, 1, 4, W, H, 32)
+    bs = sa.BufferSet(ctx, W, H, (), max_moment=3)
+    bs.accumulate(chunk.desc())
+    bs.denoise_plain(r, alpha)
+    ctx.synchronize()
+    n, st = read_stats(bs, ())
+    meanD = hipmem.read_plane(bs.view("t0-b0-film-meanD"))
+    varD = hipmem.read_plane(bs.view("t0-b0-film-varD"))
+    s2 = (st[0, 1] / (n.astype(np.float32) - np.float32(1))).astype(np.float32)
+    ref = oracle.denoise_film(n, st[0:1, 0], s2[None], st[0, 0], r, 0.0, alpha, want_varw=True,
+                              want_decisions=True)
+    dec = _decisions(ctx, bs, sa.default_params(radius=r, alpha=alpha), None, True, W, H, r)
+    diff = dec != ref["decisions"]
+    assert np.count_nonzero(diff) / dec.size <= DECISION_MISMATCH_MAX
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
+    assert np.count_nonzero(same) > 0.9 * same.size
+    np.testing.assert_allclose(meanD[same], ref["filmD"][same], rtol=FILTER_RTOL)
+    np.testing.assert_allclose(varD[same], ref["varW"][same], rtol=FILTER_RTOL)
+
+
 def _mt5_setup(ctx, W=96, H=64, spp=64, seed=2, counts=None):
     """Accumulate spp synthetic samples per pixel (or counts[y, x] of them)."""
     sc, chunk, feats = generate(ctx, 2, seed, W, H, spp)
@@ -197,19 +221,21 @@ def test_welch_filter_mt5(ctx, mt_mode, ragged):
     np.testing.assert_array_equal(discr, want)
 
 
-@pytest.mark.parametrize("W,H,r,weighted", [
-    (77, 45, 1, True),     # smallest window of the one-test-per-pair path, ragged tiles
-    (77, 45, 3, False),    # unweighted (no features, ds_factor 0)
-    (130, 37, 7, True),    # three 64-column tiles, the last one 2 wide
-    (8, 5, 10, True),      # image smaller than the window and than one tile
-    (33, 20, 12, True),    # r > 10: every ordered pair tested on its own
-    (70, 9, 0, True),      # r = 0: the film itself
+@pytest.mark.parametrize("W,H,r,weighted,mt_mode", [
+    (77, 45, 1, True, 2),     # smallest window of the one-test-per-pair path, ragged tiles
+    (77, 45, 3, False, 2),    # unweighted (no features, ds_factor 0)
+    (130, 37, 7, True, 2),    # three 64-column tiles, the last one 2 wide
+    (8, 5, 10, True, 2),      # image smaller than the window and than one tile
+    (33, 20, 12, True, 2),    # r > 10: every ordered pair tested on its own
+    (70, 9, 0, True, 2),      # r = 0: the film itself
+    (77, 45, 5, True, 3),     # mode 3 (the set with the smallest jb), ragged tiles
+    (33, 20, 12, False, 3),   # mode 3, r > 10, unweighted
 ])
-def test_welch_filter_shapes(ctx, W, H, r, weighted):
-    """Window radii, ragged and tiny films, weighted and unweighted: the
-    decisions and filmD against the oracle."""
+def test_welch_filter_shapes(ctx, W, H, r, weighted, mt_mode):
+    """Window radii, ragged and tiny films, weighted and unweighted, both
+    multi-transform modes: the decisions and filmD against the oracle."""
     sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=32, seed=5)
-    p = sa.default_params(radius=r, mt_mode=2)
+    p = sa.default_params(radius=r, mt_mode=mt_mode)
     g, feats_list = None, None
     if weighted:
         g = feats.gbuffers()
@@ -222,8 +248,9 @@ def test_welch_filter_shapes(ctx, W, H, r, weighted):
     ctx.synchronize()
     n, st = read_stats(bs, MT5_LAMBDAS)
     varCD = read_set_buf(bs, MT5_LAMBDAS, "varCD")
-    ref = oracle.denoise_film(n, st[:, 0].copy(), varCD, st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
-                              feats=feats_list, want_decisions=True)
+    jb = read_set_buf(bs, MT5_LAMBDAS, "jb")[:, 0] if mt_mode == 3 else None
+    ref = oracle.denoise_film(n, st[:, 0].copy(), varCD, st[0, 0], r, p.ds_factor, p.alpha, mt_mode=mt_mode,
+                              jb=jb, feats=feats_list, want_decisions=True)
     dec = _decisions(ctx, bs, p, g, False, W, H, r)
     diff = dec != ref["decisions"]
     assert np.count_nonzero(diff) / dec.size <= DECISION_MISMATCH_MAX
