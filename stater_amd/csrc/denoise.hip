@@ -144,8 +144,10 @@ constexpr int var_ring_slots() { return 2 * TY - 1 + kVarAhead; }
 template <int G, bool HASW, int HWC, int TY>
 constexpr int var_lds_bytes() { return var_ring_slots<TY>() * var_slot_float4s<G, HASW, HWC>() * 16; }
 
-// s_waitcnt immediate that waits for vector-memory count <= v only.
+// s_waitcnt immediates that wait for vector-memory count <= v, or LDS / scalar
+// count <= v, only.
 constexpr int waitcnt_vm(int v) { return (v & 15) | ((v >> 4) << 14) | (7 << 4) | (15 << 8); }
+constexpr int waitcnt_lgkm(int v) { return 15 | (3 << 14) | (7 << 4) | ((v > 15 ? 15 : v) << 8); }
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
@@ -240,11 +242,11 @@ struct VarPix {
     float gp[kNF];
 };
 
-// The pixel's own statistics come from its record (all items issued at
-// once, one memory latency per tile instead of one per dependent load); an
-// inactive lane reads the zero record.  The record clamps a non-finite s^2 to
-// FLT_MAX for its neighbours' sums; the pixel's own sum starts from the exact
-// value, reloaded in that (rare) case.
+// The pixel's own operands come from its record (all items issued at once,
+// one memory latency per tile instead of one per dependent load); an
+// inactive lane reads the zero record.  The sums start at zero: the pixel
+// itself is one of the window's columns (dx = dy = 0), where its weight is
+// exactly 1 and its test compares its interval with itself.
 template <int G, bool HASW>
 __device__ __forceinline__ void var_pix_init(const VarArgs &a, int x, int y, bool active, VarPix<G> &P) {
     using R = VarRec<G, HASW>;
@@ -264,15 +266,7 @@ __device__ __forceinline__ void var_pix_init(const VarArgs &a, int x, int y, boo
             const float4 it = v[R::SINGLE4 + g * 3 + c];
             P.Lp[g][c] = it.x;
             P.Zp[g][c] = it.y;
-            P.num[g][c] = it.z;
-            P.numC[g][c] = it.w;
-            P.den[g][c] = 1.f;  // the pixel itself, weight 1
-            if (it.z == FLT_MAX || it.w == FLT_MAX) {
-                const size_t o = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
-                const float inv = 1.f / (float)(a.n[pix] - 1);
-                P.num[g][c] = a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
-                P.numC[g][c] = a.m2C[o] * inv;
-            }
+            P.num[g][c] = P.numC[g][c] = P.den[g][c] = 0.f;
         }
 }
 
@@ -322,17 +316,13 @@ __device__ __forceinline__ void var_test(float Lp, float Zp, float4 q, float w, 
 // The pairs of neighbour column dx with A and B: both weights, then every
 // set-channel's log-variance test for A and for B side by side.  A row
 // outside a pixel's window comes with lw = -inf (weighted) or m = 0
-// (unweighted), so its weight is exactly 0 and every sum stays as it is; the
-// pixel itself (dx = 0 on its own row) gets weight 0 the same way.
+// (unweighted), so its weight is exactly 0 and every sum stays as it is.
 template <int G, bool HASW>
-__device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, HASW> &c, int dx, bool selfA,
-                                             bool selfB, float lwA, float lwB, float mA, float mB, VarPix<G> &A,
-                                             VarPix<G> &B) {
+__device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, HASW> &c, float dx2, float lwA,
+                                             float lwB, float mA, float mB, VarPix<G> &A, VarPix<G> &B) {
     const float fq[kNF] = {c.s0.y, c.s0.z, c.s0.w, c.s1.x, c.s1.y, c.s1.z, c.s1.w};
-    const float dx2 = (float)(dx * dx);
-    const bool zA = selfA && dx == 0, zB = selfB && dx == 0;
-    const float wA = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, zA ? -INFINITY : lwA, zA ? 0.f : mA, dx2);
-    const float wB = var_weight<HASW>(a.sp_scale, B.gp, c.s0.x, fq, zB ? -INFINITY : lwB, zB ? 0.f : mB, dx2);
+    const float wA = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, lwA, mA, dx2);
+    const float wB = var_weight<HASW>(a.sp_scale, B.gp, c.s0.x, fq, lwB, mB, dx2);
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
@@ -346,21 +336,25 @@ __device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, H
 // issued before the tests of column dx, in two register sets that
 // alternate; pairs of columns from dx = -r, then the last column alone.
 template <int G, bool HASW>
-__device__ __forceinline__ void var_cols(const VarArgs &a, const float4 *slot, int tx, bool selfA, bool selfB,
-                                         float lwA, float lwB, float mA, float mB, VarPix<G> &A, VarPix<G> &B) {
+__device__ __forceinline__ void var_cols(const VarArgs &a, const float4 *slot, int tx, float lwA, float lwB,
+                                         float mA, float mB, VarPix<G> &A, VarPix<G> &B) {
     constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
+    constexpr int ITEMS = VarRec<G, HASW>::ITEMS;
     const float4 *q = slot + tx * F4;  // column of dx = -r
     VarCol<G, HASW> X, Y;
     var_col_load<G, HASW>(q, X);
     for (int dx = -a.r; dx < a.r; dx += 2, q += 2 * F4) {
         var_col_load<G, HASW>(q + F4, Y);
+        // X (issued one column earlier) has landed once at most Y's reads are outstanding
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(ITEMS));
         __builtin_amdgcn_sched_barrier(0);
-        var_col_test<G, HASW>(a, X, dx, selfA, selfB, lwA, lwB, mA, mB, A, B);
+        var_col_test<G, HASW>(a, X, (float)(dx * dx), lwA, lwB, mA, mB, A, B);
         var_col_load<G, HASW>(q + 2 * F4, X);
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(ITEMS));
         __builtin_amdgcn_sched_barrier(0);
-        var_col_test<G, HASW>(a, Y, dx + 1, selfA, selfB, lwA, lwB, mA, mB, A, B);
+        var_col_test<G, HASW>(a, Y, (float)((dx + 1) * (dx + 1)), lwA, lwB, mA, mB, A, B);
     }
-    var_col_test<G, HASW>(a, X, a.r, selfA, selfB, lwA, lwB, mA, mB, A, B);
+    var_col_test<G, HASW>(a, X, (float)(a.r * a.r), lwA, lwB, mA, mB, A, B);
 }
 
 // Step k of the tile: the staged row at dy = k - r from A (k <= 2r) and
@@ -373,18 +367,39 @@ __device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, in
     const float lwA = offA ? -INFINITY : a.sp_scale * (float)(dyA * dyA);
     const float lwB = offB ? -INFINITY : a.sp_scale * (float)(dyB * dyB);
     const float mA = offA ? 0.f : 1.f, mB = offB ? 0.f : 1.f;
-    const bool selfA = dyA == 0, selfB = dyB == 0;
-    var_cols<G, HASW>(a, slot, tx, selfA, selfB, lwA, lwB, mA, mB, A, B);
+    var_cols<G, HASW>(a, slot, tx, lwA, lwB, mA, mB, A, B);
+}
+
+// The pixel's own column as the pair loop left it, made what the filter
+// defines: the pixel itself always counts with weight 1 (the loop rejects it
+// only when its own interval is empty, z^2 se2 < 0), and with its exact s^2
+// (the record holds FLT_MAX for a non-finite one).  Rare; once per tile.
+template <int G, bool HASW>
+__device__ __forceinline__ void var_own_fix(const VarArgs &a, size_t pix, int g, int c, float Lp, float Zp, float &den,
+                                            float &num, float &numC) {
+    using R = VarRec<G, HASW>;
+    const float dl = Lp - Lp;
+    const bool rejected = __builtin_fmaf(dl, dl, -Zp) > Zp;
+    const float4 own = a.rec[pix * R::FLOAT4S + R::SINGLE4 + g * 3 + c];
+    if (!rejected && own.z != FLT_MAX && own.w != FLT_MAX) return;
+    if (rejected) {
+        den += 1.f;
+        num += own.z;
+        numC += own.w;
+    }
+    const float inv = 1.f / (float)(a.n[pix] - 1);
+    if (own.z == FLT_MAX) num = (num - FLT_MAX) + a.m2[(size_t)(a.g0 + g) * a.set_stride + c * a.plane + pix] * inv;
+    if (own.w == FLT_MAX) numC = (numC - FLT_MAX) + a.m2C[((size_t)(a.g0 + g) * 3 + c) * a.plane + pix] * inv;
 }
 
 // A pixel with fewer than 4 samples is not filtered: it keeps its own
-// variance (the initial sums, den = 1), read again here so that the pair
-// loop runs for it like for every other lane.  Both pixels of the lane are
-// written in one pass over the set-channels, with plain per-pixel values
-// (no selection between the two pixels' state).
-template <int G>
-__device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool actA, bool actB, const VarPix<G> &A,
-                                          const VarPix<G> &B) {
+// variance, read again here so that the pair loop runs for it like for
+// every other lane.  Both pixels of the lane are written in one pass over the
+// set-channels, with plain per-pixel values (no selection between the two
+// pixels' state).
+template <int G, bool HASW>
+__device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool actA, bool actB, VarPix<G> &A,
+                                          VarPix<G> &B) {
     const size_t pA = (size_t)yA * a.ld + x, pB = pA + a.ld;
     const int nA = actA ? a.n[pA] : 4, nB = actB ? a.n[pB] : 4;
 #pragma unroll
@@ -393,6 +408,8 @@ __device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool 
         for (int c = 0; c < 3; ++c) {
             const size_t off = ((size_t)(a.g0 + g) * 3 + c) * a.plane;
             const size_t m2off = (size_t)(a.g0 + g) * a.set_stride + c * a.plane;
+            if (actA && nA >= 4) var_own_fix<G, HASW>(a, pA, g, c, A.Lp[g][c], A.Zp[g][c], A.den[g][c], A.num[g][c], A.numC[g][c]);
+            if (actB && nB >= 4) var_own_fix<G, HASW>(a, pB, g, c, B.Lp[g][c], B.Zp[g][c], B.den[g][c], B.num[g][c], B.numC[g][c]);
             float VA = A.num[g][c] / A.den[g][c], CA = A.numC[g][c] / A.den[g][c];
             float VB = B.num[g][c] / B.den[g][c], CB = B.numC[g][c] / B.den[g][c];
             if (nA < 4) {
@@ -488,13 +505,14 @@ __global__ void __launch_bounds__(64 * (TY + 1))
     VarPix<G> A, B;
     var_pix_init<G, HASW>(a, x, yA, actA, A);
     var_pix_init<G, HASW>(a, x, yA + 1, actB, B);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the own records have landed: no counter waits in the pair loop
     __syncthreads();
     for (int k = 0; k <= last; ++k) {
         const float4 *slot = vring + ((2 * wave + k) % NS) * SLOT;
         if (actA) var_row<G, HASW>(a, slot, lane, k, A, B);
         __syncthreads();
     }
-    var_store<G>(a, x, yA, actA, actB, A, B);
+    var_store<G, HASW>(a, x, yA, actA, actB, A, B);
 }
 
 template <int G, bool HASW, int HWC, int TY>
