@@ -1416,14 +1416,18 @@ template <bool HASW, int MA, int MB>
 __device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, const AccRowP &RA, const AccRowP &RB,
                                          AccPix<HASW> &A, AccPix<HASW> &B) {
     constexpr int F4 = AccRec<HASW>::FLOAT4S;
+    // LDS reads per column: film and features, and a word group per pixel that reads its neighbour's
+    constexpr int NL = AccRec<HASW>::HEAD4 + (MA == kAccBwd || MA == kAccGen) + (MB == kAccBwd || MB == kAccGen);
     const int r = a.r;
     AccCol<HASW> X, Y;
     acc_col_load<HASW, MA, MB>(q, RA, RB, -r, r, X);
     for (int dx = -r; dx < r; dx += 2, q += 2 * F4) {
         acc_col_load<HASW, MA, MB>(q + F4, RA, RB, dx + 1, r, Y);
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(NL));  // X, issued one column earlier, has landed
         __builtin_amdgcn_sched_barrier(0);
         acc_col_test<HASW, MA, MB>(a, X, dx, RA, RB, A, B);
         acc_col_load<HASW, MA, MB>(q + 2 * F4, RA, RB, dx + 2, r, X);
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(NL));
         __builtin_amdgcn_sched_barrier(0);
         acc_col_test<HASW, MA, MB>(a, Y, dx + 1, RA, RB, A, B);
     }
@@ -1488,6 +1492,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
     AccPix<HASW> A, B;
     acc_pix_init<HASW>(a, x, yA, actA, A);
     acc_pix_init<HASW>(a, x, yA + 1, actB, B);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the own records have landed: no counter waits in the loops
     int baseA = 0, baseB = 0;  // own words shifted out so far
     __syncthreads();
     for (int k = 0; k <= last; ++k) {
