@@ -91,9 +91,17 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float nf, float x
 //     Wt (2^w - 1) = (Wt 2^k)(2^f - 1) + (Wt 2^k - Wt),
 // so the scale Wt rides on the exponent and costs no multiply.  |error| is a
 // few ulp of the result at every w; w below -150 gives -Wt (Wt 2^k flushes to
-// zero), the limit x -> 0.
+// zero), the limit x -> 0.  The caller keeps w finite or +inf: a sample of 0
+// (or NaN) arrives as ln-ratio -1e30 instead of -inf, which drives k far
+// below -150; the conversion saturates (v_cvt_i32_f32 clamps) and the scale
+// flushes to zero, so the result is -Wt there too.
+__device__ __forceinline__ int cvt_i32_sat(float k) {
+    int i;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(i) : "v"(k));
+    return i;
+}
+
 __device__ __forceinline__ float wexpm1(float w, float Wt) {
-    w = fmaxf(w, -150.f);
     const float k = __builtin_rintf(w);
     const float f = w - k;
     float p = 1.540351563395118e-04f;
@@ -103,7 +111,7 @@ __device__ __forceinline__ float wexpm1(float w, float Wt) {
     p = __builtin_fmaf(p, f, 2.4022649795021536e-01f);
     p = __builtin_fmaf(p, f, 6.931471879267055e-01f);
     p *= f;  // 2^f - 1
-    const float wk = __builtin_ldexpf(Wt, (int)k);
+    const float wk = __builtin_ldexpf(Wt, cvt_i32_sat(k));
     return __builtin_fmaf(wk, p, wk - Wt);
 }
 
@@ -290,8 +298,8 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             const float xa = xD * f.a1, ya = yD * f.a1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                // ln x - L; log of 0 is -inf and expm1 then gives -1: y(0) = -1/l
-                const float dl = __builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c];
+                // ln x - L; log of 0 (-inf) and of NaN become -1e30, where expm1 gives -1: y(0) = -1/l
+                const float dl = fmaxf(__builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c], -1e30f);
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
                     meng_factored<MM>(st[c][s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[c][s]), f, xa, ya);
@@ -407,7 +415,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         const NFac fr = nfac((float)(j + 1));
         NFac fg;
         if (MM >= 5) fg = nfac((float)(n0 + j + 1));
-        const float dl = T > 0 ? __builtin_amdgcn_logf(v) * 0.69314718055994531f - L : 0.f;
+        const float dl = T > 0 ? fmaxf(__builtin_amdgcn_logf(v) * 0.69314718055994531f - L, -1e30f) : 0.f;
 #pragma unroll
         for (int s = 0; s <= T; ++s) {
             const float t = s == 0 ? v : wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[s]);
