@@ -373,7 +373,16 @@ __device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, in
 // The pixel's own column as the pair loop left it, made what the filter
 // defines: the pixel itself always counts with weight 1 (the loop rejects it
 // only when its own interval is empty, z^2 se2 < 0), and with its exact s^2
-// (the record holds FLT_MAX for a non-finite one).  Rare; once per tile.
+// (the record holds FLT_MAX for a non-finite one).  Rare: only pixel-channels
+// that var_own_suspect flags come here.
+// Whether the pixel's own column may need var_own_fix: the loop rejected it
+// (its own interval is empty), or its lnS2 is not finite, which the prepass
+// sets whenever m2 (and so the record's s^2 or m2C/(n-1)) is not finite.
+__device__ __forceinline__ bool var_own_suspect(float Lp, float Zp) {
+    const float dl = Lp - Lp;
+    return __builtin_fmaf(dl, dl, -Zp) > Zp || !__builtin_isfinite(Lp);
+}
+
 template <int G, bool HASW>
 __device__ __forceinline__ void var_own_fix(const VarArgs &a, size_t pix, int g, int c, float Lp, float Zp, float &den,
                                             float &num, float &numC) {
@@ -408,8 +417,10 @@ __device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool 
         for (int c = 0; c < 3; ++c) {
             const size_t off = ((size_t)(a.g0 + g) * 3 + c) * a.plane;
             const size_t m2off = (size_t)(a.g0 + g) * a.set_stride + c * a.plane;
-            if (actA && nA >= 4) var_own_fix<G, HASW>(a, pA, g, c, A.Lp[g][c], A.Zp[g][c], A.den[g][c], A.num[g][c], A.numC[g][c]);
-            if (actB && nB >= 4) var_own_fix<G, HASW>(a, pB, g, c, B.Lp[g][c], B.Zp[g][c], B.den[g][c], B.num[g][c], B.numC[g][c]);
+            if (actA && nA >= 4 && var_own_suspect(A.Lp[g][c], A.Zp[g][c]))
+                var_own_fix<G, HASW>(a, pA, g, c, A.Lp[g][c], A.Zp[g][c], A.den[g][c], A.num[g][c], A.numC[g][c]);
+            if (actB && nB >= 4 && var_own_suspect(B.Lp[g][c], B.Zp[g][c]))
+                var_own_fix<G, HASW>(a, pB, g, c, B.Lp[g][c], B.Zp[g][c], B.den[g][c], B.num[g][c], B.numC[g][c]);
             float VA = A.num[g][c] / A.den[g][c], CA = A.numC[g][c] / A.den[g][c];
             float VB = B.num[g][c] / B.den[g][c], CB = B.numC[g][c] / B.den[g][c];
             if (nA < 4) {
