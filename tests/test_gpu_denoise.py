@@ -313,3 +313,64 @@ def test_target_spp_exact(ctx, budget, cap):
         assert int(tgt.sum()) == budget * W * H
     else:
         assert np.array_equal(tgt, head)
+
+
+def _write_stats(bs, lambdas, n, st):
+    """Load n [H,W] and statistics [S,7,3,H,W] into a set through its views."""
+    from gpu_helpers import STAT_NAMES, set_slots
+    hipmem.write_plane(bs.view("n"), n[None].astype(np.int32))
+    for s, j in enumerate(set_slots(lambdas)):
+        pre = f"t0-b{j}-film-" if s == 0 else f"t0-b{j}-"
+        for k, nm in enumerate(STAT_NAMES):
+            hipmem.write_plane(bs.view(pre + nm), st[s, k].astype(np.float32))
+
+
+def test_variance_filter_edge_statistics(ctx):
+    """Statistics the sampler rarely makes, written straight into a set: an
+    empty own Bonett interval (se2 < 0), a non-finite m2 and pixels with too
+    few samples.  The pixel itself always counts with weight 1 and its exact
+    s^2; everything else as the oracle on the same prepass values."""
+    W, H, r = 40, 24, 4
+    rng = np.random.default_rng(17)
+    S = len(oracle.set_lambdas(MT5_LAMBDAS))
+    n = np.full((H, W), 12, np.int32)
+    n[3, 5] = 3                                        # not filtered: keeps its own s^2
+    st = np.zeros((S, 7, 3, H, W), np.float32)
+    st[:, 0] = rng.random((S, 3, H, W))
+    st[:, 1] = rng.uniform(0.5, 2.0, (S, 3, H, W))
+    st[:, 2] = rng.normal(0, 0.1, (S, 3, H, W))
+    K = rng.uniform(1.5, 4.0, (S, 3, H, W))
+    st[:, 3] = K * st[:, 1] ** 2 / 12.0
+    st[:, 4:6] = rng.normal(0, 0.05, (S, 2, 3, H, W))
+    st[:, 6] = rng.uniform(0.5, 1.0, (S, 3, H, W))
+    st[:, 3, :, 10, 10] = 0.2 * st[:, 1, :, 10, 10] ** 2 / 12.0   # K = 0.2 < (n-3)/n: se2 < 0
+    st[2, 1, 1, 20, 30] = np.inf                                  # a non-finite m2
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, max_moment=5)
+    _write_stats(bs, MT5_LAMBDAS, n, st)
+    sc, chunk, feats = generate(ctx, 2, 3, W, H, 4)
+    p = sa.default_params(radius=r)
+    bs.denoise_vars(p, feats.gbuffers())
+    ctx.synchronize()
+    lnS2, se2, m2C = (read_set_buf(bs, MT5_LAMBDAS, k) for k in ("lnS2", "se2", "m2C"))
+    assert np.all(se2[:, :, 10, 10] < 0)
+    assert np.isinf(lnS2[2, 1, 20, 30])
+    f = feats.to_numpy()
+    feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    vD, vCD, amb = oracle.denoise_vars(n, st[:, 1], lnS2, se2, m2C, r, p.ds_factor, p.var_z, feats_list,
+                                       want_ambiguous=True)
+    gD, gCD = read_set_buf(bs, MT5_LAMBDAS, "varD"), read_set_buf(bs, MT5_LAMBDAS, "varCD")
+    # the pixel with the empty own interval: finite and as the oracle
+    assert np.allclose(gD[:, :, 10, 10], vD[:, :, 10, 10], rtol=FILTER_RTOL)
+    assert np.allclose(gCD[:, :, 10, 10], vCD[:, :, 10, 10], rtol=FILTER_RTOL)
+    # the pixel with the non-finite m2 keeps it (inf, as the oracle); where
+    # it is an accepted neighbour the oracle's sums become inf, the filter's
+    # take FLT_MAX times its weight (the record's clamp), so those pixels are
+    # left out of the comparison below
+    assert np.isinf(gD[2, 1, 20, 30]) and np.isinf(vD[2, 1, 20, 30])
+    # the pixel with n < 4: its own s^2, unfiltered
+    assert np.allclose(gD[:, :, 3, 5], st[:, 1, :, 3, 5] / 2.0, rtol=1e-6)
+    ok = np.isfinite(vD) & (np.abs(vD) < 1e37) & (amb == 0)
+    assert np.count_nonzero(ok) > 0.8 * ok.size
+    assert np.max(np.abs(gD[ok] - vD[ok]) / np.maximum(np.abs(vD[ok]), 1e-30)) < FILTER_RTOL
+    ok = np.isfinite(vCD) & (np.abs(vCD) < 1e37) & (amb == 0)
+    assert np.max(np.abs(gCD[ok] - vCD[ok]) / np.maximum(np.abs(vCD[ok]), 1e-30)) < FILTER_RTOL
