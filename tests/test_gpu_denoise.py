@@ -374,3 +374,51 @@ def test_variance_filter_edge_statistics(ctx):
     assert np.max(np.abs(gD[ok] - vD[ok]) / np.maximum(np.abs(vD[ok]), 1e-30)) < FILTER_RTOL
     ok = np.isfinite(vCD) & (np.abs(vCD) < 1e37) & (amb == 0)
     assert np.max(np.abs(gCD[ok] - vCD[ok]) / np.maximum(np.abs(vCD[ok]), 1e-30)) < FILTER_RTOL
+
+
+def test_welch_filter_edge_statistics(ctx):
+    """Welch tests on statistics written straight into a set: zero variances
+    (equal means accept, different means reject), a NaN mean (abstains),
+    pixels with one or no sample, and a non-finite variance."""
+    W, H, r = 40, 24, 4
+    rng = np.random.default_rng(23)
+    S = len(oracle.set_lambdas(MT5_LAMBDAS))
+    n = np.full((H, W), 20, np.int32)
+    n[2, 2], n[2, 3], n[5, 7] = 0, 1, 2
+    st = np.zeros((S, 7, 3, H, W), np.float32)
+    st[:, 0] = rng.normal(1.0, 0.05, (S, 3, H, W))
+    st[:, 1] = rng.uniform(0.5, 2.0, (S, 3, H, W))
+    st[:, 2] = rng.normal(0, 0.1, (S, 3, H, W))
+    st[:, 3] = rng.uniform(1.5, 4.0, (S, 3, H, W)) * st[:, 1] ** 2 / 20.0
+    st[:, 4:6] = rng.normal(0, 0.05, (S, 2, 3, H, W))
+    st[:, 6] = rng.uniform(0.5, 1.0, (S, 3, H, W))
+    st[:, 1, :, 8:10, 8:12] = 0.0           # zero variance, equal means in pairs of columns
+    st[:, 0, :, 8:10, 8:10] = 0.7
+    st[:, 0, :, 8:10, 10:12] = 0.9          # and different means next to them
+    st[:, 3, :, 8:10, 8:12] = 0.0
+    st[1, 0, 2, 15, 20] = np.nan            # a NaN mean abstains
+    st[3, 1, 0, 18, 30] = np.inf            # a non-finite variance
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, max_moment=5)
+    _write_stats(bs, MT5_LAMBDAS, n, st)
+    sc, chunk, feats = generate(ctx, 2, 3, W, H, 4)
+    p = sa.default_params(radius=r, mt_mode=2)
+    g = feats.gbuffers()
+    bs.denoise_vars(p, g)
+    bs.denoise_film(p, g)
+    ctx.synchronize()
+    varCD = read_set_buf(bs, MT5_LAMBDAS, "varCD")
+    f = feats.to_numpy()
+    feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    ref = oracle.denoise_film(n, st[:, 0].copy(), varCD, st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
+                              feats=feats_list, want_decisions=True)
+    dec = _decisions(ctx, bs, p, g, False, W, H, r)
+    diff = dec != ref["decisions"]
+    assert np.count_nonzero(diff) / dec.size <= DECISION_MISMATCH_MAX
+    # the zero-variance block: equal means accept each other, different means reject
+    assert dec[8, 8, r, r + 1].all()
+    assert not dec[8, 9, r, r + 1].any()
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
+    assert np.count_nonzero(same) > 0.9 * same.size
+    filmD = hipmem.read_plane(bs.view("filmD"))
+    fin = same & np.isfinite(ref["filmD"])
+    np.testing.assert_allclose(filmD[fin], ref["filmD"][fin], rtol=FILTER_RTOL)
