@@ -370,11 +370,6 @@ __device__ __forceinline__ void var_row(const VarArgs &a, const float4 *slot, in
     var_cols<G, HASW>(a, slot, tx, lwA, lwB, mA, mB, A, B);
 }
 
-// The pixel's own column as the pair loop left it, made what the filter
-// defines: the pixel itself always counts with weight 1 (the loop rejects it
-// only when its own interval is empty, z^2 se2 < 0), and with its exact s^2
-// (the record holds FLT_MAX for a non-finite one).  Rare: only pixel-channels
-// that var_own_suspect flags come here.
 // Whether the pixel's own column may need var_own_fix: the loop rejected it
 // (its own interval is empty), or its lnS2 is not finite, which the prepass
 // sets whenever m2 (and so the record's s^2 or m2C/(n-1)) is not finite.
@@ -383,6 +378,11 @@ __device__ __forceinline__ bool var_own_suspect(float Lp, float Zp) {
     return __builtin_fmaf(dl, dl, -Zp) > Zp || !__builtin_isfinite(Lp);
 }
 
+// The pixel's own column as the pair loop left it, made what the filter
+// defines: the pixel itself always counts with weight 1 (the loop rejects it
+// only when its own interval is empty, z^2 se2 < 0), and with its exact s^2
+// (the record holds FLT_MAX for a non-finite one).  Rare: only pixel-channels
+// that var_own_suspect flags come here.
 template <int G, bool HASW>
 __device__ __forceinline__ void var_own_fix(const VarArgs &a, size_t pix, int g, int c, float Lp, float Zp, float &den,
                                             float &num, float &numC) {
