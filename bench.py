@@ -331,8 +331,12 @@ def main() -> int:
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
     if "tlane_op_s" in dk:
+        # VALU-bound: neither the HBM nor the MFMA roof describes it (no dense contraction on this
+        # path); the same call's HBM figures are given beside it
         roofline = {"bound": "valu", "achieved": dk["tlane_op_s"], "peak": PEAK_LANE_OPS, "unit": "Tlane-op/s",
                     "frac": dk["valu_frac"], "traffic": traffic, "kernel": dom}
+        if "gb_s" in dk:
+            roofline["hbm"] = {"achieved": dk["gb_s"], "peak": PEAK_HBM, "unit": "GB/s", "frac": dk["hbm_frac"]}
     else:
         roofline = {"bound": "hbm", "achieved": dk.get("gb_s"), "peak": PEAK_HBM, "unit": "GB/s",
                     "frac": dk.get("hbm_frac"), "traffic": traffic, "kernel": dom}
