@@ -121,6 +121,17 @@ class DeviceBuffer:
         _sync()
 
 
+def sub_plane(pl, x0: int, y0: int, width: int, height: int):
+    """The window [x0, x0 + width) x [y0, y0 + height) of a plane, as a plane
+    over the same memory (same pitch and channel stride)."""
+    if x0 < 0 or y0 < 0 or x0 + width > pl.width or y0 + height > pl.height:
+        raise ValueError("sub_plane: window outside the plane")
+    out = type(pl).from_buffer_copy(pl)
+    out.ptr = pl.ptr + y0 * pl.pitch_bytes + x0 * 4
+    out.width, out.height = width, height
+    return out
+
+
 def read_plane(pl, dtype=None) -> np.ndarray:
     """Copy a planar device plane (stater Plane) to a numpy array [channels, H, W]."""
     dt = dtype or (np.int32 if pl.dtype == 1 else np.float32)

@@ -50,10 +50,19 @@ class SampleChunk:
             s.count_offset = count_offset
         return s
 
-    def to_numpy(self) -> np.ndarray:
-        """[spp, 5, H, W] float32."""
-        flat = self.buf.download(np.float32, self.plane_bytes * 5 * self.spp // 4)
-        return flat.reshape(self.spp, 5, self.height, self.ld)[..., : self.width].copy()
+    def to_numpy(self, rows: tuple[int, int] | None = None) -> np.ndarray:
+        """[spp, 5, H, W] float32, or only rows y0 <= y < y1 ([spp, 5, y1 - y0, W])."""
+        if rows is None:
+            flat = self.buf.download(np.float32, self.plane_bytes * 5 * self.spp // 4)
+            return flat.reshape(self.spp, 5, self.height, self.ld)[..., : self.width].copy()
+        y0, y1 = rows
+        out = np.empty((self.spp, 5, y1 - y0, self.width), np.float32)
+        for j in range(self.spp):
+            for k in range(5):
+                off = (j * 5 + k) * self.plane_bytes + y0 * self.ld * 4
+                part = self.buf.download(np.float32, (y1 - y0) * self.ld, offset=off)
+                out[j, k] = part.reshape(y1 - y0, self.ld)[:, : self.width]
+        return out
 
     def from_numpy(self, arr: np.ndarray) -> None:
         a = np.zeros((self.spp, 5, self.height, self.ld), np.float32)
