@@ -255,8 +255,9 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by image "
-              f"and by row bands, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); on several GPUs the path "
+              f"would shard by image and by row bands with an r-row halo, one process per GPU",
+              file=sys.stderr)
         return 2
 
     if args.workload == "c4":
