@@ -35,6 +35,7 @@
 namespace stater {
 
 constexpr int TX = 64, kTY = 4;  // output tile: 64 columns x kTY rows (one wavefront per row)
+constexpr int kDecideTY = 4;     // rows of the Welch decision pass's tiles (2, 6 and 8 measured slower)
 constexpr float kLog2e = 1.4426950408889634f;
 
 
@@ -1535,20 +1536,21 @@ static int accum2_launch(const WelchArgs &a, hipStream_t st) {
 
 template <int S>
 static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
-    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = kTY;
+    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = kTY, DTY = kDecideTY;
     const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
-    size_t lds = (size_t)TY * DecideLayout<S>::FLOATS * HWC * 4;
-    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, TY>,
+    const dim3 dgrid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + DTY - 1) / DTY)));
+    size_t lds = (size_t)DTY * DecideLayout<S>::FLOATS * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, DTY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (!a.dec && a.rec) {  // the decision pass writes the weighting pass's records itself
         WelchArgs b = a;
         b.rec_f4 = hasw ? AccRec<true>::FLOAT4S : AccRec<false>::FLOAT4S;
         b.rec_head = hasw ? AccRec<true>::HEAD4 : AccRec<false>::HEAD4;
-        hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, b);
+        hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY>), dgrid, dim3(64 * DTY), lds, st, b);
         STATER_HIP(hipGetLastError());
         return hasw ? accum2_launch<true>(b, st) : accum2_launch<false>(b, st);
     }
-    hipLaunchKernelGGL((k_welch_decide<S, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
+    hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY>), dgrid, dim3(64 * DTY), lds, st, a);
     STATER_HIP(hipGetLastError());
     if (hasw) {
         lds = (size_t)TY * AccumLayout<true>::FLOATS * HWC * 4;
