@@ -1764,12 +1764,21 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v
     return t;
 }
 
+// Blocks of the passes that sum over all pixels (error sums, the search's
+// counts): each thread walks the pixels with a grid stride and each block
+// leaves its partial sums in bsum, at most kAsCand per block (bsum holds
+// max(nb, 32) entries, so G kAsCand fits).  No atomics: a few hundred
+// partials per sum are added up by one workgroup.
+inline int as_sum_blocks(int nb) {
+    const int g = nb / kAsCand;
+    return g < 1 ? 1 : (g > 2048 ? 2048 : g);
+}
+
 __global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
     __shared__ unsigned long long sh[4];
     const int P = a.W * a.H;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long q = 0, h = 0;
-    if (p < P) {
+    unsigned long long sq = 0, shh = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
 #pragma clang fp contract(off)
         const int y = p / a.W, x = p - y * a.W;
         const size_t pix = (size_t)y * a.ld + x;
@@ -1786,21 +1795,37 @@ __global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
             e = (ec[0] + ec[1] + ec[2]) / 3.f;
         }
         const float qe = e * 65536.f;
-        q = !(qe > 0.f) ? 0ull : (qe >= 4294967295.f ? 4294967295ull : (unsigned long long)qe);
+        const unsigned long long q = !(qe > 0.f) ? 0ull : (qe >= 4294967295.f ? 4294967295ull : (unsigned long long)qe);
         const long long hh = (long long)a.cap - (long long)n;
-        h = hh > 0 ? (unsigned long long)hh : 0ull;
+        const unsigned long long h = hh > 0 ? (unsigned long long)hh : 0ull;
         a.q[p] = (uint32_t)q;
         a.h[p] = (int32_t)h;
+        sq += q;
+        shh += h;
     }
-    const unsigned long long sq = block_sum_u64(q, sh);
-    const unsigned long long shh = block_sum_u64(h, sh);
+    const unsigned long long bq = block_sum_u64(sq, sh);
+    const unsigned long long bh = block_sum_u64(shh, sh);
     if (threadIdx.x == 0) {
-        atomicAdd(&a.ctl->sumq, sq);
-        atomicAdd(&a.ctl->sumh, shh);
+        a.bsum[2 * blockIdx.x] = (int64_t)bq;
+        a.bsum[2 * blockIdx.x + 1] = (int64_t)bh;
     }
 }
 
-__global__ void k_as_init(AsCtl *c, unsigned long long B, int P) {
+// Sum of the G partials of sum k (stride `stride`, offset k) into thread 0.
+__device__ __forceinline__ unsigned long long as_partials(const int64_t *part, int G, int stride, int k,
+                                                          unsigned long long *sh) {
+    unsigned long long v = 0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) v += (unsigned long long)part[b * stride + k];
+    return block_sum_u64(v, sh);
+}
+
+__global__ void __launch_bounds__(256) k_as_init(AsCtl *c, const int64_t *part, int G, unsigned long long B, int P) {
+    __shared__ unsigned long long sh[4];
+    const unsigned long long sumq = as_partials(part, G, 2, 0, sh);
+    const unsigned long long sumh = as_partials(part, G, 2, 1, sh);
+    if (threadIdx.x != 0) return;
+    c->sumq = sumq;
+    c->sumh = sumh;
     c->B = B;
     for (int i = 0; i < kAsCand; ++i) c->f[i] = 0;
     if (c->sumh <= B) { c->phase = 3; return; }
@@ -1819,22 +1844,34 @@ __device__ __forceinline__ long long as_take(double lam, uint32_t q, int32_t h) 
 
 __global__ void __launch_bounds__(256) k_as_count(AsArgs a) {
     __shared__ unsigned long long sh[4];
-    AsCtl *c = a.ctl;
+    const AsCtl *c = a.ctl;
     if (c->phase >= 2) return;
     const int P = a.W * a.H;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t q = 0;
-    int32_t h = 0;
-    if (p < P) { q = c->uniform ? 1u : a.q[p]; h = a.h[p]; }
+    const bool uniform = c->uniform;
+    double cand[kAsCand];
+    unsigned long long f[kAsCand];
+#pragma unroll
+    for (int i = 0; i < kAsCand; ++i) { cand[i] = c->cand[i]; f[i] = 0; }
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const uint32_t q = uniform ? 1u : a.q[p];
+        const int32_t h = a.h[p];
+#pragma unroll
+        for (int i = 0; i < kAsCand; ++i) f[i] += (unsigned long long)as_take(cand[i], q, h);
+    }
+#pragma unroll
     for (int i = 0; i < kAsCand; ++i) {
-        const unsigned long long v = p < P ? (unsigned long long)as_take(c->cand[i], q, h) : 0ull;
-        const unsigned long long s = block_sum_u64(v, sh);
-        if (threadIdx.x == 0) atomicAdd(&c->f[i], s);
+        const unsigned long long s = block_sum_u64(f[i], sh);
+        if (threadIdx.x == 0) a.bsum[blockIdx.x * kAsCand + i] = (int64_t)s;
     }
 }
 
-__global__ void k_as_step(AsCtl *c) {
-    if (c->phase >= 2) return;
+__global__ void __launch_bounds__(256) k_as_step(AsCtl *c, const int64_t *part, int G) {
+    __shared__ unsigned long long sh[4];
+    if (c->phase >= 2) return;  // the same for every thread
+    unsigned long long fs[kAsCand];
+    for (int i = 0; i < kAsCand; ++i) fs[i] = as_partials(part, G, kAsCand, i, sh);
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < kAsCand; ++i) c->f[i] = fs[i];
     const unsigned long long B = c->B;
     if (c->phase == 0) {
         int j = -1;
@@ -1963,13 +2000,15 @@ int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const st
     LaunchTimer t(bs->ctx, st, "target_spp");
     int rc = launch_zero(a.ctl, sizeof(AsCtl), st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_as_err, dim3(nb), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_as_init, dim3(1), dim3(1), 0, st, a.ctl, (unsigned long long)budget * (unsigned long long)P, P);
+    const int G = as_sum_blocks(nb);
+    hipLaunchKernelGGL(k_as_err, dim3(G), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_as_init, dim3(1), dim3(256), 0, st, a.ctl, (const int64_t *)a.bsum, G,
+                       (unsigned long long)budget * (unsigned long long)P, P);
     // Each count/step pair narrows [lo, hi] 17-fold (bracketing first); 40 pairs
     // exceed what a double can resolve, and finished searches skip the work.
     for (int it = 0; it < 40; ++it) {
-        hipLaunchKernelGGL(k_as_count, dim3(nb), dim3(256), 0, st, a);
-        hipLaunchKernelGGL(k_as_step, dim3(1), dim3(1), 0, st, a.ctl);
+        hipLaunchKernelGGL(k_as_count, dim3(G), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_as_step, dim3(1), dim3(256), 0, st, a.ctl, (const int64_t *)a.bsum, G);
     }
     hipLaunchKernelGGL(k_as_inc, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_as_scan_blocks, dim3(1), dim3(256), 0, st, a.bsum, nb);
