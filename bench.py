@@ -205,7 +205,7 @@ def bench_c3(args) -> int:
     from stater_amd.progressive import run_rounds
     from stater_amd.workload import MT5_LAMBDAS, Features, SampleChunk, make_scene, synth_features
 
-    W, H, rounds, cap, pass_spp = 1920, 1080, 8, 4096, 64
+    W, H, rounds, cap, pass_spp = 1920, 1080, 8, 4096, args.pass_spp
     ctx = sa.Context(0)
     sc = make_scene(2, 4, W, H)
     feats = Features(W, H)
@@ -307,6 +307,7 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=32, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="4,8,16", help="c5: offered arrival rates, jobs/s")
     ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
+    ap.add_argument("--pass-spp", type=int, default=256, help="c3: samples per pixel of one accumulation pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); on several GPUs the path "
