@@ -132,7 +132,7 @@ def bench_c4(args) -> int:
     from stater_amd.serve import job_lambdas
     from stater_amd.workload import FEATURE_SDS, Features, SampleChunk, make_scene, synth_features, synth_samples
 
-    chunk_spp = 32
+    chunk_spp = args.chunk_spp
     ctx = sa.Context(0)
     p = sa.default_params(radius=args.radius)
     imgs = []
@@ -307,7 +307,7 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=32, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="4,8,16", help="c5: offered arrival rates, jobs/s")
     ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
-    ap.add_argument("--chunk-spp", type=int, default=64, help="c5: samples per pixel of one accumulation pass")
+    ap.add_argument("--chunk-spp", type=int, default=64, help="c4, c5: samples per pixel of one accumulation pass")
     ap.add_argument("--pass-spp", type=int, default=256, help="c3: samples per pixel of one accumulation pass")
     args = ap.parse_args()
     if args.gpus != 1:
