@@ -7,7 +7,9 @@ target number of further samples (the sampler's SetSPP/SetSampleNumber of
 statpath.cpp:253-256), continuing its own sample sequence.  Here the samples
 come from the seeded generator and the statistics never leave the device:
 each round is consumed in passes of ``pass_spp`` samples through the
-per-pixel ``counts``/``count_offset`` of the sample chunk.
+per-pixel ``counts``/``count_offset`` of the sample chunk.  Every pass reads
+and writes the state of each pixel it gives samples, so passes are large by
+default (256 spp: a 10.6 GB chunk at 1080p).
 """
 from __future__ import annotations
 
@@ -22,7 +24,7 @@ from .workload import Features, SampleChunk, synth_samples
 
 def run_rounds(ctx: Context, bs: BufferSet, scene, feats: Features | None, params: DenoiseParams,
                gbufs: GBuffers | None, rounds: int = 8, first_spp: int = 32, budget_spp: int = 32,
-               cap: int = 4096, pass_spp: int = 64, mode: int = STATER_ACC_SEQUENTIAL,
+               cap: int = 4096, pass_spp: int = 256, mode: int = STATER_ACC_SEQUENTIAL,
                chunk: SampleChunk | None = None, denoise_last: bool = True) -> list[dict]:
     """Runs the rounds; returns per-round {round, samples, max_target, ms}."""
     W, H = bs.width, bs.height
