@@ -119,7 +119,7 @@ TCritPoly make_tcrit_poly(double alpha) {
     tp.alpha = alpha;
     // Measure the FP32 evaluation error against the FP64 tail on a grid and
     // size the exact-tail band well above it.
-    double maxerr = 0;
+    double maxerr = 0, cmax = 0;
     for (int i = 0; i <= 1000; ++i) {
         const double u = umax * i / 1000.0;
         const float uf = (float)u;
@@ -127,10 +127,13 @@ TCritPoly make_tcrit_poly(double alpha) {
         for (int k = D - 1; k >= 0; --k) acc = std::fma(acc, uf, tp.c[k]);
         const double ref = u == 0.0 ? normal_t2_crit(alpha) : t2_crit_host(1.0 / u, alpha);
         maxerr = std::max(maxerr, std::fabs((double)acc / ref - 1.0));
+        cmax = std::max(cmax, std::max((double)acc, ref));
     }
     tp.band = (float)std::max(2e-6, 8.0 * maxerr + 2e-6);
     tp.neg_hi = -(1.f + tp.band);
     tp.neg_lo = -(1.f - tp.band);
+    // crit rises with u and is flat between grid points to far below 1 %
+    tp.band_abs = (float)((double)tp.band * cmax * 1.01);
     return tp;
 }
 

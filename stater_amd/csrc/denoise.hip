@@ -613,6 +613,28 @@ struct PairR {
 
 __device__ __forceinline__ PairR pair_r(float rp, float rq) { return PairR{rp, rq, rp + rq, -2.f * rq, rq}; }
 
+// t2 = dm^2 / V and the fitted critical value at u = 1/nu (clamped to umax)
+// of one FP32 Welch test.
+template <bool SMALLN>
+__device__ __forceinline__ void welch_t2_crit(const TCritPoly &tp, float mp, float vp, float mq, float vq,
+                                              const PairR &r, float &t2, float &crit, float &u) {
+    const float V = vp + vq;
+    const float iV = __builtin_amdgcn_rcpf(V);
+    const float dm = mp - mq;
+    t2 = (dm * iV) * dm;
+    const float ap = vp * iV;
+    if (SMALLN) {
+        const float aq = vq * iV;
+        u = __builtin_fmaf(ap * ap, r.rp, (aq * aq) * r.rq);
+    } else {
+        u = __builtin_fmaf(__builtin_fmaf(r.A, ap, r.B), ap, r.C);
+    }
+    const float uc = fminf(u, tp.umax);  // NaN (V = 0) -> umax
+    crit = tp.c[kTCritDeg];
+#pragma unroll
+    for (int k = kTCritDeg - 1; k >= 0; --k) crit = __builtin_fmaf(crit, uc, tp.c[k]);
+}
+
 // Decision margins of one FP32 Welch test, t2 = dm^2/V against crit(u):
 //   e > 0: t2 > (1 + band) crit(u), u <= umax -> rejected beyond doubt;
 //   f < 0: t2 < (1 - band) crit(u)            -> accepted beyond doubt
@@ -624,25 +646,22 @@ __device__ __forceinline__ PairR pair_r(float rp, float rq) { return PairR{rp, r
 template <bool SMALLN>
 __device__ __forceinline__ void welch_margin(const TCritPoly &tp, float mp, float vp, float mq, float vq,
                                              const PairR &r, float &e, float &f) {
-    const float V = vp + vq;
-    const float iV = __builtin_amdgcn_rcpf(V);
-    const float dm = mp - mq;
-    const float t2 = (dm * iV) * dm;
-    const float ap = vp * iV;
-    float u;
-    if (SMALLN) {
-        const float aq = vq * iV;
-        u = __builtin_fmaf(ap * ap, r.rp, (aq * aq) * r.rq);
-    } else {
-        u = __builtin_fmaf(__builtin_fmaf(r.A, ap, r.B), ap, r.C);
-    }
-    const float uc = fminf(u, tp.umax);  // NaN (V = 0) -> umax
-    float crit = tp.c[kTCritDeg];
-#pragma unroll
-    for (int k = kTCritDeg - 1; k >= 0; --k) crit = __builtin_fmaf(crit, uc, tp.c[k]);
+    float t2, crit, u;
+    welch_t2_crit<SMALLN>(tp, mp, vp, mq, vq, r, t2, crit, u);
     e = __builtin_fmaf(crit, tp.neg_hi, t2);
     f = __builtin_fmaf(crit, tp.neg_lo, t2);
     if (SMALLN) e = __builtin_fmaf(-kUBig, fmaxf(u - tp.umax, 0.f), e);
+}
+
+// Both margins of the large-n case as one number, g = t2 - crit(u): g >
+// band_abs (band times the largest crit on [0, umax]) implies e > 0 and g <
+// -band_abs implies f < 0.  Within a factor 2 of each other t2 and crit
+// subtract exactly, so g is exact wherever it is compared with +-band_abs.
+__device__ __forceinline__ float welch_gap(const TCritPoly &tp, float mp, float vp, float mq, float vq,
+                                           const PairR &r) {
+    float t2, crit, u;
+    welch_t2_crit<false>(tp, mp, vp, mq, vq, r, t2, crit, u);
+    return t2 - crit;
 }
 
 // Per-pixel test operands: M (mean; NaN = abstain) and V = var/n (discr).
@@ -977,15 +996,28 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
         float E[3], F[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
+        if (SMALLN) {
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float e, f;
+                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
+                    E[c] = fmaxf(E[c], e);
+                    F[c] = fmaxf(F[c], f);
+                }
+        } else {  // one running maximum per channel, the band applied once
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    E[c] = fmaxf(E[c], welch_gap(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr));
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                float e, f;
-                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
-                E[c] = fmaxf(E[c], e);
-                F[c] = fmaxf(F[c], f);
+                F[c] = E[c] + a.tp.band_abs;
+                E[c] = E[c] - a.tp.band_abs;
             }
+        }
         bool near = false;
 #pragma unroll
         for (int c = 0; c < 3; ++c) near |= !(E[c] > 0.f) && F[c] >= 0.f;

@@ -63,6 +63,7 @@ struct TCritPoly {
     float umax;
     float band;              // relative half-width of the exact-tail band
     float neg_hi, neg_lo;    // -(1 + band), -(1 - band)
+    float band_abs;          // band x the largest crit on [0, umax]: t2 - crit beyond +-band_abs is decided
     double alpha;
 };
 
