@@ -664,6 +664,42 @@ __device__ __forceinline__ float welch_gap(const TCritPoly &tp, float mp, float 
     return t2 - crit;
 }
 
+// The fast margins of one pair per channel, max-reduced over the NT tested
+// sets: E[c] > 0 rejected beyond doubt, F[c] < 0 accepted beyond doubt.  With
+// every n - 1 of the window large, one running maximum of g per channel with
+// the band applied once (welch_gap); otherwise both margins (welch_margin).
+// The decision pass keeps its own copy of this loop, which schedules two
+// instructions per column shorter there.
+template <int NT, bool SMALLN>
+__device__ __forceinline__ void welch_fast(const TCritPoly &tp, const float (&Mp)[NT][3], const float (&Vp)[NT][3],
+                                           const float2 (&mv)[NT][3], const PairR &pr, float (&E)[3],
+                                           float (&F)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
+    if (SMALLN) {
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float e, f;
+                welch_margin<SMALLN>(tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
+                E[c] = fmaxf(E[c], e);
+                F[c] = fmaxf(F[c], f);
+            }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NT; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                E[c] = fmaxf(E[c], welch_gap(tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            F[c] = E[c] + tp.band_abs;
+            E[c] = E[c] - tp.band_abs;
+        }
+    }
+}
+
 // Per-pixel test operands: M (mean; NaN = abstain) and V = var/n (discr).
 __device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, int s, int c, float &M, float &V) {
     const size_t pix = (size_t)yq * a.ld + xq;
@@ -794,17 +830,7 @@ __device__ __forceinline__ void welch_row(const WelchArgs &a, const float *slot,
             w *= __builtin_amdgcn_exp2f(lw);
         }
         float E[3], F[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
-#pragma unroll
-        for (int s = 0; s < NT; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float e, f;
-                welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
-                E[c] = fmaxf(E[c], e);
-                F[c] = fmaxf(F[c], f);
-            }
+        welch_fast<NT, SMALLN>(a.tp, Mp, Vp, mv, pr, E, F);
         // A channel neither rejected nor accepted beyond doubt by its fast
         // tests is decided by the exact tails of its tests.
         bool near = false;
