@@ -85,7 +85,7 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
     feats = [(f["albedo"], 0.02), (f["normal"], 0.1), (f["depth"], 0.5)]
     t0 = time.perf_counter()
     n, st = oracle.accumulate(smp, lambdas, 5, sanitize=True, nthreads=nthreads)
-    film = oracle.film_resolve(oracle.film_add(smp, sanitize=True))
+    film = oracle.film_image(oracle.film_add(smp, sanitize=True))
     t1 = time.perf_counter()
     jb, lnS2, se2, m2C = oracle.prepass(n, st, p.var_z)
     vD, vCD = oracle.denoise_vars(n, st[:, 1], lnS2.astype(np.float32), se2.astype(np.float32),

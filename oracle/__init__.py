@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
                                    C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, _f32p, _f64p, _vp, _vp, _vp, C.c_int]
     L.orc_target_spp.argtypes = [C.c_int, _i32p, _f32p, _f32p, C.c_uint64, C.c_uint32, _i32p]
     L.orc_film_add.argtypes = [C.c_int, C.c_int, _f32p, C.c_int, _vp, C.c_int, _f32p]
+    L.orc_film_xyz.argtypes = [C.c_int, _f32p, _f32p]
     L.orc_film_resolve.argtypes = [C.c_int, _f32p, C.c_float, _f32p]
     _lib = L
     return L
@@ -135,24 +136,40 @@ def accumulate(samples_arr: np.ndarray, lambdas, mm: int, fp64: bool = False, co
     return n.reshape(H, W), st.reshape(S, 7, 3, H, W)
 
 
-def film_add(samples_arr: np.ndarray, xyzw: np.ndarray | None = None, counts=None, sanitize=False) -> np.ndarray:
-    """pbrt Film sums [4,H,W] (X, Y, Z, weight) after adding one call's
-    samples [spp,5,H,W] (src/core/film.h:152-190, film.cpp:136-149)."""
+def film_add(samples_arr: np.ndarray, sums: np.ndarray | None = None, counts=None, sanitize=False) -> np.ndarray:
+    """pbrt Film running sums [4,H,W] (contribSum R, G, B and the weight sum)
+    after adding one call's samples [spp,5,H,W] in sample order
+    (src/core/film.h:152-190; the tiles live for the whole render,
+    src/statistics/statpath.cpp:144)."""
     spp, _, H, W = samples_arr.shape
     P = W * H
-    out = np.zeros(4 * P, np.float32) if xyzw is None else np.ascontiguousarray(xyzw, np.float32).reshape(-1).copy()
+    out = np.zeros(4 * P, np.float32) if sums is None else np.ascontiguousarray(sums, np.float32).reshape(-1).copy()
     cnt = None if counts is None else np.ascontiguousarray(counts.reshape(P), np.int32)
     lib().orc_film_add(W, H, np.ascontiguousarray(samples_arr.reshape(spp, 5, P), np.float32), spp,
                        None if cnt is None else cnt.ctypes.data, int(sanitize), out)
     return out.reshape(4, H, W)
 
 
+def film_xyz(sums: np.ndarray) -> np.ndarray:
+    """The Film's pixels [4,H,W] (X, Y, Z, weight) after Film::Clear and one
+    MergeFilmTile (src/core/film.cpp:127-149)."""
+    _, H, W = sums.shape
+    out = np.zeros(4 * W * H, np.float32)
+    lib().orc_film_xyz(W * H, np.ascontiguousarray(sums, np.float32).reshape(-1), out)
+    return out.reshape(4, H, W)
+
+
 def film_resolve(xyzw: np.ndarray, scale: float = 1.0) -> np.ndarray:
-    """Film::UpdateImage (src/core/film.cpp:188-220): RGB [3,H,W]."""
+    """Film::UpdateImage (src/core/film.cpp:188-220) of the Film's pixels: RGB [3,H,W]."""
     _, H, W = xyzw.shape
     out = np.zeros(3 * W * H, np.float32)
     lib().orc_film_resolve(W * H, np.ascontiguousarray(xyzw, np.float32).reshape(-1), scale, out)
     return out.reshape(3, H, W)
+
+
+def film_image(sums: np.ndarray, scale: float = 1.0) -> np.ndarray:
+    """The image of an iteration from the running sums: Clear, merge, UpdateImage."""
+    return film_resolve(film_xyz(sums), scale)
 
 
 def moments_exact(samples_arr: np.ndarray, lambdas, counts=None, sanitize=False) -> np.ndarray:

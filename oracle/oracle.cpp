@@ -507,38 +507,48 @@ ORC_API void orc_accumulate_f32(int W, int H, int S, const float *lambdas, int m
 }
 
 // ---- Film: the image denoiseFilm filters --------------------------------
-// pbrt-v3's Film fed one call's samples with the box filter of radius 1/2:
-// FilmTile::AddSample (src/core/film.h:152-190) adds L * 1 * 1 to the pixel's
-// contribSum in sample order and 1 to its weight sum; Film::MergeFilmTile
-// (src/core/film.cpp:136-149) adds RGBToXYZ(contribSum) and the weight sum
-// (RGBToXYZ as written at src/core/spectrum.h:72-76).  Samples exactly on a
+// pbrt-v3's Film with the box filter of radius 1/2, as the render loop drives
+// it.  The FilmTiles are created once per render loop
+// (src/statistics/statpath.cpp:144), so a pixel's contribSum and weight sum
+// run over all its samples, across iterations: FilmTile::AddSample
+// (src/core/film.h:152-190) adds L * 1 * 1 and 1 in sample order.  Each
+// iteration then runs Film::Clear (src/core/film.cpp:127-134, statpath.cpp:176)
+// and merges every tile once (Film::MergeFilmTile, film.cpp:136-149,
+// statpath.cpp:388): the Film pixel becomes 0 + RGBToXYZ(contribSum)
+// (src/core/spectrum.h:72-76) and 0 + the weight sum.  Samples exactly on a
 // pixel's left or top edge count in their own pixel only (the box filter
-// would add them to the neighbour across that edge too).  xyzw: [4][P].
+// would add them to the neighbour across that edge too).
+// sums: [4][P] = contribSum R, G, B and the weight sum, continued by each call.
 ORC_API void orc_film_add(int W, int H, const float *samples, int spp, const int32_t *counts, int san,
-                          float *xyzw) {
+                          float *sums) {
     const size_t P = (size_t)W * H;
     for (size_t p = 0; p < P; ++p) {
         const int cnt = counts ? std::min(counts[p], spp) : spp;
-        if (cnt <= 0) continue;
-        float sum[3] = {0.f, 0.f, 0.f};
         for (int j = 0; j < cnt; ++j) {
             const float *sj = samples + (size_t)j * 5 * P + p;
             float rgb[3] = {sj[0], sj[P], sj[2 * P]};
             if (san) sanitize(rgb[0], rgb[1], rgb[2]);
-            for (int c = 0; c < 3; ++c) sum[c] += rgb[c];
+            for (int c = 0; c < 3; ++c) sums[(size_t)c * P + p] += rgb[c] * 1.f * 1.f;
+            sums[3 * P + p] += 1.f;
         }
-        const float X = 0.412453f * sum[0] + 0.357580f * sum[1] + 0.180423f * sum[2];
-        const float Y = 0.212671f * sum[0] + 0.715160f * sum[1] + 0.072169f * sum[2];
-        const float Z = 0.019334f * sum[0] + 0.119193f * sum[1] + 0.950227f * sum[2];
-        xyzw[p] += X;
-        xyzw[P + p] += Y;
-        xyzw[2 * P + p] += Z;
-        xyzw[3 * P + p] += (float)cnt;
+    }
+}
+
+// The Film's pixels after Film::Clear and one MergeFilmTile per pixel:
+// xyzw = (0 + X, 0 + Y, 0 + Z, 0 + weight) from the running sums.
+ORC_API void orc_film_xyz(int P, const float *sums, float *xyzw) {
+    for (int p = 0; p < P; ++p) {
+        const float r = sums[p], g = sums[(size_t)P + p], b = sums[2 * (size_t)P + p];
+        const float xyz[3] = {0.412453f * r + 0.357580f * g + 0.180423f * b,
+                              0.212671f * r + 0.715160f * g + 0.072169f * b,
+                              0.019334f * r + 0.119193f * g + 0.950227f * b};
+        for (int i = 0; i < 3; ++i) xyzw[(size_t)i * P + p] = 0.f + xyz[i];
+        xyzw[3 * (size_t)P + p] = 0.f + sums[3 * (size_t)P + p];
     }
 }
 
 // Film::UpdateImage (src/core/film.cpp:188-220) without splats: XYZToRGB
-// (src/core/spectrum.h:66-70), times 1/weight and clamped at 0 when the
+// (src/core/spectrum.h:66-70), times 1/weight and std::max(0, v) when the
 // weight sum is not 0, times scale.  rgb: [3][P].
 ORC_API void orc_film_resolve(int P, const float *xyzw, float scale, float *rgb) {
     for (int p = 0; p < P; ++p) {
@@ -789,16 +799,18 @@ ORC_API void orc_denoise_film(int W, int H, int S, const int32_t *n, const float
     });
 }
 
-// Adaptive-sampling map (integer exact, see the product's description):
-// e_p = mean_c sqrt(varD_c/n)/max(|mean_c|,1e-3) in FP32, q_p = e_p*2^16 as
-// integer, water-filled with caps n+target <= cap so that the targets sum to
-// budget*P, remainders handed out in pixel order.
+// Adaptive-sampling map (integer exact).  e_p = mean_c sqrt(varD_c/n)/max(|mean_c|,1e-3)
+// in FP32, q_p = floor(e_p 2^16) as integer, water-filled with caps
+// n + target <= cap so that the targets sum to budget*P, remainders handed
+// out in pixel order.  If the pixels with q > 0 cannot take the budget
+// (their headroom sums to <= budget*P), they take all of it and the rest is
+// shared uniformly (weight 1) over the q = 0 pixels; with every q = 0 that
+// is the uniform distribution.
 ORC_API void orc_target_spp(int P, const int32_t *n, const float *varD, const float *mean, uint64_t budget_spp,
                             uint32_t cap, int32_t *target) {
     std::vector<uint64_t> q(P);
     std::vector<int64_t> h(P);
-    uint64_t sq = 0;
-    int64_t sh = 0;
+    int64_t sh = 0, shq = 0;
     for (int p = 0; p < P; ++p) {
         float e = 0.f;
         if (n[p] > 0) {
@@ -813,25 +825,31 @@ ORC_API void orc_target_spp(int P, const int32_t *n, const float *varD, const fl
         }
         const float qe = e * 65536.f;
         q[p] = !(qe > 0.f) ? 0ull : (qe >= 4294967295.f ? 4294967295ull : (uint64_t)qe);
-        sq += q[p];
         h[p] = std::max<int64_t>(0, (int64_t)cap - (int64_t)n[p]);
         sh += h[p];
+        if (q[p]) shq += h[p];
     }
     const uint64_t B = budget_spp * (uint64_t)P;
     if ((uint64_t)sh <= B) { for (int p = 0; p < P; ++p) target[p] = (int32_t)h[p]; return; }
-    if (sq == 0) { for (int p = 0; p < P; ++p) q[p] = 1; sq = P; }
+    if (B == 0) { for (int p = 0; p < P; ++p) target[p] = 0; return; }
+    const bool uniform = (uint64_t)shq <= B;
+    auto share = [&](double lam, int p) -> int64_t {
+        if (uniform && q[p]) return h[p];
+        const double t = std::floor(lam * (double)(uniform ? 1ull : q[p]));
+        return t >= (double)h[p] ? h[p] : (int64_t)t;
+    };
     auto f = [&](double lam, int32_t *out) -> uint64_t {
         uint64_t tot = 0;
         for (int p = 0; p < P; ++p) {
-            const double t = std::floor(lam * (double)q[p]);
-            const int64_t ti = t >= (double)h[p] ? h[p] : (int64_t)t;
+            const int64_t ti = share(lam, p);
             if (out) out[p] = (int32_t)ti;
             tot += (uint64_t)ti;
         }
         return tot;
     };
-    double lo = 0.0, hi = (double)B / (double)sq;
-    while (f(hi, nullptr) <= B) hi *= 2.0;
+    // f rises to sh > B (uniform) or to shq > B, so the doubling ends.
+    double lo = 0.0, hi = 1.0;
+    while (f(hi, nullptr) <= B) { lo = hi; hi *= 2.0; }
     for (int it = 0; it < 4000; ++it) {  // until lo and hi are adjacent doubles
         const double mid = 0.5 * (lo + hi);
         if (mid <= lo || mid >= hi) break;

@@ -189,14 +189,23 @@ __device__ __forceinline__ float load_sample(const char *base, int64_t off, int6
     return keep_sample(r, g, b) ? v : 0.f;
 }
 
-// This call's film contribution of a pixel-channel: the FilmTile sum of its
-// samples (FilmTile::AddSample, src/core/film.h:152-190: box filter of radius
-// 1/2, weight 1, so contribSum += L in sample order) and, from the R lane,
-// the weight count.  Merged into the Film's XYZ sums by k_film_merge.
+// The Film of the render (STATER_BUFSET_FILM): pbrt creates the FilmTiles
+// once per render loop (src/statistics/statpath.cpp:144) and every sample
+// adds L * 1 * 1 to its pixel's contribSum and 1 to its weight sum
+// (FilmTile::AddSample, src/core/film.h:152-190, box filter of radius 1/2),
+// so both sums run over all the pixel's samples in order, across calls.
+// film_load gives the running sum a call continues from (0 for a fresh call),
+// film_store writes it back with the weight sum from the R lane.
+__device__ __forceinline__ float film_load(const AccArgs &a, int c, uint32_t pix) {
+    return (a.film_sum && !a.fresh) ? a.film_sum[(size_t)c * a.plane + pix] : 0.f;
+}
 __device__ __forceinline__ void film_store(const AccArgs &a, int c, uint32_t pix, float sum, int cnt) {
-    if (!a.film_chunk) return;
-    a.film_chunk[(size_t)c * a.plane + pix] = sum;
-    if (c == 0) a.film_chunk[3 * a.plane + pix] = (float)cnt;
+    if (!a.film_sum) return;
+    a.film_sum[(size_t)c * a.plane + pix] = sum;
+    if (c == 0) {
+        float *w = a.film_sum + 3 * a.plane + pix;
+        *w = (a.fresh ? 0.f : *w) + (float)cnt;  // exact: the weights are 1 and the sum stays below 2^24
+    }
 }
 
 // A fresh call leaves what stater_bufset_reset leaves in the statistic slots
@@ -221,11 +230,13 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
                            a.count_offset);
     if (cnt <= 0) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) film_store(a, c, pix, 0.f, 0);
         if (a.fresh) {
             a.n[pix] = 0;
-            for (int c = 0; c < 3; ++c) clear_slots(a, T + 1, c, 0, pix);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                clear_slots(a, T + 1, c, 0, pix);
+                film_store(a, c, pix, 0.f, 0);
+            }
         }
         return;
     }
@@ -268,7 +279,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         }
     }
 
-    float fsum[3] = {0.f, 0.f, 0.f};
+    float fsum[3] = {film_load(a, 0, pix), film_load(a, 1, pix), film_load(a, 2, pix)};
     // The next sample's loads are issued before this sample's arithmetic.
     auto load = [&](int j, float (&r)[5]) {
         const char *sj = base + (int64_t)j * a.sstride;
@@ -361,10 +372,10 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         cnt = min(cnt, *(const int32_t *)((const char *)a.counts + (int64_t)y * a.counts_pitch + x * 4) -
                            a.count_offset);
     if (cnt <= 0) {
-        film_store(a, c, pix, 0.f, 0);
         if (a.fresh) {
             if (c == 0) a.n[pix] = 0;
             clear_slots(a, T + 1, c, 0, pix);
+            film_store(a, c, pix, 0.f, 0);
         }
         return;
     }
@@ -402,7 +413,7 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         mg[s] = st[s][0];
     }
 
-    float fsum = 0.f;  // the Film tile's sum of this pixel-channel's samples
+    float fsum = film_load(a, c, pix);  // the Film's running sum of this pixel-channel's samples
     for (int j = 0; j < cnt; ++j) {
         const int64_t off = (int64_t)j * a.sstride;
         const float v = load_sample(base, off, a.comp, c, a.sanitize);
@@ -472,50 +483,33 @@ __global__ void k_zero(uint32_t *p, size_t words) {
         p[i] = 0u;
 }
 
-// Film::MergeFilmTile (src/core/film.cpp:136-149): the Film's XYZ sums take
-// this call's tile sum through RGBToXYZ (src/core/spectrum.h:72-76, evaluated
-// as written) and its weight sum the sample count.  fresh: the sums start
-// from zero (as after a reset).
-__global__ void __launch_bounds__(256) k_film_merge(const float *chunk, float *xyzw, int W, int ld, size_t plane,
-                                                    int fresh) {
-#pragma clang fp contract(off)
-    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
-    if (x >= W) return;
-    const size_t pix = (size_t)y * ld + x;
-    const float cnt = chunk[3 * plane + pix];
-    if (!(cnt > 0.f)) {
-        if (fresh)
-            for (int k = 0; k < 4; ++k) xyzw[k * plane + pix] = 0.f;
-        return;
-    }
-    const float r = chunk[pix], g = chunk[plane + pix], b = chunk[2 * plane + pix];
-    const float X = 0.412453f * r + 0.357580f * g + 0.180423f * b;
-    const float Y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
-    const float Z = 0.019334f * r + 0.119193f * g + 0.950227f * b;
-    const float *old = fresh ? nullptr : xyzw;
-    xyzw[pix] = (old ? old[pix] : 0.f) + X;
-    xyzw[plane + pix] = (old ? old[plane + pix] : 0.f) + Y;
-    xyzw[2 * plane + pix] = (old ? old[2 * plane + pix] : 0.f) + Z;
-    xyzw[3 * plane + pix] = (old ? old[3 * plane + pix] : 0.f) + cnt;
-}
-
-// Film::UpdateImage (src/core/film.cpp:188-220) without splats: XYZToRGB
-// (src/core/spectrum.h:66-70), divided by the weight sum and clamped at 0,
-// times scale.
-__global__ void __launch_bounds__(256) k_film_resolve(const float *xyzw, int W, int ld, size_t plane, float scale,
+// The Film image after an iteration: Film::Clear and one MergeFilmTile per
+// pixel (src/core/film.cpp:127-149, called at statpath.cpp:176,388) leave
+// xyz = 0 + RGBToXYZ(contribSum) (src/core/spectrum.h:72-76) and the weight
+// sum; Film::UpdateImage (film.cpp:188-220, no splats) then takes XYZToRGB
+// (spectrum.h:66-70), times 1/weight and clamped at 0 (std::max(0, v): +0
+// for -0 and NaN) when the weight is not 0, times scale.  Every operation is
+// evaluated as written, uncontracted.
+__global__ void __launch_bounds__(256) k_film_resolve(const float *sum, int W, int ld, size_t plane, float scale,
                                                       float *out, int64_t out_pitch, int64_t out_cs, int out_il) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
     if (x >= W) return;
     const size_t pix = (size_t)y * ld + x;
-    const float X = xyzw[pix], Y = xyzw[plane + pix], Z = xyzw[2 * plane + pix], w = xyzw[3 * plane + pix];
+    const float r = sum[pix], g = sum[plane + pix], b = sum[2 * plane + pix], w = sum[3 * plane + pix];
+    const float X = 0.f + (0.412453f * r + 0.357580f * g + 0.180423f * b);
+    const float Y = 0.f + (0.212671f * r + 0.715160f * g + 0.072169f * b);
+    const float Z = 0.f + (0.019334f * r + 0.119193f * g + 0.950227f * b);
     float rgb[3];
     rgb[0] = 3.240479f * X - 1.537150f * Y - 0.498535f * Z;
     rgb[1] = -0.969256f * X + 1.875991f * Y + 0.041556f * Z;
     rgb[2] = 0.055648f * X - 0.204043f * Y + 1.057311f * Z;
     if (w != 0.f) {
         const float inv = 1.f / w;
-        for (int c = 0; c < 3; ++c) rgb[c] = fmaxf(0.f, rgb[c] * inv);
+        for (int c = 0; c < 3; ++c) {
+            const float v = rgb[c] * inv;
+            rgb[c] = 0.f < v ? v : 0.f;
+        }
     }
     char *row = (char *)out + (int64_t)y * out_pitch;
     for (int c = 0; c < 3; ++c) {
@@ -524,16 +518,9 @@ __global__ void __launch_bounds__(256) k_film_resolve(const float *xyzw, int W, 
     }
 }
 
-int launch_film_merge(stater_bufset *bs, bool fresh, hipStream_t st) {
-    hipLaunchKernelGGL(k_film_merge, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_chunk,
-                       bs->film_xyzw, bs->W, bs->ld, bs->plane, fresh ? 1 : 0);
-    STATER_HIP(hipGetLastError());
-    return STATER_OK;
-}
-
 int launch_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, hipStream_t st) {
     LaunchTimer t(bs->ctx, st, "film_resolve");
-    hipLaunchKernelGGL(k_film_resolve, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_xyzw, bs->W,
+    hipLaunchKernelGGL(k_film_resolve, dim3((bs->W + 255) / 256, bs->H), dim3(256), 0, st, bs->film_sum, bs->W,
                        bs->ld, bs->plane, scale, (float *)out->ptr, out->pitch_bytes, out->channel_stride_bytes,
                        out->layout == STATER_INTERLEAVED ? 1 : 0);
     STATER_HIP(hipGetLastError());
@@ -588,7 +575,7 @@ AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s) {
         a.lam[k] = k < bs->S ? bs->set_lambda[k] : 1.f;
         a.inv_lam[k] = 1.f / a.lam[k];
     }
-    a.film_chunk = bs->film_chunk;
+    a.film_sum = bs->film_sum;
     return a;
 }
 
@@ -610,7 +597,7 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
         default: launch_t<5>(bs->mm, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
-    return bs->film_xyzw ? launch_film_merge(bs, a.fresh != 0, st) : STATER_OK;
+    return STATER_OK;
 }
 
 }  // namespace stater

@@ -287,7 +287,7 @@ static int make_layout(int W, int H, int n_lambdas, const float *lambdas, int mm
     L.plane = (size_t)L.ld * H;
     // n + moments (all 7 slots) + Box-Cox reference point + shifted Box-Cox means
     size_t np = 1 + (size_t)L.S * N_STATS * 3 + 3 + (size_t)(L.S - 1) * 3;
-    if (flags & STATER_BUFSET_FILM) np += 3 + 4 + 4;  // film, XYZ + weight sums, per-call sums
+    if (flags & STATER_BUFSET_FILM) np += 3 + 4;  // resolved film, RGB contribSum + weight sum
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
         np += (size_t)L.S * (1 + 6 * 3) + 9 + 1 + 3 * kBitsWords + kVarRecFloats;
         L.as_bytes = as_scratch_bytes_for((size_t)W * H);
@@ -325,8 +325,7 @@ static stater_bufset *bufset_over(stater_ctx *ctx, const Layout &L, int width, i
     bs->mshift = p; p += (size_t)(L.S - 1) * 3 * L.plane;
     if (flags & STATER_BUFSET_FILM) {
         bs->film_rgb = p; p += 3 * L.plane;
-        bs->film_xyzw = p; p += 4 * L.plane;
-        bs->film_chunk = p; p += 4 * L.plane;
+        bs->film_sum = p; p += 4 * L.plane;
     }
     if (!(flags & STATER_BUFSET_NO_DERIVED)) {
         bs->jb = p; p += (size_t)L.S * L.plane;
@@ -405,7 +404,7 @@ int stater_bufset_reset(stater_bufset *bs, void *stream) {
     if (!bs) return fail(STATER_E_ARG, "stater_bufset_reset: NULL set");
     const size_t bytes = (1 + (size_t)bs->S * N_STATS * 3) * bs->plane * 4;
     int rc = launch_zero(bs->base, bytes, pick_stream(bs, stream));
-    if (!rc && bs->film_xyzw) rc = launch_zero(bs->film_xyzw, 4 * bs->plane * 4, pick_stream(bs, stream));
+    if (!rc && bs->film_sum) rc = launch_zero(bs->film_sum, 4 * bs->plane * 4, pick_stream(bs, stream));
     return rc;
 }
 
@@ -471,7 +470,7 @@ int stater_accumulate(stater_bufset *bs, const stater_samples *samples, int mode
 
 int stater_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, void *stream) {
     if (!bs) return fail(STATER_E_ARG, "stater_film_resolve: NULL set");
-    if (!bs->film_xyzw) return fail(STATER_E_STATE, "stater_film_resolve: set was created without STATER_BUFSET_FILM");
+    if (!bs->film_sum) return fail(STATER_E_STATE, "stater_film_resolve: set was created without STATER_BUFSET_FILM");
     stater_plane own;
     if (!out) {
         fill_plane(bs, bs->film_rgb, 3, STATER_F32, &own);
@@ -482,10 +481,13 @@ int stater_film_resolve(stater_bufset *bs, float scale, const stater_plane *out,
     return launch_film_resolve(bs, scale, out, pick_stream(bs, stream));
 }
 
+// The reference's integrator defaults (CreateStatPathIntegrator,
+// src/statistics/statpath.cpp:1070-1071: filterSD 10, "filterradius" 20) and
+// Estimator constants (z = 2.80703 = Phi^-1(1 - 0.005/2), estimator.h:353).
 void stater_default_params(stater_denoise_params *p) {
     if (!p) return;
     std::memset(p, 0, sizeof(*p));
-    p->radius = 10;
+    p->radius = 20;
     p->ds_factor = -0.5f / (10.f * 10.f);
     p->var_z = 2.80703f;
     p->alpha = 0.005f;

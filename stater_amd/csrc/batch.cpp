@@ -164,7 +164,7 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
         I.has_g = im.gbufs != nullptr;
         if (I.has_g) I.g = *im.gbufs;
         I.spp = im.spp;
-        I.has_film = im.set->film_xyzw != nullptr;
+        I.has_film = im.set->film_sum != nullptr;
         if (I.has_film) (void)stater_bufset_view(im.set, "film", &I.film);
         I.ty_var = kVarTileRows;
         I.ty_welch = 4;

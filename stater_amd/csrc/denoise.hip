@@ -1781,10 +1781,15 @@ int launch_mean_vars(stater_bufset *bs, hipStream_t st) {
 // result does not depend on the reduction order.  A search over lam in FP64
 // finds lo < hi with f(lo) <= B < f(hi); targets are min(floor(lo q), h) plus
 // the first R = B - f(lo) of the increments to hi, in pixel order.
+// When the pixels with q > 0 cannot take the budget (sum of their h <= B,
+// e.g. a flat background beside pixels near the cap), f saturates below B:
+// those pixels then take their whole headroom and the rest of the budget is
+// shared uniformly over the q = 0 pixels (weight 1 each), which is also the
+// rule when every q is 0.
 constexpr int kAsCand = 16;
 
 struct AsCtl {
-    unsigned long long sumq, sumh;
+    unsigned long long sumq, sumh, sumhq;
     double lo, hi;
     double cand[kAsCand];
     unsigned long long f[kAsCand];
@@ -1835,7 +1840,7 @@ inline int as_sum_blocks(int nb) {
 __global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
     __shared__ unsigned long long sh[4];
     const int P = a.W * a.H;
-    unsigned long long sq = 0, shh = 0;
+    unsigned long long sq = 0, shh = 0, shq = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
 #pragma clang fp contract(off)
         const int y = p / a.W, x = p - y * a.W;
@@ -1860,12 +1865,15 @@ __global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
         a.h[p] = (int32_t)h;
         sq += q;
         shh += h;
+        shq += q ? h : 0ull;
     }
     const unsigned long long bq = block_sum_u64(sq, sh);
     const unsigned long long bh = block_sum_u64(shh, sh);
+    const unsigned long long bhq = block_sum_u64(shq, sh);
     if (threadIdx.x == 0) {
-        a.bsum[2 * blockIdx.x] = (int64_t)bq;
-        a.bsum[2 * blockIdx.x + 1] = (int64_t)bh;
+        a.bsum[3 * blockIdx.x] = (int64_t)bq;
+        a.bsum[3 * blockIdx.x + 1] = (int64_t)bh;
+        a.bsum[3 * blockIdx.x + 2] = (int64_t)bhq;
     }
 }
 
@@ -1879,15 +1887,17 @@ __device__ __forceinline__ unsigned long long as_partials(const int64_t *part, i
 
 __global__ void __launch_bounds__(256) k_as_init(AsCtl *c, const int64_t *part, int G, unsigned long long B, int P) {
     __shared__ unsigned long long sh[4];
-    const unsigned long long sumq = as_partials(part, G, 2, 0, sh);
-    const unsigned long long sumh = as_partials(part, G, 2, 1, sh);
+    const unsigned long long sumq = as_partials(part, G, 3, 0, sh);
+    const unsigned long long sumh = as_partials(part, G, 3, 1, sh);
+    const unsigned long long sumhq = as_partials(part, G, 3, 2, sh);
     if (threadIdx.x != 0) return;
     c->sumq = sumq;
     c->sumh = sumh;
+    c->sumhq = sumhq;
     c->B = B;
     for (int i = 0; i < kAsCand; ++i) c->f[i] = 0;
     if (c->sumh <= B) { c->phase = 3; return; }
-    c->uniform = c->sumq == 0;
+    c->uniform = c->sumhq <= B;
     const double sq = c->uniform ? (double)P : (double)c->sumq;
     c->lo = 0.0;
     c->hi = (double)B / sq;
@@ -1898,6 +1908,13 @@ __global__ void __launch_bounds__(256) k_as_init(AsCtl *c, const int64_t *part, 
 __device__ __forceinline__ long long as_take(double lam, uint32_t q, int32_t h) {
     const double t = floor(lam * (double)q);
     return t >= (double)h ? (long long)h : (long long)t;
+}
+
+// A pixel's share at lam: in the uniform rule the q > 0 pixels are saturated
+// and every q = 0 pixel has weight 1.
+__device__ __forceinline__ long long as_share(double lam, bool uniform, uint32_t q, int32_t h) {
+    if (uniform) return q ? (long long)h : as_take(lam, 1u, h);
+    return as_take(lam, q, h);
 }
 
 __global__ void __launch_bounds__(256) k_as_count(AsArgs a) {
@@ -1911,10 +1928,10 @@ __global__ void __launch_bounds__(256) k_as_count(AsArgs a) {
 #pragma unroll
     for (int i = 0; i < kAsCand; ++i) { cand[i] = c->cand[i]; f[i] = 0; }
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-        const uint32_t q = uniform ? 1u : a.q[p];
+        const uint32_t q = a.q[p];
         const int32_t h = a.h[p];
 #pragma unroll
-        for (int i = 0; i < kAsCand; ++i) f[i] += (unsigned long long)as_take(cand[i], q, h);
+        for (int i = 0; i < kAsCand; ++i) f[i] += (unsigned long long)as_share(cand[i], uniform, q, h);
     }
 #pragma unroll
     for (int i = 0; i < kAsCand; ++i) {
@@ -1974,9 +1991,9 @@ __global__ void __launch_bounds__(256) k_as_inc(AsArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     long long inc = 0;
     if (p < P && c->phase != 3) {
-        const uint32_t q = c->uniform ? 1u : a.q[p];
+        const uint32_t q = a.q[p];
         const int32_t h = a.h[p];
-        inc = as_take(c->hi, q, h) - as_take(c->lo, q, h);
+        inc = as_share(c->hi, c->uniform, q, h) - as_share(c->lo, c->uniform, q, h);
         a.inc[p] = (int32_t)inc;
     }
     const unsigned long long s = block_sum_u64((unsigned long long)inc, sh);
@@ -2026,8 +2043,7 @@ __global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
     const int y = p / a.W, x = p - y * a.W;
     int32_t *o = (int32_t *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4);
     if (c->phase == 3) { *o = a.h[p]; return; }
-    const uint32_t q = c->uniform ? 1u : a.q[p];
-    const int64_t base = as_take(c->lo, q, a.h[p]);
+    const int64_t base = as_share(c->lo, c->uniform, a.q[p], a.h[p]);
     const int64_t R = (int64_t)(c->B - c->flo);
     const int64_t E = a.bsum[blockIdx.x] + tmp[threadIdx.x] - inc;  // exclusive prefix
     int64_t add = R - E;

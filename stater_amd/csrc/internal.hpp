@@ -44,9 +44,9 @@ struct stater_bufset {
     float *stats = nullptr;  // [S][7][3][plane]
     float *lref = nullptr;   // [3][plane] Box-Cox reference point ln x1 (sets 1..)
     float *mshift = nullptr; // [S-1][3][plane] Box-Cox means relative to the reference point
-    float *film_rgb = nullptr;    // STATER_BUFSET_FILM: [3][plane] resolved film
-    float *film_xyzw = nullptr;   // [4][plane] the Film's XYZ sums and weight sum
-    float *film_chunk = nullptr;  // [4][plane] the last call's RGB sums and sample count
+    float *film_rgb = nullptr;  // STATER_BUFSET_FILM: [3][plane] resolved film
+    float *film_sum = nullptr;  // [4][plane] the Film's per-pixel RGB contribSum and weight sum,
+                                // carried across calls (FilmTile pixels of the whole render)
     float *jb = nullptr;     // [S][plane]
     float *lnS2 = nullptr, *se2 = nullptr, *m2C = nullptr, *varD = nullptr, *varCD = nullptr,
           *discr = nullptr;  // [S][3][plane]
@@ -120,7 +120,6 @@ int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const st
                       hipStream_t st);
 int launch_mean_vars(stater_bufset *bs, hipStream_t st);
 int launch_zero(void *p, size_t bytes, hipStream_t st);
-int launch_film_merge(stater_bufset *bs, bool fresh, hipStream_t st);
 int launch_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, hipStream_t st);
 int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                            const stater_plane *film, int plain, uint8_t *dec, hipStream_t st);

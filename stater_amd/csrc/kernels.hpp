@@ -25,7 +25,7 @@ struct AccArgs {
     int W, H;
     float lam[kMaxSets];
     float inv_lam[kMaxSets];
-    float *film_chunk;  // [4][plane] this call's film RGB sums and sample count, or null
+    float *film_sum;    // [4][plane] the Film's RGB contribSum and weight sum (continued), or null
     int fresh;          // start from an empty set: old n and statistics are neither read nor kept
 };
 

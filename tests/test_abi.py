@@ -109,7 +109,8 @@ def test_bufset_bytes_for_rejects(lib, W, H, lams, mm, msg):
 def test_default_params_match_reference_values(lib):
     p = sa.DenoiseParams()
     lib.stater_default_params(C.byref(p))
-    assert p.radius == 10 and p.mt_mode == 2 and p.cascade == 1 and p.use_levc == 1
+    # statpath.cpp:1071 (filterradius 20), scenes/render-denoise-mt5.pbrt:19
+    assert p.radius == 20 and p.mt_mode == 2 and p.cascade == 1 and p.use_levc == 1
     assert np.isclose(p.var_z, 2.80703) and np.isclose(p.alpha, 0.005)
     assert np.isclose(p.ds_factor, -0.5 / 100.0)
 
@@ -145,7 +146,7 @@ def test_server_default_config(lib):
     lib.stater_server_default_config(C.byref(c))
     assert c.slots >= 1 and c.chunk_spp >= 1 and c.max_bytes >= 1 << 30
     assert (c.max_width, c.max_height) == (3840, 2160)
-    assert np.allclose(list(c.feature_sds), [0.02, 0.1, 0.5]) and c.params.radius == 10
+    assert np.allclose(list(c.feature_sds), [0.02, 0.1, 0.5]) and c.params.radius == 20
 
 
 def test_scene_init_is_deterministic(lib):

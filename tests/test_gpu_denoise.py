@@ -315,6 +315,32 @@ def test_target_spp_exact(ctx, budget, cap):
         assert np.array_equal(tgt, head)
 
 
+@pytest.mark.parametrize("budget,cap", [(32, 4096), (4, 64), (0, 4096)])
+def test_target_spp_zero_error_pixels(ctx, budget, cap):
+    """A flat background (varD = 0) beside noisy pixels one sample below the
+    cap: the GPU map equals the oracle's and meets the budget exactly."""
+    W, H = 150, 40
+    rng = np.random.default_rng(12)
+    bs = sa.BufferSet(ctx, W, H, (), 5)
+    n = np.full((H, W), 16, np.int32)
+    noisy = rng.random((H, W)) < 0.002
+    n[noisy] = cap - 1
+    varD = np.zeros((3, H, W), np.float32)
+    varD[:, noisy] = rng.lognormal(0, 1, (3, int(noisy.sum()))).astype(np.float32)
+    mean = np.full((3, H, W), 0.25, np.float32)
+    hipmem.write_plane(bs.view("n"), n[None])
+    hipmem.write_plane(bs.view("t0-b0-film-varD"), varD)
+    hipmem.write_plane(bs.view("t0-b0-film-mean"), mean)
+    bs.calc_target_spp(budget, cap)
+    ctx.synchronize()
+    tgt = hipmem.read_plane(bs.view("targetspp"))[0]
+    ref = oracle.target_spp(n, varD, mean, budget, cap)
+    assert np.array_equal(tgt, ref)
+    assert int(tgt.sum()) == budget * W * H
+    if budget:
+        assert np.array_equal(tgt[noisy], (cap - n)[noisy])
+
+
 def _write_stats(bs, lambdas, n, st):
     """Load n [H,W] and statistics [S,7,3,H,W] into a set through its views."""
     from gpu_helpers import STAT_NAMES, set_slots

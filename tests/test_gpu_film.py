@@ -38,7 +38,7 @@ def test_film_matches_pbrt_film(ctx, mode):
     ref = oracle.film_add(smp[:30], counts=counts, sanitize=True)
     ref = oracle.film_add(smp[30:48], ref, sanitize=True)
     film = hipmem.read_plane(bs.view("film"))
-    assert np.array_equal(_bits(film), _bits(oracle.film_resolve(ref)))
+    assert np.array_equal(_bits(film), _bits(oracle.film_image(ref)))
     assert np.array_equal(hipmem.read_plane(bs.view("t0-b0-film-n"))[0], counts + 18)
 
 
@@ -51,7 +51,7 @@ def test_resolve_scale_interleaved_and_reset(ctx):
     inter = sa.plane(buf.ptr, W, H, 3, 3 * 64 * 4, 4, layout=sa.STATER_INTERLEAVED)
     bs.film_resolve(2.0, inter)
     ctx.synchronize()
-    ref = oracle.film_resolve(oracle.film_add(chunk.to_numpy(), sanitize=True), 2.0)
+    ref = oracle.film_image(oracle.film_add(chunk.to_numpy(), sanitize=True), 2.0)
     got = buf.download(np.float32, 3 * 64 * H).reshape(H, 3 * 64)[:, : 3 * W].reshape(H, W, 3).transpose(2, 0, 1)
     assert np.array_equal(_bits(got), _bits(ref))
     bs.reset()
@@ -61,3 +61,28 @@ def test_resolve_scale_interleaved_and_reset(ctx):
     # a set without the flag has no Film: "film" is the set-0 mean and resolving is refused
     plain = sa.BufferSet(ctx, W, H, (), 2)
     assert sa.load_library().stater_film_resolve(plain.h, 1.0, None, None) == sa.STATER_E_STATE
+
+
+@pytest.mark.parametrize("mode", [sa.STATER_ACC_SEQUENTIAL, sa.STATER_ACC_MERGE64])
+def test_frame_in_three_calls_equals_one_call(ctx, mode):
+    """A Film fed as 16 + 16 + 16 samples per pixel is bit-identical to the
+    one fed 48 in a single call, and to the oracle's pbrt Film (the tiles'
+    sums run over the whole render: statpath.cpp:144,176,388)."""
+    W, H = 72, 40
+    _, chunk, _ = generate(ctx, 2, 12, W, H, 48)
+    smp = chunk.to_numpy()
+    one = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5, flags=sa.STATER_BUFSET_FILM)
+    three = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5, flags=sa.STATER_BUFSET_FILM)
+    one.accumulate(chunk.desc(sanitize=True), mode)
+    base = chunk.desc()
+    for k in range(3):
+        d = chunk.desc(spp=16, sanitize=True)
+        d.data = base.data + 16 * k * base.sample_stride_bytes
+        three.accumulate(d, mode)
+    one.film_resolve()
+    three.film_resolve()
+    ctx.synchronize()
+    f1 = hipmem.read_plane(one.view("film"))
+    f3 = hipmem.read_plane(three.view("film"))
+    assert np.array_equal(_bits(f1), _bits(f3))
+    assert np.array_equal(_bits(f1), _bits(oracle.film_image(oracle.film_add(smp, sanitize=True))))

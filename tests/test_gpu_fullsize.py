@@ -1,9 +1,12 @@
-"""The headline frame at its full size (1920 x 1080, mt5, r = 10, guarded
-samples and the pbrt Film as in the bench step, 64 spp to keep the sample
-chunk small): the statistics and the Film bit for bit on a band of rows, the
+"""The headline frame at its full size and sample count (1920 x 1080,
+256 spp, mt5, r = 10, guarded samples and the pbrt Film as in the bench
+step): the statistics and the Film bit for bit on a band of rows, the
 prepass, the variance filter and the Welch filter against the oracle on an
 interior window (whose outputs depend only on data inside the window plus
-the radius), and size-independent properties on the whole frame."""
+the radius) with every Welch decision of the window dumped and compared, and
+size-independent properties on the whole frame."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -15,8 +18,9 @@ from stater_amd.workload import FEATURE_SDS, MT5_LAMBDAS
 
 pytestmark = pytest.mark.gpu
 
-W, H, SPP, R = 1920, 1080, 64, 10
+W, H, SPP, R = 1920, 1080, 256, 10
 FILTER_RTOL = 1e-4
+DECISION_MISMATCH_MAX = 1e-4   # fraction of neighbour decisions that may differ (0.01%)
 
 
 def _window(bs, name, x0, y0, w, h):
@@ -34,6 +38,20 @@ def _stats_window(bs, x0, y0, w, h):
 def _set_window(bs, name, x0, y0, w, h):
     return np.stack([_window(bs, (f"t0-b{j}-film-{name}" if s == 0 else f"t0-b{j}-{name}"), x0, y0, w, h)
                      for s, j in enumerate(set_slots(MT5_LAMBDAS))])
+
+
+def _decision_rows(ctx, bs, p, g, film, y0, y1):
+    """Every Welch decision of rows y0 <= y < y1 of the full-frame filter
+    (stater_debug_film_decisions over the whole frame), [y1-y0, W, win, win, 3]."""
+    win = 2 * R + 1
+    row = W * win * win * 3
+    buf = hipmem.DeviceBuffer(H * row)
+    sa.check(sa.load_library().stater_debug_film_decisions(bs.h, C.byref(p), C.byref(g), C.byref(film), 0,
+                                                           C.c_void_p(buf.ptr), None))
+    ctx.synchronize()
+    out = buf.download(np.uint8, (y1 - y0) * row, offset=y0 * row).reshape(y1 - y0, W, win, win, 3)
+    buf.free()
+    return out
 
 
 def test_full_frame(ctx):
@@ -63,7 +81,7 @@ def test_full_frame(ctx):
     st = _stats_window(bs, 0, y0, W, y1 - y0)
     assert np.array_equal(n[y0:y1], n_ref)
     assert np.array_equal(st[0].view(np.uint32), st_ref[0].view(np.uint32))
-    film_ref = oracle.film_resolve(oracle.film_add(smp, sanitize=True))
+    film_ref = oracle.film_image(oracle.film_add(smp, sanitize=True))
     assert np.array_equal(film[:, y0:y1].view(np.uint32), film_ref.view(np.uint32))
 
     # an interior window: the filters against the oracle on the window's own
@@ -88,10 +106,18 @@ def test_full_frame(ctx):
         assert np.count_nonzero(ok) > 0.9 * ok.size
         rel = np.abs(gv[inner][ok] - rv[inner][ok]) / np.maximum(np.abs(rv[inner][ok]), 1e-30)
         assert rel.max() < FILTER_RTOL
+    ambf = amb[inner].sum() / (float(amb[inner].size) * (2 * R + 1) ** 2)
+    assert ambf <= DECISION_MISMATCH_MAX, ambf
     ref = oracle.denoise_film(nw, stw[:, 0].copy(), gCD, film[:, yw:yw + h, x0:x0 + w], R, p.ds_factor, p.alpha,
-                              feats=fl)
+                              feats=fl, want_decisions=True)
+    dec = _decision_rows(ctx, bs, p, g, bs.view("film"), yw + R, yw + h - R)[:, x0 + R:x0 + w - R]
+    diff = dec != ref["decisions"][R:h - R, R:w - R]
+    frac = np.count_nonzero(diff) / diff.size
+    assert frac <= DECISION_MISMATCH_MAX, frac
+    # every pixel-channel whose decisions all agree is within the tolerance
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
     got = filmD[:, yw + R:yw + h - R, x0 + R:x0 + w - R]
     want = ref["filmD"][:, R:h - R, R:w - R]
     rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-6)
-    # pixels with a Welch decision on the FP32 threshold may differ; nearly all agree
-    assert np.count_nonzero(rel < FILTER_RTOL) > 0.99 * rel.size
+    assert np.count_nonzero(same) > 0.99 * same.size
+    assert rel[same].max() < FILTER_RTOL, rel[same].max()

@@ -288,6 +288,33 @@ def test_target_spp_invariants(budget, cap):
         assert np.array_equal(t, head)
 
 
+@pytest.mark.parametrize("n_noisy,n_flat,budget,cap", [(4, 60, 32, 4096), (10, 300, 4, 64), (0, 50, 8, 100),
+                                                         (3, 40, 0, 4096)])
+def test_target_spp_zero_error_pixels(n_noisy, n_flat, budget, cap):
+    """Pixels with zero error (a flat background, varD = 0) beside noisy
+    pixels close to the cap: the noisy pixels' headroom is smaller than the
+    budget, so they take all of it and the flat pixels share the rest evenly
+    (their targets differ by at most 1); the budget is met exactly."""
+    rng = np.random.default_rng(11)
+    P = n_noisy + n_flat
+    H, W = 1, P
+    n = np.full((H, W), 16, np.int32)
+    varD = np.zeros((3, H, W), np.float32)
+    mean = np.full((3, H, W), 0.5, np.float32)
+    noisy = rng.permutation(P)[:n_noisy]
+    n[0, noisy] = cap - 1
+    varD[:, 0, noisy] = rng.lognormal(0, 1, (3, n_noisy)).astype(np.float32)
+    t = oracle.target_spp(n, varD, mean, budget, cap)[0]
+    head = np.maximum(cap - n[0], 0)
+    assert np.all(t >= 0) and np.all(t <= head)
+    assert int(t.sum()) == min(budget * P, int(head.sum()))
+    assert np.array_equal(t[noisy], head[noisy]) or budget == 0
+    flat = np.setdiff1d(np.arange(P), noisy)
+    free = flat[t[flat] < head[flat]]
+    if free.size:
+        assert t[free].max() - t[free].min() <= 1
+
+
 def test_scene_struct_matches_library():
     """The library's scene parameters (host code) equal the oracle's; the
     trigonometric fields may differ by an ulp between the two compilers' libm
