@@ -274,6 +274,22 @@ int stater_denoise_plain(stater_bufset *bs, int radius, float alpha, void *strea
  * out: NULL = the set's targetspp plane. */
 int stater_calc_target_spp(stater_bufset *bs, uint64_t budget_spp, uint32_t cap,
                            const stater_plane *out, void *stream);
+/* The adaptive-sampling map of stater_calc_target_spp on caller planes,
+ * allocating nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  n (i32, 1 channel), varD and mean
+ * (f32, 3 planar channels) and out (i32) are W x H device planes; the result
+ * equals stater_calc_target_spp on the same values.  Its scratch is a static
+ * device array per GPU (one control block and 256-pixel block partials,
+ * 512 KiB + 512 B, held until the process ends), whose address is taken when
+ * the context is created: films above STATER_NOALLOC_MAX_PIXELS are rejected
+ * with STATER_E_ARG.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (different streams or threads) race: the caller orders
+ * them. */
+#define STATER_NOALLOC_MAX_PIXELS (4096 * 4096)
+int stater_calc_target_spp_noalloc(stater_ctx *ctx, int width, int height, const stater_plane *n,
+                                   const stater_plane *varD, const stater_plane *mean,
+                                   uint64_t budget_spp, uint32_t cap, const stater_plane *out, void *stream);
 /* meanVar = m2 / ((n-1) n) of set 0 (estimator.cpp:462-474). */
 int stater_calc_mean_vars(stater_bufset *bs, void *stream);
 int stater_synchronize(void *stream);

@@ -130,6 +130,8 @@ SIGNATURES = [
                                       C.POINTER(Plane), _P]),
     ("stater_denoise_plain", C.c_int, [_P, C.c_int, C.c_float, _P]),
     ("stater_calc_target_spp", C.c_int, [_P, C.c_uint64, C.c_uint32, C.POINTER(Plane), _P]),
+    ("stater_calc_target_spp_noalloc", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Plane), C.POINTER(Plane),
+                                                 C.POINTER(Plane), C.c_uint64, C.c_uint32, C.POINTER(Plane), _P]),
     ("stater_calc_mean_vars", C.c_int, [_P, _P]),
     ("stater_synchronize", C.c_int, [_P]),
     ("stater_scene_init", C.c_int, [C.POINTER(Scene), C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int]),

@@ -169,6 +169,8 @@ int stater_create(int device, unsigned flags, stater_ctx **out) {
     STATER_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(STATER_E_ARG, "stater_create: no such device");
     STATER_HIP(hipSetDevice(device));
+    int rc = noalloc_scratch_init(device);  // the static scratch's address on this device
+    if (rc) return rc;
     auto *ctx = new stater_ctx();
     ctx->device = device;
     ctx->flags = flags;

@@ -1803,17 +1803,47 @@ struct AsCtl {
 struct AsArgs {
     const int32_t *n;
     const float *varD, *mean;
-    int W, H, ld;
-    size_t plane;
+    int64_t n_ld;           // row pitch of n, elements
+    int64_t v_ld, v_cs;     // varD: row pitch and channel stride, elements
+    int64_t m_ld, m_cs;     // mean: row pitch and channel stride, elements
+    int W, H;
     uint32_t cap;
-    uint32_t *q;
-    int32_t *h;
-    int32_t *inc;
+    uint32_t *q;            // per-pixel q and h kept by the error pass, or null: recomputed
+    int32_t *h;             // from n, varD and mean wherever they are needed
+    int32_t *inc;           // per-pixel increments lo -> hi, or null: recomputed
     int64_t *bsum;
     AsCtl *ctl;
     int32_t *out;
     int64_t out_pitch;
 };
+
+// q_p and h_p of pixel p (row-major index over W x H), in the fixed FP32
+// operation order; every pass that needs them gets the same values.
+__device__ __forceinline__ void as_qh(const AsArgs &a, int p, uint32_t &q, int32_t &h) {
+#pragma clang fp contract(off)
+    const int y = p / a.W, x = p - y * a.W;
+    const int n = a.n[(size_t)y * a.n_ld + x];
+    float e = 0.f;
+    if (n > 0) {
+        const float nf = (float)n;
+        float ec[3];
+        for (int c = 0; c < 3; ++c) {
+            const float vd = fmaxf(a.varD[c * a.v_cs + (size_t)y * a.v_ld + x], 0.f);
+            const float m = fmaxf(fabsf(a.mean[c * a.m_cs + (size_t)y * a.m_ld + x]), 1e-3f);
+            ec[c] = sqrtf(vd / nf) / m;  // correctly rounded (IEEE) square root
+        }
+        e = (ec[0] + ec[1] + ec[2]) / 3.f;
+    }
+    const float qe = e * 65536.f;
+    q = !(qe > 0.f) ? 0u : (qe >= 4294967295.f ? 4294967295u : (uint32_t)qe);
+    const long long hh = (long long)a.cap - (long long)n;
+    h = hh > 0 ? (int32_t)hh : 0;
+}
+
+__device__ __forceinline__ void as_get_qh(const AsArgs &a, int p, uint32_t &q, int32_t &h) {
+    if (a.q) { q = a.q[p]; h = a.h[p]; }
+    else as_qh(a, p, q, h);
+}
 
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long *sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -1842,30 +1872,13 @@ __global__ void __launch_bounds__(256) k_as_err(AsArgs a) {
     const int P = a.W * a.H;
     unsigned long long sq = 0, shh = 0, shq = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-#pragma clang fp contract(off)
-        const int y = p / a.W, x = p - y * a.W;
-        const size_t pix = (size_t)y * a.ld + x;
-        const int n = a.n[pix];
-        float e = 0.f;
-        if (n > 0) {
-            const float nf = (float)n;
-            float ec[3];
-            for (int c = 0; c < 3; ++c) {
-                const float vd = fmaxf(a.varD[c * a.plane + pix], 0.f);
-                const float m = fmaxf(fabsf(a.mean[c * a.plane + pix]), 1e-3f);
-                ec[c] = sqrtf(vd / nf) / m;  // correctly rounded (IEEE) square root
-            }
-            e = (ec[0] + ec[1] + ec[2]) / 3.f;
-        }
-        const float qe = e * 65536.f;
-        const unsigned long long q = !(qe > 0.f) ? 0ull : (qe >= 4294967295.f ? 4294967295ull : (unsigned long long)qe);
-        const long long hh = (long long)a.cap - (long long)n;
-        const unsigned long long h = hh > 0 ? (unsigned long long)hh : 0ull;
-        a.q[p] = (uint32_t)q;
-        a.h[p] = (int32_t)h;
+        uint32_t q;
+        int32_t h;
+        as_qh(a, p, q, h);
+        if (a.q) { a.q[p] = q; a.h[p] = h; }
         sq += q;
-        shh += h;
-        shq += q ? h : 0ull;
+        shh += (unsigned long long)h;
+        shq += q ? (unsigned long long)h : 0ull;
     }
     const unsigned long long bq = block_sum_u64(sq, sh);
     const unsigned long long bh = block_sum_u64(shh, sh);
@@ -1928,8 +1941,9 @@ __global__ void __launch_bounds__(256) k_as_count(AsArgs a) {
 #pragma unroll
     for (int i = 0; i < kAsCand; ++i) { cand[i] = c->cand[i]; f[i] = 0; }
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-        const uint32_t q = a.q[p];
-        const int32_t h = a.h[p];
+        uint32_t q;
+        int32_t h;
+        as_get_qh(a, p, q, h);
 #pragma unroll
         for (int i = 0; i < kAsCand; ++i) f[i] += (unsigned long long)as_share(cand[i], uniform, q, h);
     }
@@ -1991,10 +2005,11 @@ __global__ void __launch_bounds__(256) k_as_inc(AsArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     long long inc = 0;
     if (p < P && c->phase != 3) {
-        const uint32_t q = a.q[p];
-        const int32_t h = a.h[p];
+        uint32_t q;
+        int32_t h;
+        as_get_qh(a, p, q, h);
         inc = as_share(c->hi, c->uniform, q, h) - as_share(c->lo, c->uniform, q, h);
-        a.inc[p] = (int32_t)inc;
+        if (a.inc) a.inc[p] = (int32_t)inc;
     }
     const unsigned long long s = block_sum_u64((unsigned long long)inc, sh);
     if (threadIdx.x == 0) a.bsum[blockIdx.x] = (int64_t)s;
@@ -2029,8 +2044,12 @@ __global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
     const AsCtl *c = a.ctl;
     const int P = a.W * a.H;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t q = 0;
+    int32_t h = 0;
+    if (p < P) as_get_qh(a, p, q, h);
     int64_t inc = 0;
-    if (p < P && c->phase != 3) inc = a.inc[p];
+    if (p < P && c->phase != 3)
+        inc = a.inc ? a.inc[p] : as_share(c->hi, c->uniform, q, h) - as_share(c->lo, c->uniform, q, h);
     tmp[threadIdx.x] = inc;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {
@@ -2042,8 +2061,8 @@ __global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
     if (p >= P) return;
     const int y = p / a.W, x = p - y * a.W;
     int32_t *o = (int32_t *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4);
-    if (c->phase == 3) { *o = a.h[p]; return; }
-    const int64_t base = as_share(c->lo, c->uniform, a.q[p], a.h[p]);
+    if (c->phase == 3) { *o = h; return; }
+    const int64_t base = as_share(c->lo, c->uniform, q, h);
     const int64_t R = (int64_t)(c->B - c->flo);
     const int64_t E = a.bsum[blockIdx.x] + tmp[threadIdx.x] - inc;  // exclusive prefix
     int64_t add = R - E;
@@ -2051,27 +2070,11 @@ __global__ void __launch_bounds__(256) k_as_final(AsArgs a) {
     *o = (int32_t)(base + add);
 }
 
-int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out, hipStream_t st) {
-    const int P = bs->W * bs->H;
+// The AS map's launches.  a.bsum must hold max(nb, 32, as_sum_blocks(nb) *
+// kAsCand) int64 entries (nb = 256-pixel blocks) and a.ctl one control block.
+static int run_target_spp(const AsArgs &a, uint64_t budget, hipStream_t st) {
+    const int P = a.W * a.H;
     const int nb = (P + 255) / 256;
-    static_assert(sizeof(AsCtl) <= kAsCtlBytes, "AS control block outgrew its slot");
-    if (!bs->as_scratch || bs->as_scratch_bytes < as_scratch_bytes_for((size_t)P))
-        return fail(STATER_E_STATE, "stater_calc_target_spp: set has no adaptive-sampling scratch");
-    char *sp = (char *)bs->as_scratch;
-    AsArgs a{};
-    a.n = bs->n;
-    a.varD = bs->varD;
-    a.mean = bs->stat(0, ST_MEAN, 0);
-    a.W = bs->W; a.H = bs->H; a.ld = bs->ld; a.plane = bs->plane; a.cap = cap;
-    const size_t pb = ((size_t)P * 4 + 255) / 256 * 256;
-    a.ctl = (AsCtl *)sp; sp += kAsCtlBytes;
-    a.q = (uint32_t *)sp; sp += pb;
-    a.h = (int32_t *)sp; sp += pb;
-    a.inc = (int32_t *)sp; sp += pb;
-    a.bsum = (int64_t *)sp;
-    a.out = out ? (int32_t *)out->ptr : bs->targetspp;
-    a.out_pitch = out ? out->pitch_bytes : (int64_t)bs->ld * 4;
-    LaunchTimer t(bs->ctx, st, "target_spp");
     int rc = launch_zero(a.ctl, sizeof(AsCtl), st);
     if (rc) return rc;
     const int G = as_sum_blocks(nb);
@@ -2091,7 +2094,109 @@ int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const st
     return STATER_OK;
 }
 
+int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out, hipStream_t st) {
+    const int P = bs->W * bs->H;
+    static_assert(sizeof(AsCtl) <= kAsCtlBytes, "AS control block outgrew its slot");
+    if (!bs->as_scratch || bs->as_scratch_bytes < as_scratch_bytes_for((size_t)P))
+        return fail(STATER_E_STATE, "stater_calc_target_spp: set has no adaptive-sampling scratch");
+    char *sp = (char *)bs->as_scratch;
+    AsArgs a{};
+    a.n = bs->n;
+    a.varD = bs->varD;
+    a.mean = bs->stat(0, ST_MEAN, 0);
+    a.n_ld = a.v_ld = a.m_ld = bs->ld;
+    a.v_cs = a.m_cs = (int64_t)bs->plane;
+    a.W = bs->W; a.H = bs->H; a.cap = cap;
+    const size_t pb = ((size_t)P * 4 + 255) / 256 * 256;
+    a.ctl = (AsCtl *)sp; sp += kAsCtlBytes;
+    a.q = (uint32_t *)sp; sp += pb;
+    a.h = (int32_t *)sp; sp += pb;
+    a.inc = (int32_t *)sp; sp += pb;
+    a.bsum = (int64_t *)sp;
+    a.out = out ? (int32_t *)out->ptr : bs->targetspp;
+    a.out_pitch = out ? out->pitch_bytes : (int64_t)bs->ld * 4;
+    LaunchTimer t(bs->ctx, st, "target_spp");
+    return run_target_spp(a, budget, st);
+}
+
+// ---- the AS map without allocation ----------------------------------------
+// Device memory of stater_calc_target_spp_noalloc: one control block and the
+// block partials of a film of at most STATER_NOALLOC_MAX_PIXELS pixels
+// (65536 256-pixel blocks).  Each device holds its own copy for the life of
+// the process; hipGetSymbolAddress gives its address on the current device.
+constexpr size_t kNoallocBsum = STATER_NOALLOC_MAX_PIXELS / 256;
+static_assert(kNoallocBsum >= (size_t)2048 * kAsCand, "block partials of the count passes must fit");
+constexpr size_t kNoallocBytes = kAsCtlBytes + kNoallocBsum * sizeof(int64_t);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Called by stater_create with `device` current.
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(STATER_E_UNSUPPORTED, "stater_create: device index too large");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return STATER_OK;
+    void *ptr = nullptr;
+    STATER_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(STATER_E_HIP, "hipGetSymbolAddress returned no address for the AS scratch");
+    g_scratch_ptr[device] = ptr;
+    return STATER_OK;
+}
+
+static int check_plane(const stater_plane *pl, int W, int H, int ch, stater_dtype dt, const char *what) {
+    if (!pl || !pl->ptr || pl->width != W || pl->height != H || pl->channels != ch || pl->dtype != dt ||
+        pl->layout != STATER_PLANAR || pl->pitch_bytes < (int64_t)W * 4 || pl->pitch_bytes % 4 ||
+        (ch > 1 && pl->channel_stride_bytes % 4))
+        return fail(STATER_E_ARG, std::string("stater_calc_target_spp_noalloc: bad ") + what + " plane");
+    return STATER_OK;
+}
+
+int launch_target_spp_noalloc(stater_ctx *ctx, int W, int H, const stater_plane *n, const stater_plane *varD,
+                              const stater_plane *mean, uint64_t budget, uint32_t cap, const stater_plane *out,
+                              hipStream_t st) {
+    if (W <= 0 || H <= 0) return fail(STATER_E_ARG, "stater_calc_target_spp_noalloc: empty film");
+    if ((uint64_t)W * (uint64_t)H > STATER_NOALLOC_MAX_PIXELS)
+        return fail(STATER_E_ARG, "stater_calc_target_spp_noalloc: film larger than STATER_NOALLOC_MAX_PIXELS");
+    int rc;
+    if ((rc = check_plane(n, W, H, 1, STATER_I32, "n")) || (rc = check_plane(varD, W, H, 3, STATER_F32, "varD")) ||
+        (rc = check_plane(mean, W, H, 3, STATER_F32, "mean")) || (rc = check_plane(out, W, H, 1, STATER_I32, "out")))
+        return rc;
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return fail(STATER_E_STATE, "stater_calc_target_spp_noalloc: no scratch on the context's device");
+    AsArgs a{};
+    a.n = (const int32_t *)n->ptr;
+    a.n_ld = n->pitch_bytes / 4;
+    a.varD = (const float *)varD->ptr;
+    a.v_ld = varD->pitch_bytes / 4;
+    a.v_cs = varD->channel_stride_bytes / 4;
+    a.mean = (const float *)mean->ptr;
+    a.m_ld = mean->pitch_bytes / 4;
+    a.m_cs = mean->channel_stride_bytes / 4;
+    a.W = W; a.H = H; a.cap = cap;
+    a.ctl = (AsCtl *)base;
+    a.bsum = (int64_t *)((char *)base + kAsCtlBytes);
+    a.out = (int32_t *)out->ptr;
+    a.out_pitch = out->pitch_bytes;
+    return run_target_spp(a, budget, st);
+}
+
 }  // namespace stater
+
+extern "C" int stater_calc_target_spp_noalloc(stater_ctx *ctx, int width, int height, const stater_plane *n,
+                                              const stater_plane *varD, const stater_plane *mean,
+                                              uint64_t budget_spp, uint32_t cap, const stater_plane *out,
+                                              void *stream) {
+    if (!ctx) return stater::fail(STATER_E_ARG, "stater_calc_target_spp_noalloc: NULL context");
+    if (budget_spp > (uint64_t)1 << 32) return stater::fail(STATER_E_ARG, "stater_calc_target_spp_noalloc: budget too large");
+    return stater::launch_target_spp_noalloc(ctx, width, height, n, varD, mean, budget_spp, cap, out,
+                                             stream ? (hipStream_t)stream : ctx->stream);
+}
 
 extern "C" int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *p,
                                            const stater_gbuffers *gbufs, const stater_plane *film, int plain,

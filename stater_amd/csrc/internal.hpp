@@ -119,6 +119,12 @@ int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const
 int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out,
                       hipStream_t st);
 int launch_mean_vars(stater_bufset *bs, hipStream_t st);
+// The AS map over caller planes with the static scratch (stater_calc_target_spp_noalloc);
+// noalloc_scratch_init takes that scratch's address on the current device.
+int noalloc_scratch_init(int device);
+int launch_target_spp_noalloc(stater_ctx *ctx, int W, int H, const stater_plane *n, const stater_plane *varD,
+                              const stater_plane *mean, uint64_t budget, uint32_t cap, const stater_plane *out,
+                              hipStream_t st);
 int launch_zero(void *p, size_t bytes, hipStream_t st);
 int launch_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, hipStream_t st);
 int launch_welch_decisions(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
