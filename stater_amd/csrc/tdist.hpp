@@ -2,9 +2,9 @@
 //
 // The two-sided tail of t^2 with nu degrees of freedom is the regularised
 // incomplete beta I_x(nu/2, 1/2) with x = nu/(nu + t^2).  It is evaluated by
-// the modified Lentz continued fraction (Numerical Recipes, 3rd ed., 6.4) in
-// FP64.  The Welch kernel calls it only for pairs whose FP32 statistic lies in
-// a narrow band around the critical value; everywhere else it compares t^2
+// its continued fraction (DLMF 8.17.22, modified Lentz evaluation) in FP64.
+// The Welch kernel calls it only for pairs whose FP32 statistic lies in a
+// narrow band around the critical value; everywhere else it compares t^2
 // against a polynomial in u = 1/nu that the host fits to this function.
 #pragma once
 
@@ -13,36 +13,43 @@
 
 namespace stater {
 
-__host__ __device__ inline double betacf(double a, double b, double x) {
-    const double EPS = 1e-16, FPMIN = 1e-300;
-    const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
-    double c = 1.0, d = 1.0 - qab * x / qap;
-    if (fabs(d) < FPMIN) d = FPMIN;
-    d = 1.0 / d;
-    double h = d;
-    for (int m = 1; m <= 100000; ++m) {
-        const double m2 = 2.0 * m;
-        double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
-        d = 1.0 + aa * d; if (fabs(d) < FPMIN) d = FPMIN;
-        c = 1.0 + aa / c; if (fabs(c) < FPMIN) c = FPMIN;
-        d = 1.0 / d; h *= d * c;
-        aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
-        d = 1.0 + aa * d; if (fabs(d) < FPMIN) d = FPMIN;
-        c = 1.0 + aa / c; if (fabs(c) < FPMIN) c = FPMIN;
-        d = 1.0 / d;
-        const double del = d * c;
-        h *= del;
-        if (fabs(del - 1.0) < EPS) break;
+// The continued fraction of DLMF 8.17.22,
+//     I_x(a, b) = x^a (1 - x)^b / (a B(a, b)) / (1 + d_1/(1 + d_2/(1 + ...))),
+//     d_{2m}   =  m (b - m) x / ((a + 2m - 1)(a + 2m)),
+//     d_{2m+1} = -(a + m)(a + b + m) x / ((a + 2m)(a + 2m + 1)),
+// returned as 1/(1 + d_1/(1 + ...)).  The denominator is evaluated front to
+// back, one partial numerator d_j per step, by the modified Lentz method
+// (Thompson & Barnett, J. Comput. Phys. 64 (1986) 490): with C_0 = f_0 = 1,
+// D_0 = 0, C_j = 1 + d_j/C_{j-1}, D_j = 1/(1 + d_j D_{j-1}) and
+// f_j = f_{j-1} C_j D_j; a vanishing C_j or D_j denominator is replaced by
+// a tiny number.  It converges fast for x < (a + 1)/(a + b + 2).
+__host__ __device__ inline double dlmf_beta_cf(double a, double b, double x) {
+    const double tiny = 1e-300, tol = 1e-16;
+    double f = 1.0, C = 1.0, D = 0.0;
+    for (int j = 1; j <= 200000; ++j) {
+        const double m = (double)(j >> 1);
+        const double dj = (j & 1) ? -(a + m) * (a + b + m) * x / ((a + 2.0 * m) * (a + 2.0 * m + 1.0))
+                                  : m * (b - m) * x / ((a + 2.0 * m - 1.0) * (a + 2.0 * m));
+        D = 1.0 + dj * D;
+        if (fabs(D) < tiny) D = tiny;
+        D = 1.0 / D;
+        C = 1.0 + dj / C;
+        if (fabs(C) < tiny) C = tiny;
+        const double delta = C * D;
+        f *= delta;
+        if (fabs(delta - 1.0) < tol) break;
     }
-    return h;
+    return 1.0 / f;
 }
 
+// I_x(a, b) from the fraction, using the symmetry I_x(a, b) = 1 - I_{1-x}(b, a)
+// where the direct fraction would converge slowly.
 __host__ __device__ inline double betainc_reg(double a, double b, double x) {
     if (x <= 0.0) return 0.0;
     if (x >= 1.0) return 1.0;
-    const double lbt = lgamma(a + b) - lgamma(a) - lgamma(b) + a * log(x) + b * log1p(-x);
-    if (x < (a + 1.0) / (a + b + 2.0)) return exp(lbt) * betacf(a, b, x) / a;
-    return 1.0 - exp(lbt) * betacf(b, a, 1.0 - x) / b;
+    const double front = exp(lgamma(a + b) - lgamma(a) - lgamma(b) + a * log(x) + b * log1p(-x));
+    if (x < (a + 1.0) / (a + b + 2.0)) return front * dlmf_beta_cf(a, b, x) / a;
+    return 1.0 - front * dlmf_beta_cf(b, a, 1.0 - x) / b;
 }
 
 // P(|T_nu| >= sqrt(t2)); nu = +inf gives the normal tail.
