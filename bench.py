@@ -74,7 +74,10 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
+def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads, runs=5):
+    """The CPU leg: the oracle (a C++ port of the reference's accumulation and
+    of the filters) timed on a crop of the same frame, one warm-up run and
+    then the median of `runs` runs."""
     import oracle
 
     x0, y0, w, h = region
@@ -83,27 +86,85 @@ def cpu_baseline(sc, W, H, spp, lambdas, r, p, region, nthreads):
     smp = oracle.samples(sc, f["albedo"], 0, spp, nthreads=nthreads, region=region)
     log(f"cpu baseline inputs generated in {time.perf_counter() - t0:.1f}s")
     feats = [(f["albedo"], 0.02), (f["normal"], 0.1), (f["depth"], 0.5)]
-    t0 = time.perf_counter()
-    n, st = oracle.accumulate(smp, lambdas, 5, sanitize=True, nthreads=nthreads)
-    film = oracle.film_image(oracle.film_add(smp, sanitize=True))
-    t1 = time.perf_counter()
-    jb, lnS2, se2, m2C = oracle.prepass(n, st, p.var_z)
-    vD, vCD = oracle.denoise_vars(n, st[:, 1], lnS2.astype(np.float32), se2.astype(np.float32),
-                                  m2C.astype(np.float32), r, p.ds_factor, p.var_z, feats, nthreads=nthreads)
-    oracle.denoise_film(n, st[:, 0], vCD.astype(np.float32), film, r, p.ds_factor, p.alpha, mt_mode=2,
-                        jb=jb.astype(np.float32), feats=feats, nthreads=nthreads)
-    t2 = time.perf_counter()
-    total = t2 - t0
+
+    def one():
+        t0 = time.perf_counter()
+        n, st = oracle.accumulate(smp, lambdas, 5, sanitize=True, nthreads=nthreads)
+        film = oracle.film_image(oracle.film_add(smp, sanitize=True))
+        t1 = time.perf_counter()
+        jb, lnS2, se2, m2C = oracle.prepass(n, st, p.var_z)
+        vD, vCD = oracle.denoise_vars(n, st[:, 1], lnS2.astype(np.float32), se2.astype(np.float32),
+                                      m2C.astype(np.float32), r, p.ds_factor, p.var_z, feats, nthreads=nthreads)
+        oracle.denoise_film(n, st[:, 0], vCD.astype(np.float32), film, r, p.ds_factor, p.alpha, mt_mode=2,
+                            jb=jb.astype(np.float32), feats=feats, nthreads=nthreads)
+        return t1 - t0, time.perf_counter() - t1
+
+    one()  # warm-up
+    times = []
+    for i in range(runs):
+        times.append(one())
+        log(f"cpu baseline run {i + 1}/{runs}: {sum(times[-1]):.2f}s")
+    tot = sorted(sum(t) for t in times)[runs // 2]
+    acc = sorted(t[0] for t in times)[runs // 2]
+    filt = sorted(t[1] for t in times)[runs // 2]
     return {
-        "value": round(w * h / total / 1e6, 4),
+        "value": round(w * h / tot / 1e6, 4),
         "unit": "Mpixel/s",
         "cores": nthreads,
         "kind": "port",
         "sample": f"{w}x{h} crop at ({x0},{y0}) of the same 1080p frame, {spp} spp mt5; oracle accumulate "
-                  f"(reference FP32 recurrence, 16x16 tiles, sanitised) and Film {t1 - t0:.2f}s + FP64 prepass/variance/Welch "
-                  f"filters {t2 - t1:.2f}s on {nthreads} threads of {cpu_model()}",
-        "accumulate_gsample_s": round(w * h * spp / (t1 - t0) / 1e9, 4),
+                  f"(reference FP32 recurrence, 16x16 tiles, sanitised) and Film {acc:.2f}s + FP64 prepass/variance/"
+                  f"Welch filters {filt:.2f}s on {nthreads} threads of {cpu_model()}; median of {runs} runs after "
+                  f"one warm-up",
+        "runs_s": [round(sum(t), 3) for t in times],
+        "accumulate_gsample_s": round(w * h * spp / acc / 1e9, 4),
     }
+
+
+def fp64_error(ctx, bs, chunk, feats, p, W, H, r, region):
+    """Checker on the timed frame: a crop of its samples through the oracle's
+    full-FP64 pipeline, against the GPU's FP32 buffers of the same frame (the
+    variance outputs where their window lies in the crop, the Welch outputs
+    and decisions where their neighbours' windows do too)."""
+    import ctypes as C
+
+    import oracle
+    import stater_amd as sa
+    from stater_amd import hipmem
+    from stater_amd.workload import MT5_LAMBDAS
+
+    x0, y0, w, h = region
+    win = 2 * r + 1
+    row = W * win * win * 3
+    buf = hipmem.DeviceBuffer(H * row)
+    film = bs.view("film")
+    g = feats.gbuffers()
+    sa.check(sa.load_library().stater_debug_film_decisions(bs.h, C.byref(p), C.byref(g), C.byref(film), 0,
+                                                           C.c_void_p(buf.ptr), None))
+    ctx.synchronize()
+    dec = buf.download(np.uint8, h * row, offset=y0 * row).reshape(h, W, win, win, 3)[:, x0:x0 + w]
+    buf.free()
+    lam = oracle.set_lambdas(MT5_LAMBDAS)
+    slots = [0] + [list(np.float32(MT5_LAMBDAS)).index(np.float32(l)) for l in lam[1:]]
+
+    def crop(name):
+        return hipmem.read_plane(hipmem.sub_plane(bs.view(name), x0, y0, w, h))
+
+    def sets(name):
+        return np.stack([crop(f"t0-b{j}-film-{name}" if s == 0 else f"t0-b{j}-{name}") for s, j in enumerate(slots)])
+
+    gpu = {"n": crop("n")[0], "lnS2": sets("lnS2"), "se2": sets("se2"), "varD": sets("varD"),
+           "varCD": sets("varCD"), "filmD": crop("filmD"), "decisions": dec}
+    fn = feats.to_numpy()
+    fl = [(fn["albedo"][:, y0:y0 + h, x0:x0 + w], 0.02), (fn["normal"][:, y0:y0 + h, x0:x0 + w], 0.1),
+          (fn["depth"][:, y0:y0 + h, x0:x0 + w], 0.5)]
+    smp = chunk.to_numpy(rows=(y0, y0 + h))[..., x0:x0 + w]
+    ref = oracle.pipeline64(smp, MT5_LAMBDAS, r, p.ds_factor, p.var_z, p.alpha, feats=fl)
+    rep = oracle.fp64_error_report(gpu, ref, p.var_z, r, crop=True)
+    rep["crop"] = f"{w}x{h} at ({x0},{y0}); variance outputs on the inner {w - 2 * r}x{h - 2 * r}, Welch outputs " \
+                  f"on the inner {w - 4 * r}x{h - 4 * r}"
+    rep["bars"] = {"decisions_differ": 1e-4, "max_rel": 1e-4}
+    return rep
 
 
 def c4_images(n=48):
@@ -300,7 +361,12 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=256)
     ap.add_argument("--radius", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-region", default="480,270,960,540", help="x0,y0,w,h of the CPU baseline's crop")
+    ap.add_argument("--cpu-region", default="768,432,384,216", help="x0,y0,w,h of the CPU baseline's crop")
+    ap.add_argument("--error-region", default="900,500,112,96",
+                    help="x0,y0,w,h of the crop checked against the FP64 pipeline")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="GiB: only copy a device buffer of this size three times (hipMemcpy, device to device), "
+                         "the known-size copy that calibrates the HBM traffic counters")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: the headline 1080p frame; c4: 48 uneven images, batched vs one image per launch; "
                          "c5: served Poisson arrivals vs one job at a time")
@@ -310,6 +376,17 @@ def main() -> int:
     ap.add_argument("--chunk-spp", type=int, default=64, help="c4, c5: samples per pixel of one accumulation pass")
     ap.add_argument("--pass-spp", type=int, default=256, help="c3: samples per pixel of one accumulation pass")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        from stater_amd import hipmem
+
+        nb = int(args.copy_probe * 2**30)
+        a, b = hipmem.DeviceBuffer(nb), hipmem.DeviceBuffer(nb)
+        a.zero()
+        for _ in range(3):
+            hipmem._ok(hipmem.hip().hipMemcpy(b.ptr, a.ptr, nb, hipmem.D2D), "hipMemcpy D2D")
+        hipmem._ok(hipmem.hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        print(json.dumps({"copy_probe_bytes": nb, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); on several GPUs the path "
               f"would shard by image and by row bands with an r-row halo, one process per GPU",
@@ -385,15 +462,18 @@ def main() -> int:
         kernels[name] = k
     dom = max(kernels, key=lambda n: kernels[n]["avg_ms"] * kernels[n]["launches"])
     dk = kernels[dom]
-    traffic = None
+    traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / f"traffic_{dom}.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        traffic_src = (f"profiles/{tfile.name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this command, scaled "
+                       f"by the same counters on a copy of known size (bench.py --copy-probe); not measured in this "
+                       f"run")
     if "tlane_op_s" in dk:
         # VALU-bound: neither the HBM nor the MFMA roof describes it (no dense contraction on this
         # path); the same call's HBM figures are given beside it
         roofline = {"bound": "valu", "achieved": dk["tlane_op_s"], "peak": PEAK_LANE_OPS, "unit": "Tlane-op/s",
-                    "frac": dk["valu_frac"], "traffic": traffic, "kernel": dom}
+                    "frac": dk["valu_frac"], "traffic": traffic, "traffic_source": traffic_src, "kernel": dom}
         if "gb_s" in dk:
             roofline["hbm"] = {"achieved": dk["gb_s"], "peak": PEAK_HBM, "unit": "GB/s", "frac": dk["hbm_frac"]}
     else:
@@ -402,8 +482,11 @@ def main() -> int:
     acc = kernels.get("accumulate")
     gsample = round(P * spp / (acc["avg_ms"] * 1e-3) / 1e9, 3) if acc else None
 
-    cpu = None
+    cpu, error = None, None
     if not args.no_cpu:
+        region = tuple(int(v) for v in args.error_region.split(","))
+        log(f"fp64 check of a {region[2]}x{region[3]} crop of the timed frame")
+        error = fp64_error(ctx, bs, chunk, feats, p, W, H, r, region)
         region = tuple(int(v) for v in args.cpu_region.split(","))
         nthreads = max(1, min(os.cpu_count() or 1, 16))
         log(f"cpu baseline on a {region[2]}x{region[3]} crop, {nthreads} threads")
@@ -433,6 +516,7 @@ def main() -> int:
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "error": error,
     }
     print(json.dumps(out), flush=True)
     bs.close()

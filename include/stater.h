@@ -274,6 +274,14 @@ int stater_denoise_plain(stater_bufset *bs, int radius, float alpha, void *strea
  * out: NULL = the set's targetspp plane. */
 int stater_calc_target_spp(stater_bufset *bs, uint64_t budget_spp, uint32_t cap,
                            const stater_plane *out, void *stream);
+/* The same map with the variance it is built from chosen by as_mode, the
+ * reference's AS_MODE build flag (scripts/alternate-builds/build-asn+mcl.sh:5,
+ * README.md:140-154; calcTargetSPP receives both m2 and varD,
+ * estimator.cpp:502-511): 1 ("ASM", the default, what stater_calc_target_spp
+ * does) = the denoised variance varD of set 0; 0 ("ASN") = the pixel's own
+ * sample variance m2/(n-1). */
+int stater_calc_target_spp_ex(stater_bufset *bs, uint64_t budget_spp, uint32_t cap, int as_mode,
+                              const stater_plane *out, void *stream);
 /* The adaptive-sampling map of stater_calc_target_spp on caller planes,
  * allocating nothing.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving

@@ -61,6 +61,16 @@ def lib() -> C.CDLL:
     L.orc_t2_crit.argtypes = [C.c_double, C.c_double]
     L.orc_t2_crit.restype = C.c_double
     L.orc_prepass.argtypes = [C.c_int, C.c_int, _i32p, _f32p, C.c_double, _f64p, _f64p, _f64p, _f64p]
+    L.orc_prepass64.argtypes = [C.c_int, C.c_int, _i32p, _f64p, C.c_double, _f64p, _f64p, _f64p, _f64p]
+    L.orc_denoise_vars64.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int,
+                                     C.c_double, C.c_double, C.c_int, _vp, _vp, _vp, _f64p, _f64p, _vp, _vp,
+                                     C.c_int]
+    L.orc_denoise_film64.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, _f64p, _vp, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, _f64p, _f64p, _vp, _vp, _vp,
+                                     C.c_int]
+    L.orc_var_decisions.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f32p, _f32p, C.c_int, C.c_double, _u8p]
+    L.orc_var_decisions64.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f64p, _f64p, C.c_int, C.c_double, _u8p]
+    L.orc_film_image64.argtypes = [C.c_int, C.c_int, _f32p, C.c_int, _vp, C.c_int, _f64p]
     L.orc_denoise_vars.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f32p, _f32p, _f32p, _f32p, C.c_int,
                                    C.c_double, C.c_double, C.c_int, _vp, _vp, _vp, _f64p, _f64p, _vp, _vp, C.c_int]
     L.orc_denoise_film.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _f32p, _f32p, _vp, C.c_int, C.c_double,
@@ -167,6 +177,17 @@ def film_resolve(xyzw: np.ndarray, scale: float = 1.0) -> np.ndarray:
     return out.reshape(3, H, W)
 
 
+def film_image64(samples_arr: np.ndarray, counts=None, sanitize=False) -> np.ndarray:
+    """pbrt's Film image of the samples [spp,5,H,W] evaluated in FP64 [3,H,W]."""
+    spp, _, H, W = samples_arr.shape
+    P = W * H
+    out = np.zeros(3 * P)
+    cnt = None if counts is None else np.ascontiguousarray(counts.reshape(P), np.int32)
+    lib().orc_film_image64(W, H, np.ascontiguousarray(samples_arr.reshape(spp, 5, P), np.float32), spp,
+                           None if cnt is None else cnt.ctypes.data, int(sanitize), out)
+    return out.reshape(3, H, W)
+
+
 def film_image(sums: np.ndarray, scale: float = 1.0) -> np.ndarray:
     """The image of an iteration from the running sums: Clear, merge, UpdateImage."""
     return film_resolve(film_xyz(sums), scale)
@@ -184,13 +205,17 @@ def moments_exact(samples_arr: np.ndarray, lambdas, counts=None, sanitize=False)
 
 
 def prepass(n: np.ndarray, stats: np.ndarray, z: float):
+    """FP32 statistics (the GPU's, or the reference recurrence's) or FP64
+    ones (the full-FP64 pipeline: accumulate(..., fp64=True))."""
     S = stats.shape[0]
     H, W = n.shape
     P = W * H
     jb = np.zeros(S * P)
     l, s, m = (np.zeros(S * 3 * P) for _ in range(3))
-    lib().orc_prepass(P, S, np.ascontiguousarray(n, np.int32).reshape(-1),
-                      np.ascontiguousarray(stats, np.float32).reshape(-1), z, jb, l, s, m)
+    f64 = stats.dtype == np.float64
+    fn = lib().orc_prepass64 if f64 else lib().orc_prepass
+    fn(P, S, np.ascontiguousarray(n, np.int32).reshape(-1),
+       np.ascontiguousarray(stats, np.float64 if f64 else np.float32).reshape(-1), z, jb, l, s, m)
     return jb.reshape(S, H, W), l.reshape(S, 3, H, W), s.reshape(S, 3, H, W), m.reshape(S, 3, H, W)
 
 
@@ -208,8 +233,9 @@ def _feat_args(feats):
 
 def denoise_vars(n, m2, lnS2, se2, m2C, radius, ds_factor, z, feats=None, nthreads: int = 8, want_count=False,
                  want_ambiguous=False):
-    """All per-set arrays [S,3,H,W] float32.  Returns varD, varCD (FP64)
-    [, accepted count] [, count of decisions within 1e-5 of the threshold]."""
+    """All per-set arrays [S,3,H,W], float32 (or all float64: the full-FP64
+    pipeline).  Returns varD, varCD (FP64) [, accepted count] [, count of
+    decisions within 1e-5 of the threshold]."""
     S = m2.shape[0]
     H, W = n.shape
     P = W * H
@@ -217,8 +243,11 @@ def denoise_vars(n, m2, lnS2, se2, m2C, radius, ds_factor, z, feats=None, nthrea
     vd, vcd = np.zeros(S * 3 * P), np.zeros(S * 3 * P)
     cnt = np.zeros(S * 3 * P, np.int32) if want_count else None
     amb = np.zeros(S * 3 * P, np.int32) if want_ambiguous else None
-    f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
-    lib().orc_denoise_vars(W, H, S, np.ascontiguousarray(n, np.int32).reshape(-1), f(m2), f(lnS2), f(se2), f(m2C),
+    f64 = np.asarray(m2).dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    f = lambda a: np.ascontiguousarray(a, dt).reshape(-1)
+    fn = lib().orc_denoise_vars64 if f64 else lib().orc_denoise_vars
+    fn(W, H, S, np.ascontiguousarray(n, np.int32).reshape(-1), f(m2), f(lnS2), f(se2), f(m2C),
                            radius, ds_factor, z, ng, gp, gc, gd, vd, vcd,
                            None if cnt is None else cnt.ctypes.data, None if amb is None else amb.ctypes.data,
                            nthreads)
@@ -227,9 +256,26 @@ def denoise_vars(n, m2, lnS2, se2, m2C, radius, ds_factor, z, feats=None, nthrea
     return out + ((amb.reshape(S, 3, H, W),) if want_ambiguous else ())
 
 
+def var_decisions(n, lnS2, se2, radius, z) -> np.ndarray:
+    """The variance filter's decisions [S,3,H,W,win,win] (uint8, 1 = accepted)
+    for FP32 or FP64 lnS2/se2 [S,3,H,W]."""
+    S = lnS2.shape[0]
+    H, W = n.shape
+    win = 2 * radius + 1
+    out = np.zeros(S * 3 * H * W * win * win, np.uint8)
+    f64 = np.asarray(lnS2).dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    (lib().orc_var_decisions64 if f64 else lib().orc_var_decisions)(
+        W, H, S, np.ascontiguousarray(n, np.int32).reshape(-1), np.ascontiguousarray(lnS2, dt).reshape(-1),
+        np.ascontiguousarray(se2, dt).reshape(-1), radius, z, out)
+    return out.reshape(S, 3, H, W, win, win)
+
+
 def denoise_film(n, mean, var, film, radius, ds_factor, alpha, mt_mode=2, jb=None, feats=None, want_varw=False,
                  want_decisions=False, nthreads: int = 8):
-    """mean/var [S,3,H,W] float32 (set 0 mean = film); film [3,H,W].  Returns dict."""
+    """mean/var [S,3,H,W] float32 (set 0 mean = film); film [3,H,W].  With
+    float64 mean the inputs are taken in FP64 (the full-FP64 pipeline).
+    Returns dict."""
     S = mean.shape[0]
     H, W = n.shape
     P = W * H
@@ -238,9 +284,11 @@ def denoise_film(n, mean, var, film, radius, ds_factor, alpha, mt_mode=2, jb=Non
     varw = np.zeros(3 * P) if want_varw else None
     win = 2 * radius + 1
     dec = np.zeros(P * win * win * 3, np.uint8) if want_decisions else None
-    jbc = None if jb is None else np.ascontiguousarray(jb, np.float32).reshape(-1)
-    f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
-    lib().orc_denoise_film(W, H, S, np.ascontiguousarray(n, np.int32).reshape(-1), f(mean), f(var),
+    f64 = np.asarray(mean).dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    jbc = None if jb is None else np.ascontiguousarray(jb, dt).reshape(-1)
+    f = lambda a: np.ascontiguousarray(a, dt).reshape(-1)
+    (lib().orc_denoise_film64 if f64 else lib().orc_denoise_film)(W, H, S, np.ascontiguousarray(n, np.int32).reshape(-1), f(mean), f(var),
                            None if jbc is None else jbc.ctypes.data, radius, ds_factor, alpha, mt_mode, ng, gp, gc,
                            gd, f(film), filmD, None if varw is None else varw.ctypes.data,
                            None if dec is None else dec.ctypes.data, None, nthreads)
@@ -267,3 +315,73 @@ def mean_var(n, m2_0) -> np.ndarray:
     lib().orc_mean_var(W * H, np.ascontiguousarray(n, np.int32).reshape(-1),
                        np.ascontiguousarray(m2_0, np.float32).reshape(-1), out)
     return out.reshape(3, H, W)
+
+
+def pipeline64(samples_arr: np.ndarray, lambdas, radius: int, ds_factor: float, z: float, alpha: float,
+               feats=None, budget_spp: int = 32, cap: int = 4096, sanitize=True, film=True, mt_mode=2):
+    """The whole path in FP64 from the same samples [spp,5,H,W]: FP64
+    recurrence with the FP64 Box-Cox, FP64 prepass, variance filter and Welch
+    filter (of the FP64 pbrt Film, or of the set-0 mean), and the AS map from
+    the FP64 varD and mean (rounded to FP32 for its fixed FP32 metric)."""
+    n, st = accumulate(samples_arr, lambdas, 5, fp64=True, sanitize=sanitize)
+    jb, lnS2, se2, m2C = prepass(n, st, z)
+    vD, vCD = denoise_vars(n, st[:, 1], lnS2, se2, m2C, radius, ds_factor, z, feats)
+    fm = film_image64(samples_arr, sanitize=sanitize) if film else st[0, 0]
+    mean = st[:, 0].copy()
+    mean[0] = fm
+    out = denoise_film(n, mean, vCD, fm, radius, ds_factor, alpha, mt_mode=mt_mode, jb=jb, feats=feats,
+                       want_decisions=True)
+    tgt = target_spp(n, vD[0].astype(np.float32), st[0, 0].astype(np.float32), budget_spp, cap)
+    return {"n": n, "stats": st, "jb": jb, "lnS2": lnS2, "se2": se2, "m2C": m2C, "varD": vD, "varCD": vCD,
+            "film": fm, "filmD": out["filmD"], "decisions": out["decisions"], "targetspp": tgt}
+
+
+def fp64_error_report(gpu: dict, ref: dict, z: float, radius: int, crop: bool = False) -> dict:
+    """Per-buffer error of the GPU's FP32 path against the full-FP64 pipeline
+    (pipeline64) on the same pixels.
+
+    gpu: n [H,W], lnS2, se2, varD, varCD [S,3,H,W], filmD [3,H,W], welch
+    decisions [H,W,win,win,3] and optionally targetspp [H,W].  crop: the
+    arrays are a crop of a larger frame, whose FP64 pipeline ran on the crop
+    alone; then the variance outputs are compared where their window lies in
+    the crop (a margin of radius) and the Welch outputs where the windows of
+    their neighbours' variances do too (a margin of 2 radius: the Welch test
+    reads the neighbours' filtered variances).  For each buffer: the fraction of
+    accept/reject decisions that differ (variance filter: the decisions the
+    oracle takes on the GPU's FP32 lnS2/se2, which the kernel reproduces, vs
+    the FP64 ones), the largest and the 99.9th-percentile relative error
+    over pixel-channels whose decisions all agree, and how many of those
+    there are."""
+    m = radius if crop else 0
+    H, W = gpu["n"].shape
+    sy, sx = slice(m, H - m), slice(m, W - m)
+    out = {}
+    d32 = var_decisions(gpu["n"], gpu["lnS2"], gpu["se2"], radius, z)[:, :, sy, sx]
+    d64 = var_decisions(ref["n"], ref["lnS2"], ref["se2"], radius, z)[:, :, sy, sx]
+    vdiff = d32 != d64
+    vsame = ~np.any(vdiff, axis=(4, 5))
+    for name in ("varD", "varCD"):
+        g = np.asarray(gpu[name], np.float64)[:, :, sy, sx]
+        r = np.asarray(ref[name], np.float64)[:, :, sy, sx]
+        ok = vsame & np.isfinite(r) & (np.abs(r) < 1e30)
+        rel = np.abs(g[ok] - r[ok]) / np.maximum(np.abs(r[ok]), 1e-30)
+        out[name] = {"decisions_differ": float(np.count_nonzero(vdiff) / vdiff.size),
+                     "max_rel": float(rel.max()) if rel.size else None,
+                     "p999_rel": float(np.percentile(rel, 99.9)) if rel.size else None,
+                     "agreeing": float(np.count_nonzero(ok) / ok.size)}
+    sy, sx = slice(2 * m, H - 2 * m), slice(2 * m, W - 2 * m)
+    wd = gpu["decisions"][sy, sx] != ref["decisions"][sy, sx]
+    wsame = ~np.any(wd, axis=(2, 3)).transpose(2, 0, 1)
+    g = np.asarray(gpu["filmD"], np.float64)[:, sy, sx]
+    r = np.asarray(ref["filmD"], np.float64)[:, sy, sx]
+    rel = np.abs(g[wsame] - r[wsame]) / np.maximum(np.abs(r[wsame]), 1e-6)
+    out["filmD"] = {"decisions_differ": float(np.count_nonzero(wd) / wd.size),
+                    "max_rel": float(rel.max()) if rel.size else None,
+                    "p999_rel": float(np.percentile(rel, 99.9)) if rel.size else None,
+                    "agreeing": float(np.count_nonzero(wsame) / wsame.size)}
+    if "targetspp" in gpu:
+        tg, tr = gpu["targetspp"][sy, sx], ref["targetspp"][sy, sx]
+        out["targetspp"] = {"pixels_differ": float(np.count_nonzero(tg != tr) / tg.size),
+                            "max_abs": int(np.max(np.abs(tg.astype(np.int64) - tr))),
+                            "sum_gpu": int(tg.sum(dtype=np.int64)), "sum_fp64": int(tr.sum(dtype=np.int64))}
+    return out

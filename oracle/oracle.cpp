@@ -637,11 +637,12 @@ ORC_API double orc_t2_crit(double nu, double alpha) { return t2_crit(nu, alpha);
 // Pixels with n < 4 are degenerate (all outputs 0).  A set-channel whose m2
 // or m4 is not finite (FP32 overflow, e.g. lambda = 3 on fireflies) gets
 // lnS2 = se2 = jb = +inf so that its variance tests abstain.
-ORC_API void orc_prepass(int P, int S, const int32_t *n, const float *stats, double z, double *jb,
-                         double *lnS2, double *se2, double *m2C) {
+template <typename T>
+void prepass_impl(int P, int S, const int32_t *n, const T *stats, double z, double *jb, double *lnS2, double *se2,
+                  double *m2C) {
     const size_t cs = 3 * (size_t)P;
     for (int s = 0; s < S; ++s) {
-        const float *st = stats + (size_t)s * N_STATS * cs;
+        const T *st = stats + (size_t)s * N_STATS * cs;
         for (int p = 0; p < P; ++p) {
             const double nn = (double)n[p];
             double jbs = 0.0;
@@ -677,15 +678,27 @@ ORC_API void orc_prepass(int P, int S, const int32_t *n, const float *stats, dou
     }
 }
 
+ORC_API void orc_prepass(int P, int S, const int32_t *n, const float *stats, double z, double *jb,
+                         double *lnS2, double *se2, double *m2C) {
+    prepass_impl<float>(P, S, n, stats, z, jb, lnS2, se2, m2C);
+}
+
+// The same from FP64 statistics (the full-FP64 pipeline).
+ORC_API void orc_prepass64(int P, int S, const int32_t *n, const double *stats, double z, double *jb,
+                           double *lnS2, double *se2, double *m2C) {
+    prepass_impl<double>(P, S, n, stats, z, jb, lnS2, se2, m2C);
+}
+
 // Variance filter (FP64 from FP32 inputs): per set s and channel c,
 // varD = sum_q w acc s2_q / sum_q w acc, varCD likewise with m2C_q/(n_q-1);
 // acc = Bonett equality test (lnS2_p - lnS2_q)^2 <= z^2 (se2_p + se2_q).
 // m2, lnS2, se2, m2C: [S][3][P] float.  Outputs [S][3][P] double and, if
 // given, the accepted-neighbour count [S][3][P].
-ORC_API void orc_denoise_vars(int W, int H, int S, const int32_t *n, const float *m2, const float *lnS2,
-                              const float *se2, const float *m2C, int radius, double dsF, double z,
-                              int ng, const float *const *gplanes, const int *gch, const double *gdr,
-                              double *varD, double *varCD, int32_t *count, int32_t *ambiguous, int nthreads) {
+template <typename T>
+void denoise_vars_impl(int W, int H, int S, const int32_t *n, const T *m2, const T *lnS2, const T *se2,
+                       const T *m2C, int radius, double dsF, double z, int ng, const float *const *gplanes,
+                       const int *gch, const double *gdr, double *varD, double *varCD, int32_t *count,
+                       int32_t *ambiguous, int nthreads) {
     const size_t P = (size_t)W * H;
     Feat f{ng, gplanes, gch, gdr, P};
     const double z2 = z * z;
@@ -733,6 +746,68 @@ ORC_API void orc_denoise_vars(int W, int H, int S, const int32_t *n, const float
     });
 }
 
+
+ORC_API void orc_denoise_vars(int W, int H, int S, const int32_t *n, const float *m2, const float *lnS2,
+                              const float *se2, const float *m2C, int radius, double dsF, double z,
+                              int ng, const float *const *gplanes, const int *gch, const double *gdr,
+                              double *varD, double *varCD, int32_t *count, int32_t *ambiguous, int nthreads) {
+    denoise_vars_impl<float>(W, H, S, n, m2, lnS2, se2, m2C, radius, dsF, z, ng, gplanes, gch, gdr, varD, varCD,
+                             count, ambiguous, nthreads);
+}
+
+// The same on FP64 inputs (the full-FP64 pipeline).
+ORC_API void orc_denoise_vars64(int W, int H, int S, const int32_t *n, const double *m2, const double *lnS2,
+                                const double *se2, const double *m2C, int radius, double dsF, double z,
+                                int ng, const float *const *gplanes, const int *gch, const double *gdr,
+                                double *varD, double *varCD, int32_t *count, int32_t *ambiguous, int nthreads) {
+    denoise_vars_impl<double>(W, H, S, n, m2, lnS2, se2, m2C, radius, dsF, z, ng, gplanes, gch, gdr, varD, varCD,
+                              count, ambiguous, nthreads);
+}
+
+// The variance filter's accept/reject decisions alone (the same rule as
+// denoise_vars_impl): dec[(((s*3 + c)*P + p)*win + dy+r)*win + dx+r], 1 =
+// accepted; neighbours outside the image are 0.
+template <typename T>
+void var_decisions_impl(int W, int H, int S, const int32_t *n, const T *lnS2, const T *se2, int radius, double z,
+                        uint8_t *dec) {
+    const size_t P = (size_t)W * H;
+    const int win = 2 * radius + 1;
+    const double z2 = z * z;
+    for (int s = 0; s < S; ++s)
+        for (int c = 0; c < 3; ++c) {
+            const size_t ip = ((size_t)s * 3 + c) * P;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const size_t p = (size_t)y * W + x;
+                    uint8_t *o = dec + (ip + p) * win * win;
+                    for (int dy = -radius; dy <= radius; ++dy)
+                        for (int dx = -radius; dx <= radius; ++dx) {
+                            const int yq = y + dy, xq = x + dx;
+                            bool acc = false;
+                            if (yq >= 0 && yq < H && xq >= 0 && xq < W) {
+                                const size_t q = (size_t)yq * W + xq;
+                                if (q == p) acc = true;
+                                else if (n[p] >= 4 && n[q] >= 4) {
+                                    const double dl = (double)lnS2[ip + p] - (double)lnS2[ip + q];
+                                    acc = !(dl * dl > z2 * ((double)se2[ip + p] + (double)se2[ip + q]));
+                                }
+                            }
+                            o[(dy + radius) * win + dx + radius] = acc ? 1 : 0;
+                        }
+                }
+        }
+}
+
+ORC_API void orc_var_decisions(int W, int H, int S, const int32_t *n, const float *lnS2, const float *se2,
+                               int radius, double z, uint8_t *dec) {
+    var_decisions_impl<float>(W, H, S, n, lnS2, se2, radius, z, dec);
+}
+
+ORC_API void orc_var_decisions64(int W, int H, int S, const int32_t *n, const double *lnS2, const double *se2,
+                                 int radius, double z, uint8_t *dec) {
+    var_decisions_impl<double>(W, H, S, n, lnS2, se2, radius, z, dec);
+}
+
 // Welch-test mean filter (FP64 decisions with exact tails).
 //   mean: [S][3][P] per-set means (set 0: the film or raw mean the test uses)
 //   var : [S][3][P] variance fed to the test (v = var/n, w = v^2/(n-1))
@@ -740,11 +815,11 @@ ORC_API void orc_denoise_vars(int W, int H, int S, const int32_t *n, const float
 //   film: [3][P] values that are averaged.
 // Outputs filmD [3][P]; varW [3][P] (mean of accepted var of set 0) if given;
 // decisions [P][(2r+1)^2][3] (uint8) if given; count [3][P] if given.
-ORC_API void orc_denoise_film(int W, int H, int S, const int32_t *n, const float *mean, const float *var,
-                              const float *jb, int radius, double dsF, double alpha, int mt_mode, int ng,
-                              const float *const *gplanes, const int *gch, const double *gdr,
-                              const float *film, double *filmD, double *varW, uint8_t *decisions,
-                              int32_t *count, int nthreads) {
+template <typename T>
+void denoise_film_impl(int W, int H, int S, const int32_t *n, const T *mean, const T *var, const T *jb, int radius,
+                       double dsF, double alpha, int mt_mode, int ng, const float *const *gplanes, const int *gch,
+                       const double *gdr, const T *film, double *filmD, double *varW, uint8_t *decisions,
+                       int32_t *count, int nthreads) {
     const size_t P = (size_t)W * H;
     const int win = 2 * radius + 1;
     Feat f{ng, gplanes, gch, gdr, P};
@@ -774,10 +849,11 @@ ORC_API void orc_denoise_film(int W, int H, int S, const int32_t *n, const float
                                 for (int s = 0; s < S && acc; ++s) {
                                     if (s_only >= 0 && s != s_only) continue;
                                     const size_t i = ((size_t)s * 3 + c) * P;
-                                    // v = var/n rounded to FP32: the per-pixel discr value.
+                                    // v = var/n in the input's precision: for FP32 inputs the
+                                    // per-pixel discr value.
                                     const double np = n[p], nq = n[q];
-                                    const double vp = (double)(var[i + p] / (float)n[p]);
-                                    const double vq = (double)(var[i + q] / (float)n[q]);
+                                    const double vp = (double)(var[i + p] / (T)n[p]);
+                                    const double vq = (double)(var[i + q] / (T)n[q]);
                                     acc = welch_accept(tab, mean[i + p], vp, vp * vp / (np - 1.0), mean[i + q], vq,
                                                        vq * vq / (nq - 1.0), alpha);
                                 }
@@ -798,6 +874,51 @@ ORC_API void orc_denoise_film(int W, int H, int S, const int32_t *n, const float
             }
         }
     });
+}
+
+ORC_API void orc_denoise_film(int W, int H, int S, const int32_t *n, const float *mean, const float *var,
+                              const float *jb, int radius, double dsF, double alpha, int mt_mode, int ng,
+                              const float *const *gplanes, const int *gch, const double *gdr,
+                              const float *film, double *filmD, double *varW, uint8_t *decisions,
+                              int32_t *count, int nthreads) {
+    denoise_film_impl<float>(W, H, S, n, mean, var, jb, radius, dsF, alpha, mt_mode, ng, gplanes, gch, gdr, film,
+                             filmD, varW, decisions, count, nthreads);
+}
+
+// The same on FP64 inputs (the full-FP64 pipeline).
+ORC_API void orc_denoise_film64(int W, int H, int S, const int32_t *n, const double *mean, const double *var,
+                                const double *jb, int radius, double dsF, double alpha, int mt_mode, int ng,
+                                const float *const *gplanes, const int *gch, const double *gdr,
+                                const double *film, double *filmD, double *varW, uint8_t *decisions,
+                                int32_t *count, int nthreads) {
+    denoise_film_impl<double>(W, H, S, n, mean, var, jb, radius, dsF, alpha, mt_mode, ng, gplanes, gch, gdr, film,
+                              filmD, varW, decisions, count, nthreads);
+}
+
+// pbrt's Film image (the same Clear / merge / UpdateImage steps as
+// orc_film_add, orc_film_xyz and orc_film_resolve) evaluated in FP64, the
+// matrices taken at their FP32 values: the film of the full-FP64 pipeline.
+ORC_API void orc_film_image64(int W, int H, const float *samples, int spp, const int32_t *counts, int san,
+                              double *rgb) {
+    const size_t P = (size_t)W * H;
+    const double M[3][3] = {{0.412453f, 0.357580f, 0.180423f}, {0.212671f, 0.715160f, 0.072169f},
+                            {0.019334f, 0.119193f, 0.950227f}};
+    const double Mi[3][3] = {{3.240479f, -1.537150f, -0.498535f}, {-0.969256f, 1.875991f, 0.041556f},
+                             {0.055648f, -0.204043f, 1.057311f}};
+    for (size_t p = 0; p < P; ++p) {
+        const int cnt = counts ? std::min(counts[p], spp) : spp;
+        double sum[3] = {0, 0, 0};
+        for (int j = 0; j < cnt; ++j) {
+            const float *sj = samples + (size_t)j * 5 * P + p;
+            float v[3] = {sj[0], sj[P], sj[2 * P]};
+            if (san) sanitize(v[0], v[1], v[2]);
+            for (int c = 0; c < 3; ++c) sum[c] += v[c];
+        }
+        double xyz[3], o[3];
+        for (int i = 0; i < 3; ++i) xyz[i] = M[i][0] * sum[0] + M[i][1] * sum[1] + M[i][2] * sum[2];
+        for (int i = 0; i < 3; ++i) o[i] = Mi[i][0] * xyz[0] + Mi[i][1] * xyz[1] + Mi[i][2] * xyz[2];
+        for (int c = 0; c < 3; ++c) rgb[(size_t)c * P + p] = cnt > 0 ? std::max(0.0, o[c] / cnt) : o[c];
+    }
 }
 
 // Adaptive-sampling map (integer exact).  e_p = mean_c sqrt(varD_c/n)/max(|mean_c|,1e-3)

@@ -130,6 +130,7 @@ SIGNATURES = [
                                       C.POINTER(Plane), _P]),
     ("stater_denoise_plain", C.c_int, [_P, C.c_int, C.c_float, _P]),
     ("stater_calc_target_spp", C.c_int, [_P, C.c_uint64, C.c_uint32, C.POINTER(Plane), _P]),
+    ("stater_calc_target_spp_ex", C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(Plane), _P]),
     ("stater_calc_target_spp_noalloc", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Plane), C.POINTER(Plane),
                                                  C.POINTER(Plane), C.c_uint64, C.c_uint32, C.POINTER(Plane), _P]),
     ("stater_calc_mean_vars", C.c_int, [_P, _P]),
@@ -324,8 +325,11 @@ class BufferSet:
     def denoise_plain(self, radius: int = 10, alpha: float = 0.005, stream=None) -> None:
         check(self.lib.stater_denoise_plain(self.h, radius, alpha, stream))
 
-    def calc_target_spp(self, budget_spp: int, cap: int, out: Plane | None = None, stream=None) -> None:
-        check(self.lib.stater_calc_target_spp(self.h, budget_spp, cap, C.byref(out) if out else None, stream))
+    def calc_target_spp(self, budget_spp: int, cap: int, out: Plane | None = None, stream=None,
+                        as_mode: int = 1) -> None:
+        """as_mode 1: from the denoised variance varD; 0: from the sample variance m2/(n-1)."""
+        check(self.lib.stater_calc_target_spp_ex(self.h, budget_spp, cap, as_mode, C.byref(out) if out else None,
+                                                 stream))
 
     def calc_mean_vars(self, stream=None) -> None:
         check(self.lib.stater_calc_mean_vars(self.h, stream))

@@ -568,13 +568,20 @@ int stater_denoise_plain(stater_bufset *bs, int radius, float alpha, void *strea
 
 int stater_calc_target_spp(stater_bufset *bs, uint64_t budget_spp, uint32_t cap, const stater_plane *out,
                            void *stream) {
+    return stater_calc_target_spp_ex(bs, budget_spp, cap, 1, out, stream);
+}
+
+int stater_calc_target_spp_ex(stater_bufset *bs, uint64_t budget_spp, uint32_t cap, int as_mode,
+                              const stater_plane *out, void *stream) {
     if (!bs) return fail(STATER_E_ARG, "stater_calc_target_spp: NULL set");
+    if (as_mode != 0 && as_mode != 1) return fail(STATER_E_ARG, "stater_calc_target_spp: as_mode must be 0 or 1");
+    if (as_mode == 0 && bs->mm < 2) return fail(STATER_E_STATE, "stater_calc_target_spp: as_mode 0 needs m2");
     if (bs->flags & STATER_BUFSET_NO_DERIVED) return fail(STATER_E_STATE, "stater_calc_target_spp: NO_DERIVED set");
     if (out && (!out->ptr || out->width != bs->W || out->height != bs->H || out->dtype != STATER_I32 ||
                 out->pitch_bytes % 4))
         return fail(STATER_E_ARG, "stater_calc_target_spp: bad output plane");
     if (budget_spp > (uint64_t)1 << 32) return fail(STATER_E_ARG, "stater_calc_target_spp: budget too large");
-    return launch_target_spp(bs, budget_spp, cap, out, pick_stream(bs, stream));
+    return launch_target_spp(bs, budget_spp, cap, as_mode, out, pick_stream(bs, stream));
 }
 
 int stater_calc_mean_vars(stater_bufset *bs, void *stream) {

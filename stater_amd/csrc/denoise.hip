@@ -1806,6 +1806,7 @@ struct AsArgs {
     int64_t n_ld;           // row pitch of n, elements
     int64_t v_ld, v_cs;     // varD: row pitch and channel stride, elements
     int64_t m_ld, m_cs;     // mean: row pitch and channel stride, elements
+    int var_is_m2;          // 1: varD holds m2 and the variance is m2/(n-1) (AS_MODE 0)
     int W, H;
     uint32_t cap;
     uint32_t *q;            // per-pixel q and h kept by the error pass, or null: recomputed
@@ -1828,7 +1829,9 @@ __device__ __forceinline__ void as_qh(const AsArgs &a, int p, uint32_t &q, int32
         const float nf = (float)n;
         float ec[3];
         for (int c = 0; c < 3; ++c) {
-            const float vd = fmaxf(a.varD[c * a.v_cs + (size_t)y * a.v_ld + x], 0.f);
+            float vd = a.varD[c * a.v_cs + (size_t)y * a.v_ld + x];
+            if (a.var_is_m2) vd = n >= 2 ? vd / (float)(n - 1) : 0.f;
+            vd = fmaxf(vd, 0.f);
             const float m = fmaxf(fabsf(a.mean[c * a.m_cs + (size_t)y * a.m_ld + x]), 1e-3f);
             ec[c] = sqrtf(vd / nf) / m;  // correctly rounded (IEEE) square root
         }
@@ -2094,7 +2097,8 @@ static int run_target_spp(const AsArgs &a, uint64_t budget, hipStream_t st) {
     return STATER_OK;
 }
 
-int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out, hipStream_t st) {
+int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, int as_mode, const stater_plane *out,
+                      hipStream_t st) {
     const int P = bs->W * bs->H;
     static_assert(sizeof(AsCtl) <= kAsCtlBytes, "AS control block outgrew its slot");
     if (!bs->as_scratch || bs->as_scratch_bytes < as_scratch_bytes_for((size_t)P))
@@ -2102,7 +2106,8 @@ int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const st
     char *sp = (char *)bs->as_scratch;
     AsArgs a{};
     a.n = bs->n;
-    a.varD = bs->varD;
+    a.varD = as_mode ? bs->varD : bs->stat(0, ST_M2, 0);
+    a.var_is_m2 = as_mode ? 0 : 1;
     a.mean = bs->stat(0, ST_MEAN, 0);
     a.n_ld = a.v_ld = a.m_ld = bs->ld;
     a.v_cs = a.m_cs = (int64_t)bs->plane;

@@ -116,7 +116,8 @@ int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const s
 int launch_welch_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                         const stater_plane *film, const stater_plane *film_out, bool plain, hipStream_t st,
                         int ty = 4);
-int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, const stater_plane *out,
+// as_mode 1: the denoised variance varD of set 0; 0: the sample variance m2/(n-1).
+int launch_target_spp(stater_bufset *bs, uint64_t budget, uint32_t cap, int as_mode, const stater_plane *out,
                       hipStream_t st);
 int launch_mean_vars(stater_bufset *bs, hipStream_t st);
 // The AS map over caller planes with the static scratch (stater_calc_target_spp_noalloc);

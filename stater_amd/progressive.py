@@ -25,8 +25,10 @@ from .workload import Features, SampleChunk, synth_samples
 def run_rounds(ctx: Context, bs: BufferSet, scene, feats: Features | None, params: DenoiseParams,
                gbufs: GBuffers | None, rounds: int = 8, first_spp: int = 32, budget_spp: int = 32,
                cap: int = 4096, pass_spp: int = 256, mode: int = STATER_ACC_SEQUENTIAL,
-               chunk: SampleChunk | None = None, denoise_last: bool = True) -> list[dict]:
-    """Runs the rounds; returns per-round {round, samples, max_target, ms}."""
+               chunk: SampleChunk | None = None, denoise_last: bool = True, as_mode: int = 1) -> list[dict]:
+    """Runs the rounds; returns per-round {round, samples, max_target, ms}.
+    as_mode: the AS map's variance, 1 = denoised varD, 0 = sample variance
+    (the reference's AS_MODE build flag)."""
     W, H = bs.width, bs.height
     n_view = bs.view("n")
     tgt_view = bs.view("targetspp")
@@ -37,8 +39,9 @@ def run_rounds(ctx: Context, bs: BufferSet, scene, feats: Features | None, param
         if r == 0:
             counts, total, samples = None, first_spp, first_spp * W * H
         else:
-            bs.denoise_vars(params, gbufs)
-            bs.calc_target_spp(budget_spp, cap)
+            if as_mode:
+                bs.denoise_vars(params, gbufs)
+            bs.calc_target_spp(budget_spp, cap, as_mode=as_mode)
             ctx.synchronize()
             tgt = read_plane(tgt_view)[0]
             counts, total, samples = tgt_view, int(tgt.max()), int(tgt.sum(dtype=np.int64))

@@ -221,6 +221,43 @@ def test_welch_filter_mt5(ctx, mt_mode, ragged):
     np.testing.assert_array_equal(discr, want)
 
 
+@pytest.mark.parametrize("cascade,use_levc", [(0, 0), (1, 0)])
+def test_welch_filter_variants(ctx, cascade, use_levc):
+    """The reference's variant builds: "M" (CASCADE_MODE 0, USE_LEVC 0: the
+    Welch test on the raw sample variances) and "MC" (USE_LEVC 0: on the
+    denoised variance varD without the LEV correction),
+    scripts/alternate-builds/build-asm+m.sh:6 and build-asm+mc.sh:5."""
+    W, H, r = 96, 64, 10
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, counts=_ragged_counts(W, H))
+    p = sa.default_params(radius=r, cascade=cascade, use_levc=use_levc)
+    g = feats.gbuffers()
+    bs.denoise_vars(p, g)
+    bs.denoise_film(p, g)
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    if cascade:
+        var = read_set_buf(bs, MT5_LAMBDAS, "varD")
+    else:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            var = np.where(n >= 2, st[:, 1] / (n.astype(np.float32) - np.float32(1)), np.float32(0))
+        var = var.astype(np.float32)
+    f = feats.to_numpy()
+    feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    ref = oracle.denoise_film(n, st[:, 0].copy(), var, st[0, 0], r, p.ds_factor, p.alpha, feats=feats_list,
+                              want_decisions=True)
+    dec = _decisions(ctx, bs, p, g, False, W, H, r)
+    diff = dec != ref["decisions"]
+    frac = np.count_nonzero(diff) / dec.size
+    assert frac <= DECISION_MISMATCH_MAX, frac
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
+    filmD = hipmem.read_plane(bs.view("filmD"))
+    np.testing.assert_allclose(filmD[same], ref["filmD"][same], rtol=FILTER_RTOL)
+    # the variants really differ from the default "MCL" on this film
+    assert np.count_nonzero(ref["decisions"] != oracle.denoise_film(
+        n, st[:, 0].copy(), read_set_buf(bs, MT5_LAMBDAS, "varCD"), st[0, 0], r, p.ds_factor, p.alpha,
+        feats=feats_list, want_decisions=True)["decisions"]) > 0
+
+
 @pytest.mark.parametrize("W,H,r,weighted,mt_mode", [
     (77, 45, 1, True, 2),     # smallest window of the one-test-per-pair path, ragged tiles
     (77, 45, 3, False, 2),    # unweighted (no features, ds_factor 0)
@@ -313,6 +350,22 @@ def test_target_spp_exact(ctx, budget, cap):
         assert int(tgt.sum()) == budget * W * H
     else:
         assert np.array_equal(tgt, head)
+
+
+def test_target_spp_as_mode_0_uses_the_sample_variance(ctx):
+    """AS_MODE 0 ("ASN"): the map is built from each pixel's own sample
+    variance m2/(n-1) instead of the denoised varD; ragged counts include
+    pixels with n < 2 (variance 0)."""
+    W, H, budget, cap = 120, 72, 16, 400
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, counts=_ragged_counts(W, H))
+    bs.calc_target_spp(budget, cap, as_mode=0)
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s2 = np.where(n >= 2, st[0, 1] / (n.astype(np.float32) - np.float32(1)), np.float32(0)).astype(np.float32)
+    tgt = hipmem.read_plane(bs.view("targetspp"))[0]
+    assert np.array_equal(tgt, oracle.target_spp(n, s2, st[0, 0], budget, cap))
+    assert int(tgt.sum()) == budget * W * H
 
 
 @pytest.mark.parametrize("budget,cap", [(32, 4096), (4, 64), (0, 4096)])

@@ -121,3 +121,18 @@ def test_full_frame(ctx):
     rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-6)
     assert np.count_nonzero(same) > 0.99 * same.size
     assert rel[same].max() < FILTER_RTOL, rel[same].max()
+
+    # the same window against the full-FP64 pipeline on its samples: the
+    # end-to-end error of the FP32 path at the headline workload
+    smw = chunk.to_numpy(rows=(yw, yw + h))[..., x0:x0 + w]
+    ref64 = oracle.pipeline64(smw, MT5_LAMBDAS, R, p.ds_factor, p.var_z, p.alpha, feats=fl)
+    win = 2 * R + 1
+    decw = np.zeros((h, w, win, win, 3), np.uint8)
+    decw[R:h - R, R:w - R] = dec
+    gpu = {"n": nw, "lnS2": lnS2, "se2": se2, "varD": gD, "varCD": gCD,
+           "filmD": filmD[:, yw:yw + h, x0:x0 + w], "decisions": decw}
+    rep = oracle.fp64_error_report(gpu, ref64, p.var_z, R, crop=True)
+    print("\n1080p 256 spp window vs FP64:", rep)
+    for name in ("varD", "varCD", "filmD"):
+        assert rep[name]["decisions_differ"] <= DECISION_MISMATCH_MAX, (name, rep[name])
+        assert rep[name]["max_rel"] <= FILTER_RTOL, (name, rep[name])
