@@ -167,21 +167,6 @@ def fp64_error(ctx, bs, chunk, feats, p, W, H, r, region):
     return rep
 
 
-def c4_images(n=48):
-    """The uneven-image batch: W ~ U[256, 2048], H ~ U[256, 1152], spp ~
-    LogU[16, 1024], T ~ U{1..5} (first T non-identity mt5 lambdas + identity),
-    per-image seeds 1000 + i."""
-    out = []
-    for i in range(n):
-        rng = np.random.default_rng(1000 + i)
-        W = int(rng.integers(256, 2049))
-        H = int(rng.integers(256, 1153))
-        spp = int(round(float(np.exp(rng.uniform(np.log(16), np.log(1024))))))
-        T = int(rng.integers(1, 6))
-        out.append((W, H, spp, T))
-    return out
-
-
 def bench_c4(args) -> int:
     """48 uneven images: the plain path (one image per launch, images one after
     another on one stream) against the batch (the same launches spread over 4
@@ -191,7 +176,8 @@ def bench_c4(args) -> int:
     with bit-identical buffers."""
     import stater_amd as sa
     from stater_amd.serve import job_lambdas
-    from stater_amd.workload import FEATURE_SDS, Features, SampleChunk, make_scene, synth_features, synth_samples
+    from stater_amd.workload import (FEATURE_SDS, Features, SampleChunk, c4_images, make_scene, synth_features,
+                                     synth_samples)
 
     chunk_spp = args.chunk_spp
     ctx = sa.Context(0)

@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define STATER_ABI_VERSION 1
+#define STATER_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define STATER_OK             0
@@ -359,12 +359,17 @@ int stater_synth_samples(stater_ctx *ctx, const stater_scene *s,
  * Replaces nothing in the reference (it renders one film per process). */
 typedef struct stater_batch stater_batch;
 typedef struct {
-    stater_bufset         *set;     /* full set (max_moment >= 4) of the scene's size */
-    const stater_scene    *scene;   /* seeded sample source (copied) */
+    stater_bufset         *set;     /* full set (max_moment >= 4) of the film's size */
+    const stater_scene    *scene;   /* seeded sample source (copied), used when samples is NULL */
     const stater_plane    *albedo;  /* the scene's albedo plane when it modulates */
     const stater_gbuffers *gbufs;   /* range-weight features or NULL (copied) */
     int32_t                spp;     /* samples per pixel of this image */
     int32_t                reserved;
+    const stater_samples  *samples; /* the caller's samples (a renderer's chunk, copied; its device
+                                       memory must stay valid while the batch runs): the first spp
+                                       of them are taken in passes of chunk_spp, counts and
+                                       count_offset as for stater_accumulate.  NULL: generated
+                                       from scene */
 } stater_batch_image;
 
 int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *images,
@@ -385,9 +390,17 @@ int stater_batch_destroy(stater_batch *b);
  * thread; each is admitted when its device memory fits the server's budget
  * (FIFO) and a slot is free.  Every slot is a worker with its own HIP stream,
  * so admitted jobs run concurrently and their kernels overlap on the device;
- * a job's statistics stay resident across its rounds.  Job memory comes from
- * one arena allocated at creation: admission never calls hipMalloc/hipFree.
- * A job's buffers are the same whether it runs alone or among others.
+ * a job's statistics stay resident across its rounds, and a kept job stays
+ * resident across requests: a continuation (continue_id) adds samples to it
+ * and filters again.  Job memory comes from one arena allocated at creation:
+ * admission never calls hipMalloc/hipFree.  A job's buffers are the same
+ * whether it runs alone or among others.
+ * Samples come from the job's seeded scene or from the caller (a renderer's
+ * chunk, device memory that must stay valid until the request is waited
+ * for).  The caller's adaptive loop: submit a kept job with its first chunk
+ * and next_spp, wait, read "targetspp" (stater_server_view), render that many
+ * samples per pixel, submit a continuation whose chunk takes the job's
+ * "targetspp" plane as its counts, and so on; release the job at the end.
  * Replaces nothing in the reference (it renders one film per process); it is
  * the serving form of the statistics-and-denoise path. */
 typedef struct stater_server stater_server;
@@ -416,7 +429,21 @@ typedef struct {
     int32_t  film;            /* 1: the renderer's loop: guarded samples, the set's pbrt Film
                                  (STATER_BUFSET_FILM) resolved and used as the Welch
                                  filter's film; 0: the radiance mean is the film */
-    int32_t  reserved[3];
+    int32_t  next_spp;        /* > 0: after the filters, the AS map of a next round of
+                                 next_spp x pixels samples (cap cap_spp) into "targetspp" */
+    uint64_t continue_id;     /* 0: a new job.  Else the id of a finished kept job whose
+                                 resident state this request continues: its size, lambdas,
+                                 film and features are the kept job's; with samples the
+                                 chunk is accumulated, without them `rounds` adaptive rounds
+                                 of `spp` x pixels samples are generated from its scene
+                                 (synthetic jobs); then the filters run again */
+    const stater_samples  *samples;  /* the caller's samples of this request (copied; one
+                                        accumulate call over the chunk, counts honoured), or
+                                        NULL: generated from the seeded scene */
+    const stater_gbuffers *gbufs;    /* a new caller-sample job's features (copied; planes of
+                                        the film's size kept valid while the job is kept), or
+                                        NULL: no range weights */
+    int32_t  reserved[2];
 } stater_job;
 
 typedef struct {
@@ -435,9 +462,11 @@ int64_t stater_server_job_bytes(const stater_server *s, const stater_job *job);
 int  stater_server_submit(stater_server *s, const stater_job *job, uint64_t *id);
 /* Blocks until the job has finished; fills res. */
 int  stater_server_wait(stater_server *s, uint64_t id, stater_job_result *res);
-/* A finished job's buffer by name (jobs submitted with keep = 1). */
+/* A finished job's buffer by name (jobs submitted with keep = 1; a
+ * continuation's results are its kept job's buffers). */
 int  stater_server_view(stater_server *s, uint64_t id, const char *name, stater_plane *out);
-/* Frees a kept job's buffers. */
+/* Frees a kept job's buffers (STATER_E_STATE while a continuation of it is
+ * pending or running). */
 int  stater_server_release(stater_server *s, uint64_t id);
 /* Blocks until every submitted job has finished. */
 int  stater_server_drain(stater_server *s);

@@ -58,7 +58,7 @@ class GBuffers(C.Structure):
 
 class BatchImage(C.Structure):
     _fields_ = [("set", C.c_void_p), ("scene", C.c_void_p), ("albedo", C.c_void_p), ("gbufs", C.c_void_p),
-                ("spp", C.c_int32), ("reserved", C.c_int32)]
+                ("spp", C.c_int32), ("reserved", C.c_int32), ("samples", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
@@ -72,7 +72,8 @@ class Job(C.Structure):
                 ("lambdas", C.c_float * MAX_LAMBDAS), ("spp", C.c_int32), ("rounds", C.c_int32),
                 ("cap_spp", C.c_uint32), ("scene_model", C.c_int32), ("seed", C.c_uint32), ("image_id", C.c_uint32),
                 ("feature_dtype", C.c_int32), ("keep", C.c_int32), ("film", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("next_spp", C.c_int32), ("continue_id", C.c_uint64), ("samples", C.c_void_p),
+                ("gbufs", C.c_void_p), ("reserved", C.c_int32 * 2)]
 
 
 class JobResult(C.Structure):
@@ -342,16 +343,21 @@ class BufferSet:
 class Batch:
     """Uneven films processed together over several streams (optionally one hipGraph).
 
-    images: list of (BufferSet, Scene, albedo Plane | None, GBuffers | None, spp)."""
+    images: list of (BufferSet, Scene | Samples, albedo Plane | None, GBuffers | None, spp): a
+    seeded scene the batch generates samples from, or a caller's sample chunk
+    (Samples, device memory kept valid while the batch runs)."""
 
     def __init__(self, ctx: Context, images, params: DenoiseParams, chunk_spp: int = 32):
         self.ctx, self.lib = ctx, ctx.lib
         self._keep = []
         arr = (BatchImage * len(images))()
-        for i, (bs, sc, alb, g, spp) in enumerate(images):
-            self._keep += [sc, alb, g]
-            arr[i] = BatchImage(bs.h.value, C.addressof(sc), C.addressof(alb) if alb is not None else None,
-                                C.addressof(g) if g is not None else None, spp, 0)
+        for i, (bs, src, alb, g, spp) in enumerate(images):
+            self._keep += [src, alb, g]
+            caller = isinstance(src, Samples)
+            arr[i] = BatchImage(bs.h.value, None if caller else C.addressof(src),
+                                C.addressof(alb) if alb is not None else None,
+                                C.addressof(g) if g is not None else None, spp, 0,
+                                C.addressof(src) if caller else None)
         h = C.c_void_p()
         check(self.lib.stater_batch_create(ctx.h, len(images), arr, C.byref(params), chunk_spp, C.byref(h)))
         self.h = h

@@ -1,6 +1,7 @@
 // batch.cpp — films of uneven size processed together.
 //
-// Every film's chain (passes of chunk_spp generated samples, each followed by
+// Every film's chain (passes of chunk_spp samples -- generated from the film's
+// seeded scene, or taken from the caller's sample chunk -- each followed by
 // accumulation, the first from an empty set; prepass and variance filter;
 // Welch filter) runs as the single-film launches, and the chains are spread
 // over a few HIP streams (longest chain first, onto the least loaded
@@ -40,6 +41,8 @@ struct stater_batch {
     struct Img {
         stater_bufset *bs;
         stater_scene sc;
+        stater_samples samples;  // the caller's chunk (has_samples) or unused
+        bool has_samples;
         stater_plane albedo;
         bool has_alb;
         stater_gbuffers g;
@@ -81,12 +84,25 @@ int welch(const stater_batch *b, const stater_batch::Img &I, hipStream_t st, int
     return launch_welch_filter(I.bs, &b->p, gb(I), film(I), nullptr, false, st, ty);
 }
 
+// Pass [off, off + spp) of the caller's chunk: its data from sample off on,
+// and its per-pixel counts (if any) shifted by off.
+stater_samples caller_pass(const stater_batch::Img &I, int off, int spp) {
+    stater_samples s = I.samples;
+    s.data = (const float *)((const char *)I.samples.data + (int64_t)off * I.samples.sample_stride_bytes);
+    s.spp = spp;
+    s.count_offset = I.samples.count_offset + off;
+    if (I.has_film) s.sanitize = 1;
+    return s;
+}
+
 // One film's chain on stream st.
 int enqueue_film(const stater_batch *b, const stater_batch::Img &I, hipStream_t st) {
     int rc = STATER_OK;
     for (int off = 0; off < I.spp && !rc; off += b->chunk_spp) {
-        const stater_samples s = chunk_desc(b, I, std::min(b->chunk_spp, I.spp - off));
-        rc = launch_synth_samples(b->ctx, &I.sc, I.has_alb ? &I.albedo : nullptr, (uint32_t)off, nullptr, &s, st);
+        const int k = std::min(b->chunk_spp, I.spp - off);
+        const stater_samples s = I.has_samples ? caller_pass(I, off, k) : chunk_desc(b, I, k);
+        if (!I.has_samples)
+            rc = launch_synth_samples(b->ctx, &I.sc, I.has_alb ? &I.albedo : nullptr, (uint32_t)off, nullptr, &s, st);
         // the first pass starts the film from an empty set (no clearing pass)
         if (!rc) rc = launch_accumulate(I.bs, &s, STATER_ACC_SEQUENTIAL | (off == 0 ? STATER_ACC_FRESH : 0), st);
     }
@@ -149,16 +165,25 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
     b->chunk_spp = chunk_spp;
     for (int i = 0; i < count; ++i) {
         const stater_batch_image &im = images[i];
-        if (!im.set || !im.scene || im.spp <= 0 || (im.set->flags & STATER_BUFSET_NO_DERIVED) || im.set->ctx != ctx ||
-            im.set->mm < 4 || im.scene->width != im.set->W || im.scene->height != im.set->H ||
-            (im.scene->albedo_modulation && (!im.albedo || !im.albedo->ptr))) {
+        const bool set_ok = im.set && im.spp > 0 && !(im.set->flags & STATER_BUFSET_NO_DERIVED) &&
+                            im.set->ctx == ctx && im.set->mm >= 4;
+        const bool src_ok =
+            im.samples ? (im.samples->data && im.spp <= im.samples->spp && im.samples->pitch_bytes % 4 == 0 &&
+                          im.samples->comp_stride_bytes % 4 == 0 && im.samples->sample_stride_bytes % 4 == 0 &&
+                          im.samples->pitch_bytes >= (int64_t)im.set->W * 4)
+                       : (im.scene && im.scene->width == im.set->W && im.scene->height == im.set->H &&
+                          (!im.scene->albedo_modulation || (im.albedo && im.albedo->ptr)));
+        if (!set_ok || !src_ok) {
             delete b;
             return fail(STATER_E_ARG, "stater_batch_create: image " + std::to_string(i) +
-                                          " needs a full set (max_moment >= 4) of its scene's size and a scene");
+                                          " needs a full set (max_moment >= 4) and a scene of its size or a "
+                                          "sample chunk of at least spp samples");
         }
         stater_batch::Img I{};
         I.bs = im.set;
-        I.sc = *im.scene;
+        I.has_samples = im.samples != nullptr;
+        if (I.has_samples) I.samples = *im.samples;
+        else I.sc = *im.scene;
         I.has_alb = im.albedo != nullptr;
         if (I.has_alb) I.albedo = *im.albedo;
         I.has_g = im.gbufs != nullptr;
@@ -196,7 +221,8 @@ int stater_batch_create(stater_ctx *ctx, int count, const stater_batch_image *im
         return fail_out(STATER_E_HIP, "event creation failed");
     for (int k = 0; k < nb; ++k) {
         size_t need = 0;
-        for (int i : b->order[k]) need = std::max(need, (size_t)chunk_spp * 5 * b->imgs[i].bs->plane * 4);
+        for (int i : b->order[k])
+            if (!b->imgs[i].has_samples) need = std::max(need, (size_t)chunk_spp * 5 * b->imgs[i].bs->plane * 4);
         hipStream_t s = nullptr;
         hipEvent_t e = nullptr;
         char *c = nullptr;

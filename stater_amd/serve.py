@@ -14,8 +14,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (MAX_LAMBDAS, STATER_ACC_SEQUENTIAL, STATER_BF16, STATER_F32, Context, Job, JobResult, Plane,
-               ServerConfig, check)
+from . import (MAX_LAMBDAS, STATER_ACC_SEQUENTIAL, STATER_BF16, STATER_F32, Context, GBuffers, Job, JobResult,
+               Plane, Samples, ServerConfig, check)
 from .workload import FEATURE_SDS, MT5_LAMBDAS
 
 NONID = tuple(l for l in MT5_LAMBDAS if l != 1.0)
@@ -43,12 +43,17 @@ class Server:
         h = C.c_void_p()
         check(self.lib.stater_server_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h
+        self._refs = {}  # caller structs a submitted request points to, until it is waited for
         ctx._users.add(self)
 
     @staticmethod
     def job(width: int, height: int, lambdas=MT5_LAMBDAS, spp: int = 64, rounds: int = 1, cap: int = 4096,
             model: int = 2, seed: int = 5, image_id: int = 0, bf16: bool = False, keep: bool = False,
-            film: bool = False) -> Job:
+            film: bool = False, samples: Samples | None = None, gbufs: GBuffers | None = None,
+            continue_id: int = 0, next_spp: int = 0) -> Job:
+        """A job: samples from the seeded scene (model, seed, image_id), or the
+        caller's chunk `samples` with features `gbufs`; continue_id continues
+        a kept job; next_spp > 0 computes the AS map of a next round."""
         j = Job()
         j.width, j.height, j.n_lambdas = width, height, len(lambdas)
         assert len(lambdas) <= MAX_LAMBDAS
@@ -59,6 +64,11 @@ class Server:
         j.feature_dtype = STATER_BF16 if bf16 else STATER_F32
         j.keep = 1 if keep else 0
         j.film = 1 if film else 0
+        j.next_spp = next_spp
+        j.continue_id = continue_id
+        j._refs = (samples, gbufs)
+        j.samples = C.addressof(samples) if samples is not None else None
+        j.gbufs = C.addressof(gbufs) if gbufs is not None else None
         return j
 
     def job_bytes(self, job: Job) -> int:
@@ -70,11 +80,15 @@ class Server:
     def submit(self, job: Job) -> int:
         jid = C.c_uint64()
         check(self.lib.stater_server_submit(self.h, C.byref(job), C.byref(jid)))
+        self._refs[jid.value] = getattr(job, "_refs", None)
         return jid.value
 
     def wait(self, jid: int) -> dict:
         r = JobResult()
-        check(self.lib.stater_server_wait(self.h, jid, C.byref(r)))
+        try:
+            check(self.lib.stater_server_wait(self.h, jid, C.byref(r)))
+        finally:
+            self._refs.pop(jid, None)
         return {"id": r.id, "status": r.status, "slot": r.slot, "t_submit": r.t_submit, "t_admit": r.t_admit,
                 "t_done": r.t_done, "samples": r.samples, "bytes": r.bytes}
 

@@ -109,3 +109,18 @@ def synth_samples(ctx: Context, sc, chunk: SampleChunk, first_sample: int = 0, f
     desc = chunk.desc(spp, counts=counts, count_offset=count_offset)
     check(lib.stater_synth_samples(ctx.h, C.byref(sc), C.byref(feats.albedo) if feats else None, first_sample,
                                    C.byref(first) if first else None, C.byref(desc), stream))
+
+
+def c4_images(n: int = 48) -> list[tuple[int, int, int, int]]:
+    """The uneven-image batch (W, H, spp, T): W ~ U[256, 2048], H ~ U[256, 1152],
+    spp ~ LogU[16, 1024], T ~ U{1..5} (the first T non-identity mt5 lambdas
+    plus the identity), image i drawn with seed 1000 + i."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(1000 + i)
+        W = int(rng.integers(256, 2049))
+        H = int(rng.integers(256, 1153))
+        spp = int(round(float(np.exp(rng.uniform(np.log(16), np.log(1024))))))
+        T = int(rng.integers(1, 6))
+        out.append((W, H, spp, T))
+    return out

@@ -95,13 +95,11 @@ def test_batch_of_film_sets_runs_the_renderer_loop(ctx):
         got = _outputs(bs, lam)
         got["film"] = hipmem.read_plane(bs.view("film"))
         solo = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM)
-        # the Film merges each accumulate call's sums (one pbrt tile merge per
-        # call), so the single-image calls take the batch's 16-sample passes
-        chunk = SampleChunk(W, H, 16)
-        for off in range(0, spp, 16):
-            k = min(16, spp - off)
-            synth_samples(ctx, sc, chunk, off, feats, spp=k)
-            solo.accumulate(chunk.desc(spp=k, sanitize=True))
+        # one call with all the samples: the Film's sums run over the render,
+        # so the batch's 16-sample passes leave the same Film
+        chunk = SampleChunk(W, H, spp)
+        synth_samples(ctx, sc, chunk, 0, feats)
+        solo.accumulate(chunk.desc(sanitize=True))
         solo.denoise_vars(p, g)
         solo.film_resolve()
         solo.denoise_film(p, g, film=solo.view("film"))
@@ -111,5 +109,56 @@ def test_batch_of_film_sets_runs_the_renderer_loop(ctx):
         _same(got, want)
         # the Welch filter saw the Film, not the radiance mean
         assert not np.array_equal(want["film"], want["stats"][0, 0])
+        solo.close()
+    batch.close()
+
+
+def test_batch_takes_caller_sample_chunks(ctx):
+    """Films whose samples come from the caller (a renderer's chunks, one with
+    ragged per-pixel counts) beside a film generated from its scene: each
+    equals the single-image calls on the same samples."""
+    p = sa.default_params(radius=5)
+    shapes = [(90, 60, 3, 40, True, False), (150, 44, 2, 33, False, True), (64, 64, 5, 24, False, False)]
+    imgs, solo_in = [], []
+    for i, (W, H, T, spp, film, ragged) in enumerate(shapes):
+        sc = make_scene(2, 3000 + i, W, H, image_id=i)
+        feats = Features(W, H)
+        synth_features(ctx, sc, feats)
+        g = feats.gbuffers(FEATURE_SDS)
+        bs = sa.BufferSet(ctx, W, H, _image_lambdas(T), 5, flags=sa.STATER_BUFSET_FILM if film else 0)
+        counts = None
+        if i < 2:  # the caller's chunk
+            chunk = SampleChunk(W, H, spp)
+            synth_samples(ctx, sc, chunk, 0, feats)
+            if ragged:
+                cbuf = hipmem.DeviceBuffer(W * H * 4)
+                counts = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+                hipmem.write_plane(counts, np.random.default_rng(i).integers(0, spp + 3, (1, H, W)).astype(np.int32))
+                solo_in.append((chunk, cbuf))
+            desc = chunk.desc(counts=counts, sanitize=film)
+            imgs.append((bs, desc, None, g, spp))
+            solo_in.append((sc, feats, chunk, desc))
+        else:
+            imgs.append((bs, sc, feats.albedo, g, spp))
+            solo_in.append((sc, feats, None, None))
+    batch = sa.Batch(ctx, imgs, p, chunk_spp=16)
+    batch.run(use_graph=True)
+    ctx.synchronize()
+    entries = [e for e in solo_in if len(e) == 4]
+    for (bs, _, _, g, spp), (sc, feats, chunk, desc), (W, H, T, _, film, _) in zip(imgs, entries, shapes):
+        lam = _image_lambdas(T)
+        got = _outputs(bs, lam)
+        solo = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM if film else 0)
+        if chunk is None:
+            chunk = SampleChunk(W, H, spp)
+            synth_samples(ctx, sc, chunk, 0, feats)
+            desc = chunk.desc()
+        solo.accumulate(desc)
+        solo.denoise_vars(p, g)
+        if film:
+            solo.film_resolve()
+        solo.denoise_film(p, g, film=solo.view("film") if film else None)
+        ctx.synchronize()
+        _same(got, _outputs(solo, lam))
         solo.close()
     batch.close()

@@ -94,10 +94,9 @@ def test_film_job_runs_the_renderer_loop(ctx):
     feats = Features(W, H)
     synth_features(ctx, sc, feats)
     bs = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM)
-    chunk = SampleChunk(W, H, 16)
-    for off in range(0, 48, 16):  # the server's 16-sample passes: one Film merge each
-        synth_samples(ctx, sc, chunk, off, feats)
-        bs.accumulate(chunk.desc(sanitize=True))
+    chunk = SampleChunk(W, H, 48)  # one call: the server's 16-sample passes leave the same Film
+    synth_samples(ctx, sc, chunk, 0, feats)
+    bs.accumulate(chunk.desc(sanitize=True))
     p = sa.default_params(radius=6)
     g = feats.gbuffers(FEATURE_SDS)
     bs.denoise_vars(p, g)
@@ -128,6 +127,90 @@ def test_adaptive_rounds_match_the_round_driver(ctx):
     want = {n: hipmem.read_plane(bs.view(n)) for n in got}
     _same(got, want)
     bs.close()
+
+
+def test_caller_samples_and_continuations_match_the_single_image_path(ctx):
+    """A renderer's adaptive loop through the server: a kept job fed the
+    caller's first chunk (and features) with next_spp, then a continuation
+    whose chunk holds the samples its targetspp asked for (counts = the job's
+    targetspp), each request equal bit for bit to the single-image calls."""
+    W, H, lam, r = 120, 80, job_lambdas(3), 6
+    sc = make_scene(2, 5, W, H, image_id=31)
+    feats = Features(W, H)
+    synth_features(ctx, sc, feats)
+    g = feats.gbuffers(FEATURE_SDS)
+    first = SampleChunk(W, H, 24)
+    synth_samples(ctx, sc, first, 0, feats)
+    srv = _server(ctx, slots=2)
+    jid = srv.submit(Server.job(W, H, lam, spp=24, keep=True, film=True, samples=first.desc(), gbufs=g,
+                                next_spp=16, cap=300))
+    other = srv.submit(Server.job(96, 64, job_lambdas(2), spp=32, seed=5, image_id=3))  # beside it
+    res = srv.wait(jid)
+    assert res["status"] == 0 and res["samples"] == 24 * W * H and srv.wait(other)["status"] == 0
+    names = ("n", "filmD", "film", "t0-b0-film-varD", "t0-b1-m2", "targetspp")
+    got1 = _read(srv, jid, names)
+
+    # the single-image calls of the same request
+    bs = sa.BufferSet(ctx, W, H, lam, 5, flags=sa.STATER_BUFSET_FILM)
+    p = sa.default_params(radius=r)
+
+    def request():
+        bs.denoise_vars(p, g)
+        bs.film_resolve()
+        bs.denoise_film(p, g, film=bs.view("film"))
+        bs.calc_target_spp(16, 300)
+        ctx.synchronize()
+        return {n: hipmem.read_plane(bs.view(n)) for n in names}
+
+    bs.accumulate(first.desc(sanitize=True))
+    _same(got1, request())
+
+    # round 2: the renderer takes targetspp samples per pixel, continuing each pixel's sequence
+    tgt = got1["targetspp"][0]
+    more = SampleChunk(W, H, int(tgt.max()))
+    tplane = srv.view(jid, "targetspp")
+    synth_samples(ctx, sc, more, 0, feats, first=srv.view(jid, "n"), counts=tplane)
+    ctx.synchronize()
+    cid = srv.submit(Server.job(0, 0, lam, spp=1, samples=more.desc(counts=tplane), continue_id=jid,
+                                next_spp=16, cap=300))
+    res2 = srv.wait(cid)
+    assert res2["status"] == 0 and res2["samples"] == int(tgt.sum())
+    got2 = _read(srv, jid, names)
+    assert np.array_equal(got2["n"][0], got1["n"][0] + tgt)
+    srv.release(jid)
+    srv.close()
+    tbuf = hipmem.DeviceBuffer(W * H * 4)
+    tp = sa.plane(tbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+    hipmem.write_plane(tp, tgt[None])
+    bs.accumulate(more.desc(counts=tp, sanitize=True))
+    _same(got2, request())
+    bs.close()
+
+
+def test_synthetic_job_continued_equals_one_longer_job(ctx):
+    """A kept synthetic job of one round, continued by a request of two more
+    adaptive rounds on its resident state, ends where a three-round job does."""
+    W, H = 96, 64
+    kw = dict(width=W, height=H, lambdas=job_lambdas(2), spp=16, cap=200, seed=5, image_id=12)
+    srv = _server(ctx, slots=1)
+    a = srv.submit(Server.job(**kw, rounds=3, keep=True))
+    b = srv.submit(Server.job(**kw, rounds=1, keep=True))
+    ra, rb = srv.wait(a), srv.wait(b)
+    long = srv.submit(Server.job(256, 192, job_lambdas(5), spp=512, seed=5, image_id=13))  # holds the one slot
+    c = srv.submit(Server.job(0, 0, kw["lambdas"], spp=16, rounds=2, cap=200, continue_id=b))
+    with pytest.raises(sa.StaterError):
+        srv.release(b)  # its continuation is queued behind the long job
+    rc = srv.wait(c)
+    assert srv.wait(long)["status"] == 0
+    assert ra["status"] == rb["status"] == rc["status"] == 0
+    assert rb["samples"] + rc["samples"] == ra["samples"] == 3 * 16 * W * H
+    names = ("n", "filmD", "t0-b0-film-m2", "t0-b1-m4", "t0-b0-film-varCD")
+    _same(_read(srv, a, names), _read(srv, b, names))
+    with pytest.raises(sa.StaterError):
+        srv.submit(Server.job(0, 0, kw["lambdas"], spp=16, continue_id=999))  # no such kept job
+    srv.release(a)
+    srv.release(b)
+    srv.close()
 
 
 def test_admission_stays_within_budget_in_arrival_order(ctx):
