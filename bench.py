@@ -303,7 +303,7 @@ def bench_c5(args) -> int:
     from stater_amd.serve import Mix, Server, run_poisson
 
     ctx = sa.Context(0)
-    budget = 128 << 30
+    budget = int(args.budget_gb * 1e9)
     mix = Mix()
     rates = [float(x) for x in args.rates.split(",")]
     p = sa.default_params(radius=args.radius)
@@ -329,8 +329,8 @@ def bench_c5(args) -> int:
         "speedup_vs_one_at_a_time": round(sat["mpixel_s"] / plain["mpixel_s"], 3),
         "saturated": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items()},
         "sweep": sweep, "dtype": "f32", "data": "synthetic, device-generated per job",
-        "config": {"workload": "Poisson arrivals, jobs %d per rate, admission budget %d GiB, sanitised samples + Film"
-                               % (args.jobs, budget >> 30)},
+        "config": {"workload": "Poisson arrivals, %d jobs per rate, admission budget %.0f GB, sanitised samples + "
+                               "Film, r=%d" % (args.jobs, budget / 1e9, args.radius)},
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -356,8 +356,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: the headline 1080p frame; c4: 48 uneven images, batched vs one image per launch; "
                          "c5: served Poisson arrivals vs one job at a time")
-    ap.add_argument("--jobs", type=int, default=32, help="c5: jobs per arrival rate")
-    ap.add_argument("--rates", default="4,8,16", help="c5: offered arrival rates, jobs/s")
+    ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
+    ap.add_argument("--rates", default="20,40,100,200,400", help="c5: offered arrival rates, jobs/s")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="c5: admission budget (device memory), GB")
     ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
     ap.add_argument("--chunk-spp", type=int, default=64, help="c4, c5: samples per pixel of one accumulation pass")
     ap.add_argument("--pass-spp", type=int, default=256, help="c3: samples per pixel of one accumulation pass")
