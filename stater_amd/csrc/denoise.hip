@@ -134,16 +134,18 @@ struct VarRec {
 
 // Ring slot: the halo row's HWC records, rounded up to whole 64-item loads
 // (the loader fills everything past the halo with zeros).  The ring holds the
-// 2 TY - 1 live rows and kVarAhead rows in flight.
+// 2 TY - 1 live rows and AHEAD rows in flight: two for r <= 10, one for the
+// wider windows, whose steps (2r + 1 columns) outlast a row's loads many
+// times over and whose wider halo rows would not leave room for 8-row tiles.
 constexpr int kVarAhead = 2;
 template <int G, bool HASW, int HWC>
 constexpr int var_loads_per_row() { return (HWC * VarRec<G, HASW>::FLOAT4S + 63) / 64; }
 template <int G, bool HASW, int HWC>
 constexpr int var_slot_float4s() { return var_loads_per_row<G, HASW, HWC>() * 64; }
-template <int TY>
-constexpr int var_ring_slots() { return 2 * TY - 1 + kVarAhead; }
-template <int G, bool HASW, int HWC, int TY>
-constexpr int var_lds_bytes() { return var_ring_slots<TY>() * var_slot_float4s<G, HASW, HWC>() * 16; }
+template <int TY, int AHEAD = kVarAhead>
+constexpr int var_ring_slots() { return 2 * TY - 1 + AHEAD; }
+template <int G, bool HASW, int HWC, int TY, int AHEAD>
+constexpr int var_lds_bytes() { return var_ring_slots<TY, AHEAD>() * var_slot_float4s<G, HASW, HWC>() * 16; }
 
 // s_waitcnt immediates that wait for vector-memory count <= v, or LDS / scalar
 // count <= v, only.
@@ -472,21 +474,21 @@ __device__ __forceinline__ void var_load_row(const VarArgs &a, float4 *dst, int 
 // One tile of 64 x 2 TY pixels per workgroup: TY filtering wavefronts (two
 // rows each) and one loader wavefront.  Wavefront ty reads row 2 ty + k at
 // step k, so rows k .. k + P - 1 (P = 2 TY - 1) are live during step k while
-// the loader keeps kVarAhead rows in flight; row j lives in slot j mod
-// (P + kVarAhead).  Both roles pass the same 2r + 3 barriers: the first once
+// the loader keeps AHEAD rows in flight; row j lives in slot j mod
+// (P + AHEAD).  Both roles pass the same 2r + 3 barriers: the first once
 // rows 0 .. P - 1 have landed, then one at the end of every step k, by which
 // row k + P has landed (it replaces nothing live: its slot held row k - 1 or
 // was never used).  Tiles of 4 and 8 rows are built for the r <= 10
 // feature-weighted filter (chosen per film by the batch tuner); everything
 // else runs 4 rows.
-template <int G, bool HASW, int HWC, int TY>
+template <int G, bool HASW, int HWC, int TY, int AHEAD>
 __global__ void __launch_bounds__(64 * (TY + 1))
-    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY>(), 64 * (TY + 1)))))
-    k_var_filter(VarArgs a) {
+    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY, AHEAD>(),
+                                                             64 * (TY + 1))))) k_var_filter(VarArgs a) {
     extern __shared__ float4 vring[];
-    constexpr int P = 2 * TY - 1, NS = var_ring_slots<TY>();
+    constexpr int P = 2 * TY - 1, NS = var_ring_slots<TY, AHEAD>();
     constexpr int SLOT = var_slot_float4s<G, HASW, HWC>();
-    constexpr int AHEAD_LOADS = var_loads_per_row<G, HASW, HWC>() * (kVarAhead - 1);
+    constexpr int AHEAD_LOADS = var_loads_per_row<G, HASW, HWC>() * (AHEAD - 1);
     const int tiles_x = (a.W + TX - 1) / TX;
     const int t = xcd_tile(tiles_x * ((a.H + 2 * TY - 1) / (2 * TY)));
     if (t < 0) return;
@@ -498,12 +500,12 @@ __global__ void __launch_bounds__(64 * (TY + 1))
     // s_barrier: a workgroup fence there would wait for the rows still in
     // flight too.  Its own counter wait covers the rows the next step reads.
     if (wave == TY) {  // loader
-        for (int j = 0; j < P + kVarAhead - 1; ++j)
+        for (int j = 0; j < P + AHEAD - 1; ++j)
             var_load_row<G, HASW, HWC>(a, vring + j * SLOT, y0 - a.r + j, x0, lane);
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // rows 0 .. P - 1 have landed
         __builtin_amdgcn_s_barrier();
         for (int k = 0; k <= last; ++k) {
-            const int j = k + P + kVarAhead - 1;
+            const int j = k + P + AHEAD - 1;
             var_load_row<G, HASW, HWC>(a, vring + (j % NS) * SLOT, y0 - a.r + j, x0, lane);
             __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // row k + P has landed
             __builtin_amdgcn_s_barrier();
@@ -527,16 +529,17 @@ __global__ void __launch_bounds__(64 * (TY + 1))
     var_store<G, HASW>(a, x, yA, actA, actB, A, B);
 }
 
-template <int G, bool HASW, int HWC, int TY>
+template <int G, bool HASW, int HWC, int TY, int AHEAD = 2>
 static int var_launch4(const VarArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
                        const_cast<float4 *>(a.zero));
     STATER_HIP(hipGetLastError());
-    const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY>();
-    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC, TY>,
+    const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY, AHEAD>();
+    static_assert(var_lds_bytes<G, HASW, HWC, TY, AHEAD>() <= 160 * 1024, "ring larger than a CU's LDS");
+    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter<G, HASW, HWC, TY, AHEAD>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + 2 * TY - 1) / (2 * TY))));
-    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC, TY>), grid, dim3(64 * (TY + 1)), lds, st, a);
+    hipLaunchKernelGGL((k_var_filter<G, HASW, HWC, TY, AHEAD>), grid, dim3(64 * (TY + 1)), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
@@ -544,7 +547,8 @@ static int var_launch4(const VarArgs &a, hipStream_t st) {
 // rows: output rows per tile (4 or 8; two per filtering wavefront).
 template <int G>
 static int var_launch(const VarArgs &a, bool hasw, int rows, hipStream_t st) {
-    if (a.r > 10) return hasw ? var_launch4<G, true, TX + 40, 2>(a, st) : var_launch4<G, false, TX + 40, 2>(a, st);
+    if (a.r > 10)
+        return hasw ? var_launch4<G, true, TX + 40, 4, 1>(a, st) : var_launch4<G, false, TX + 40, 4, 1>(a, st);
     if (!hasw) return var_launch4<G, false, TX + 20, 2>(a, st);
     return rows == 8 ? var_launch4<G, true, TX + 20, 4>(a, st) : var_launch4<G, true, TX + 20, 2>(a, st);
 }
@@ -1159,19 +1163,21 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
     decide_tile<S, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
 }
 
-// Pass 2 LDS slot: features, film[3], then the 3 x kBitsWords decision words.
-template <bool HASW>
+// Pass 2 LDS slot: features, film[3], then the 3 x NW decision words of the
+// window's radius (NW = bits_words(r) for the largest r the tile width holds).
+template <bool HASW, int HWC>
 struct AccumLayout {
     static constexpr int FILM = HASW ? kNF : 0;
     static constexpr int WORDS = FILM + 3;
-    static constexpr int FLOATS = WORDS + 3 * kBitsWords;
+    static constexpr int NW = bits_words((HWC - TX) / 2);
+    static constexpr int FLOATS = WORDS + 3 * NW;
 };
 
 template <bool HASW, int HWC>
 __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot, int yq, int x0) {
-    using LY = AccumLayout<HASW>;
+    using LY = AccumLayout<HASW, HWC>;
     const int HW = TX + 2 * a.r;
-    const int items = HW * (1 + 3 * kBitsWords);
+    const int items = HW * (1 + 3 * LY::NW);
     uint32_t *words = (uint32_t *)(slot + LY::WORDS * HWC);
     for (int i = threadIdx.x; i < items; i += blockDim.x) {
         const int kind = i / HW, col = i - kind * HW;
@@ -1188,8 +1194,8 @@ __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot,
                                           c * a.film_cs)
                        : 0.f;
         } else {
-            const int w = kind - 1;  // c * kBitsWords + k
-            words[w * HWC + col] = in ? a.bits[(size_t)w * a.plane + (size_t)yq * a.ld + xq] : 0u;
+            const int w = kind - 1, c = w / LY::NW, k = w - c * LY::NW;  // c * NW + k
+            words[w * HWC + col] = in ? a.bits[((size_t)c * kBitsWords + k) * a.plane + (size_t)yq * a.ld + xq] : 0u;
         }
     }
 }
@@ -1198,21 +1204,24 @@ __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot,
 template <bool HASW, int HWC>
 __device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot, int tx, int x, int y, int dy,
                                           const float (&gp)[kNF], float (&den)[3], float (&num)[3], size_t pix) {
-    using LY = AccumLayout<HASW>;
+    using LY = AccumLayout<HASW, HWC>;
     const int win = 2 * a.r + 1;
     const float spy = a.sp_scale * (float)(dy * dy);
     const uint32_t *words = (const uint32_t *)(slot + LY::WORDS * HWC);
-    // this pixel's own words for the forward offsets of the row (at most two per channel)
-    uint32_t own0[3] = {0u, 0u, 0u}, own1[3] = {0u, 0u, 0u};
+    // this pixel's own words for the forward offsets of the row: its 2r + 1
+    // consecutive indices span at most three words (two while 2r + 1 <= 33)
+    uint32_t own0[3] = {0u, 0u, 0u}, own1[3] = {0u, 0u, 0u}, own2[3] = {0u, 0u, 0u};
     int k0 = 0;
     if (dy >= 0) {
         const int i0 = fwd_index(dy == 0 ? 1 : -a.r, dy, a.r), i1 = fwd_index(a.r, dy, a.r);
         k0 = i0 >> 5;
+        const int k1 = i1 >> 5;
         if (a.r > 0)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 own0[c] = a.bits[((size_t)c * kBitsWords + k0) * a.plane + pix];
-                own1[c] = (i1 >> 5) != k0 ? a.bits[((size_t)c * kBitsWords + k0 + 1) * a.plane + pix] : 0u;
+                own1[c] = k1 > k0 ? a.bits[((size_t)c * kBitsWords + k0 + 1) * a.plane + pix] : 0u;
+                own2[c] = k1 > k0 + 1 ? a.bits[((size_t)c * kBitsWords + k0 + 2) * a.plane + pix] : 0u;
             }
     }
     for (int dx = -a.r; dx <= a.r; ++dx) {
@@ -1228,7 +1237,7 @@ __device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot,
         for (int c = 0; c < 3; ++c) filmq[c] = q[(LY::FILM + c) * HWC];
         if (!fwd)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) wq[c] = words[(c * kBitsWords + (idx >> 5)) * HWC + tx + a.r + dx];
+            for (int c = 0; c < 3; ++c) wq[c] = words[(c * LY::NW + (idx >> 5)) * HWC + tx + a.r + dx];
         __builtin_amdgcn_sched_barrier(0);
         float w = 1.f;
         if (HASW) {
@@ -1243,7 +1252,8 @@ __device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot,
         bool acc[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const uint32_t wd = fwd ? ((idx >> 5) == k0 ? own0[c] : own1[c]) : wq[c];
+            const int kw = (idx >> 5) - k0;
+            const uint32_t wd = fwd ? (kw == 0 ? own0[c] : kw == 1 ? own1[c] : own2[c]) : wq[c];
             acc[c] = (wd >> (idx & 31)) & 1u;
             const float wa = acc[c] ? w : 0.f;
             den[c] += wa;
@@ -1258,7 +1268,7 @@ __device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot,
 
 template <bool HASW, int HWC, int TY>
 __device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, float *ring) {
-    constexpr int SLOT = AccumLayout<HASW>::FLOATS * HWC;
+    constexpr int SLOT = AccumLayout<HASW, HWC>::FLOATS * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int x0 = bx * TX, y0 = by * TY;
     const int x = x0 + tx, y = y0 + ty;
@@ -1300,7 +1310,7 @@ __device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, fl
 
 template <bool HASW, int HWC, int TY>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
-    1, lds_waves_per_simd(TY * AccumLayout<HASW>::FLOATS * HWC * 4, 64 * TY)))) k_welch_accum(WelchArgs a) {
+    1, lds_waves_per_simd(TY * AccumLayout<HASW, HWC>::FLOATS * HWC * 4, 64 * TY)))) k_welch_accum(WelchArgs a) {
     extern __shared__ float ring[];
     const int tiles_x = (a.W + TX - 1) / TX;
     const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
@@ -1321,7 +1331,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
 template <bool HASW>
 struct AccRec {
     static constexpr int HEAD4 = HASW ? 3 : 1;                  // float4 items of film (+ features)
-    static constexpr int RAW4 = HEAD4 + kBitsWords;             // + one item per word group
+    static constexpr int RAW4 = HEAD4 + kRecWords;              // + one item per word group
     static constexpr int FLOAT4S = RAW4 % 2 ? RAW4 : RAW4 + 1;  // 11 or 9: odd
     static_assert(FLOAT4S * 4 <= kVarRecFloats, "record larger than its planes");
 };
@@ -1333,7 +1343,7 @@ constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<
 template <bool HASW>
 struct AccPix {
     float gp[kNF], den[3], num[3];
-    uint32_t own[3][kBitsWords];
+    uint32_t own[3][kRecWords];
 };
 
 // The pixel's own film, features and decision words come from its record in
@@ -1343,11 +1353,11 @@ __device__ __forceinline__ void acc_pix_init(const WelchArgs &a, int x, int y, b
     using R = AccRec<HASW>;
     const float4 *q = active ? a.rec + ((size_t)y * a.ld + x) * R::FLOAT4S : a.zero;
     float4 h[R::HEAD4];
-    uint4 wd[kBitsWords];
+    uint4 wd[kRecWords];
 #pragma unroll
     for (int i = 0; i < R::HEAD4; ++i) h[i] = q[i];
 #pragma unroll
-    for (int w = 0; w < kBitsWords; ++w) wd[w] = ((const uint4 *)q)[R::HEAD4 + w];
+    for (int w = 0; w < kRecWords; ++w) wd[w] = ((const uint4 *)q)[R::HEAD4 + w];
     const float gp[kNF] = {h[0].w, h[HASW ? 1 : 0].x, h[HASW ? 1 : 0].y, h[HASW ? 1 : 0].z,
                            h[HASW ? 1 : 0].w, h[HASW ? 2 : 0].x, h[HASW ? 2 : 0].y};
 #pragma unroll
@@ -1358,7 +1368,7 @@ __device__ __forceinline__ void acc_pix_init(const WelchArgs &a, int x, int y, b
         P.den[c] = 1.f;  // the pixel itself, weight 1
         P.num[c] = film[c];
 #pragma unroll
-        for (int w = 0; w < kBitsWords; ++w) P.own[c][w] = c == 0 ? wd[w].x : c == 1 ? wd[w].y : wd[w].z;
+        for (int w = 0; w < kRecWords; ++w) P.own[c][w] = c == 0 ? wd[w].x : c == 1 ? wd[w].y : wd[w].z;
     }
 }
 
@@ -1398,8 +1408,8 @@ __device__ __forceinline__ AccRowP acc_row_p(const WelchArgs &a, AccPix<HASW> &P
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
 #pragma unroll
-            for (int j = 0; j + 1 < kBitsWords; ++j) P.own[ch][j] = P.own[ch][j + 1];
-            P.own[ch][kBitsWords - 1] = 0u;
+            for (int j = 0; j + 1 < kRecWords; ++j) P.own[ch][j] = P.own[ch][j + 1];
+            P.own[ch][kRecWords - 1] = 0u;
         }
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -1582,7 +1592,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 
 template <bool HASW>
 static int accum2_launch(const WelchArgs &a, hipStream_t st) {
-    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = 4;
+    constexpr int HWC = TX + 2 * kRecBitsRadius, TY = 4;
     const size_t lds = (size_t)acc_lds_bytes<HASW, HWC, TY>();
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum2<HASW, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1592,15 +1602,17 @@ static int accum2_launch(const WelchArgs &a, hipStream_t st) {
     return STATER_OK;
 }
 
-template <int S>
+// HWC: halo row width of the radius class, TX + 20 (r <= 10, where the
+// weighting pass may take records) or TX + 40 (r <= 20, bit planes only).
+template <int S, int HWC>
 static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
-    constexpr int HWC = TX + 2 * kBitsMaxRadius, TY = kTY, DTY = kDecideTY;
+    constexpr int TY = kTY, DTY = kDecideTY;
     const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
     const dim3 dgrid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + DTY - 1) / DTY)));
     size_t lds = (size_t)DTY * DecideLayout<S>::FLOATS * HWC * 4;
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, DTY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (!a.dec && a.rec) {  // the decision pass writes the weighting pass's records itself
+    if (!a.dec && a.rec && a.r <= kRecBitsRadius) {  // the decision pass writes the weighting pass's records itself
         WelchArgs b = a;
         b.rec_f4 = hasw ? AccRec<true>::FLOAT4S : AccRec<false>::FLOAT4S;
         b.rec_head = hasw ? AccRec<true>::HEAD4 : AccRec<false>::HEAD4;
@@ -1611,12 +1623,12 @@ static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
     hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY>), dgrid, dim3(64 * DTY), lds, st, a);
     STATER_HIP(hipGetLastError());
     if (hasw) {
-        lds = (size_t)TY * AccumLayout<true>::FLOATS * HWC * 4;
+        lds = (size_t)TY * AccumLayout<true, HWC>::FLOATS * HWC * 4;
         STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<true, HWC, TY>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL((k_welch_accum<true, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
     } else {
-        lds = (size_t)TY * AccumLayout<false>::FLOATS * HWC * 4;
+        lds = (size_t)TY * AccumLayout<false, HWC>::FLOATS * HWC * 4;
         STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum<false, HWC, TY>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL((k_welch_accum<false, HWC, TY>), grid, dim3(64 * TY), lds, st, a);
@@ -1724,14 +1736,14 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     const bool hasw = a.f.nf > 0 || a.sp_scale != 0.f;
     LaunchTimer t(bs->ctx, st, plain ? "welch_plain" : "welch_filter");
     if (plain) return welch_launch3<1, false, true, false>(a, kTY, st);
-    if (!a.mode3 && a.bits && a.r >= 1 && a.r <= kBitsMaxRadius) {  // mode 2: each pair tested once
+    if (!a.mode3 && a.bits && a.r >= 1 && a.r <= kRecBitsRadius) {  // mode 2: each pair tested once
         switch (S) {
-            case 1: return welch_sym_launch<1>(a, hasw, st);
-            case 2: return welch_sym_launch<2>(a, hasw, st);
-            case 3: return welch_sym_launch<3>(a, hasw, st);
-            case 4: return welch_sym_launch<4>(a, hasw, st);
-            case 5: return welch_sym_launch<5>(a, hasw, st);
-            default: return welch_sym_launch<6>(a, hasw, st);
+            case 1: return welch_sym_launch<1, TX + 2 * kRecBitsRadius>(a, hasw, st);
+            case 2: return welch_sym_launch<2, TX + 2 * kRecBitsRadius>(a, hasw, st);
+            case 3: return welch_sym_launch<3, TX + 2 * kRecBitsRadius>(a, hasw, st);
+            case 4: return welch_sym_launch<4, TX + 2 * kRecBitsRadius>(a, hasw, st);
+            case 5: return welch_sym_launch<5, TX + 2 * kRecBitsRadius>(a, hasw, st);
+            default: return welch_sym_launch<6, TX + 2 * kRecBitsRadius>(a, hasw, st);
         }
     }
     switch (S) {

@@ -71,9 +71,18 @@ struct stater_bufset {
 namespace stater {
 
 // The symmetric Welch filter stores one decision bit per channel for every
-// forward neighbour offset (half of the (2r+1)^2 - 1 window) of every pixel.
+// forward neighbour offset (half of the (2r+1)^2 - 1 window) of every pixel:
+// 32-bit words per channel for a radius r.  The set's bit planes cover
+// r <= kBitsMaxRadius, as does the weighting pass that reads them from
+// per-pixel records (kRecBitsRadius).  Wider windows test every ordered pair
+// in one pass: their once-per-pair form, measured at r = 20 with bit planes
+// for 27 words and a plane-reading weighting pass, was slower (51.7 against
+// 32.7 ms at 1080p mt5).
+constexpr int bits_words(int r) { return (2 * r * r + 2 * r + 31) / 32; }
 constexpr int kBitsMaxRadius = 10;
-constexpr int kBitsWords = (2 * kBitsMaxRadius * kBitsMaxRadius + 2 * kBitsMaxRadius + 31) / 32;  // 7
+constexpr int kBitsWords = bits_words(kBitsMaxRadius);  // 7
+constexpr int kRecBitsRadius = 10;
+constexpr int kRecWords = bits_words(kRecBitsRadius);  // 7
 
 // Adaptive-sampling scratch of a set: control block, then q, h and inc
 // (4 B per pixel each), then one int64 per 256-pixel block.
