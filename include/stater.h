@@ -139,11 +139,21 @@ int stater_bufset_destroy(stater_bufset *bs);
 /* Zero the accumulated statistics (n and every moment) on the stream. */
 int stater_bufset_reset(stater_bufset *bs, void *stream);
 /* Look up a buffer by the reference's name: "n", "film", "filmD", "targetspp",
- * "t0-b<j>-film-<stat>" (untransformed statistics of lambda slot j) or
- * "t0-b<j>-<stat>" (Box-Cox statistics of slot j), stat in
+ * "t<type>-b<j>-film-<stat>" (untransformed statistics of lambda slot j) or
+ * "t<type>-b<j>-<stat>" (Box-Cox statistics of slot j), type the set's stat
+ * type (0 unless stater_bufset_set_stat_type changed it), stat in
  * mean m2 m3 m4 c2X c2Y m1 jb lnS2 se2 m2C varD varCD discr meanVar meanD n.
  * "film" is the resolved Film with STATER_BUFSET_FILM, else the set-0 mean. */
 int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out);
+/* The set's stat type, the reference's StatTypeIndex (src/statistics/
+ * statpath.h:28-37): 0 Radiance (the default), 1/2 MIS win rates,
+ * 3 material id, 4 depth, 5 normal, 6 albedo, 7 iteration radiance.  Its
+ * buffers are named "t<type>-b<j>-..." (estimator.cpp:21-22): a G-buffer set
+ * (no lambdas, max_moment 1, fed one feature value per sample as
+ * statpath.cpp:366-379 does) of type 5 answers to "t5-b0-film-mean", the
+ * normal buffer the reference's denoiser reads.  Types 1..4 are
+ * single-channel (StatTile<Float>): their views hold channel 0. */
+int stater_bufset_set_stat_type(stater_bufset *bs, int type);
 /* Copy a plane into another of the same shape, converting the layout
  * (PLANAR <-> INTERLEAVED, e.g. a caller's CV_32FC3 film) and the storage type
  * (f32 <-> bf16, round to nearest even).  Feature planes may be bf16. */
@@ -258,6 +268,8 @@ int stater_denoise_vars(stater_bufset *bs, const stater_denoise_params *p,
                         const stater_gbuffers *gbufs, void *stream);
 /* Welch-test mean filter.  film: the mean to filter and to test set 0 with
  * (NULL = the set-0 sample mean).  film_out: NULL = the set's filmD plane.
+ * Both may be PLANAR or INTERLEAVED f32 (the reference's CV_32FC3 film and
+ * filmD, estimator.cpp:421-457), read and written in place.
  * Writes discr (per set) and filmD. */
 int stater_denoise_film(stater_bufset *bs, const stater_denoise_params *p,
                         const stater_gbuffers *gbufs, const stater_plane *film,

@@ -113,6 +113,7 @@ SIGNATURES = [
     ("stater_bufset_destroy", C.c_int, [_P]),
     ("stater_convert_plane", C.c_int, [_P, C.POINTER(Plane), C.POINTER(Plane), _P]),
     ("stater_bufset_reset", C.c_int, [_P, _P]),
+    ("stater_bufset_set_stat_type", C.c_int, [_P, C.c_int]),
     ("stater_bufset_view", C.c_int, [_P, C.c_char_p, C.POINTER(Plane)]),
     ("stater_bufset_bytes", C.c_int64, [_P]),
     ("stater_bufset_bytes_for", C.c_int64, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_uint]),
@@ -276,7 +277,9 @@ class BufferSet:
     """Device-resident statistics buffers of one film (the Estimator's buffers)."""
 
     def __init__(self, ctx: Context, width: int, height: int, lambdas=(), max_moment: int = 5,
-                 flags: int = 0):
+                 flags: int = 0, stat_type: int = 0):
+        """stat_type: the reference's StatTypeIndex its buffer names carry
+        (0 radiance; 4 depth, 5 normal, 6 albedo for G-buffer sets)."""
         self.ctx, self.lib = ctx, ctx.lib
         self.width, self.height = width, height
         self.lambdas = list(lambdas)
@@ -286,6 +289,8 @@ class BufferSet:
                                             C.byref(h)))
         self.h = h
         ctx._sets.add(self)
+        if stat_type:
+            check(self.lib.stater_bufset_set_stat_type(self.h, stat_type))
 
     def close(self) -> None:
         if self.h:

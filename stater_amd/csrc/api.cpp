@@ -422,6 +422,17 @@ static void fill_plane(const stater_bufset *bs, void *ptr, int channels, stater_
     out->reserved = 0;
 }
 
+int stater_bufset_set_stat_type(stater_bufset *bs, int type) {
+    if (!bs) return fail(STATER_E_ARG, "stater_bufset_set_stat_type: NULL set");
+    if (type < 0 || type > 7) return fail(STATER_E_ARG, "stater_bufset_set_stat_type: type must be 0..7");
+    bs->stat_type = type;
+    return STATER_OK;
+}
+
+// Single-channel stat types (StatTile<Float>, src/statistics/statpath.cpp:
+// 119-129): the MIS win rates and the material-id and depth G-buffers.
+static bool float_type(int type) { return type >= 1 && type <= 4; }
+
 int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *out) {
     if (!bs || !name || !out) return fail(STATER_E_ARG, "stater_bufset_view: NULL argument");
     const std::string nm(name);
@@ -434,10 +445,11 @@ int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *
     }
     if (nm == "filmD") { if (!derived) return need_derived(); fill_plane(bs, bs->filmD, 3, STATER_F32, out); return STATER_OK; }
     if (nm == "targetspp") { if (!derived) return need_derived(); fill_plane(bs, bs->targetspp, 1, STATER_I32, out); return STATER_OK; }
-    // t<type>-b<slot>-[film-]<stat>; the set holds type 0 (Radiance).
-    int slot = -1, consumed = 0;
-    if (std::sscanf(name, "t0-b%d-%n", &slot, &consumed) != 1 || consumed == 0)
+    // t<type>-b<slot>-[film-]<stat> of the set's stat type
+    int type = -1, slot = -1, consumed = 0;
+    if (std::sscanf(name, "t%d-b%d-%n", &type, &slot, &consumed) != 2 || consumed == 0 || type != bs->stat_type)
         return fail(STATER_E_NOTFOUND, "unknown buffer '" + nm + "'");
+    const int ch = float_type(type) ? 1 : 3;
     const int nslots = std::max(1, bs->n_lambdas);
     if (slot < 0 || slot >= nslots) return fail(STATER_E_NOTFOUND, "no slot in '" + nm + "'");
     std::string rest = nm.substr(consumed);
@@ -446,16 +458,16 @@ int stater_bufset_view(const stater_bufset *bs, const char *name, stater_plane *
     const int s = film ? 0 : (bs->n_lambdas ? bs->slot_set[slot] : 0);
     static const char *stat_names[N_STATS] = {"mean", "m2", "m3", "m4", "c2X", "c2Y", "m1"};
     for (int k = 0; k < N_STATS; ++k)
-        if (rest == stat_names[k]) { fill_plane(bs, bs->stat(s, k, 0), 3, STATER_F32, out); return STATER_OK; }
+        if (rest == stat_names[k]) { fill_plane(bs, bs->stat(s, k, 0), ch, STATER_F32, out); return STATER_OK; }
     if (film && rest == "n") { fill_plane(bs, bs->n, 1, STATER_I32, out); return STATER_OK; }
     if (!derived) return need_derived();
     if (rest == "jb") { fill_plane(bs, bs->jb + (size_t)s * bs->plane, 1, STATER_F32, out); return STATER_OK; }
     struct { const char *n; float *b; } der[] = {{"lnS2", bs->lnS2}, {"se2", bs->se2}, {"m2C", bs->m2C},
                                                  {"varD", bs->varD}, {"varCD", bs->varCD}, {"discr", bs->discr}};
     for (auto &d : der)
-        if (rest == d.n) { fill_plane(bs, bs->per_set(d.b, s, 0), 3, STATER_F32, out); return STATER_OK; }
-    if (film && rest == "meanVar") { fill_plane(bs, bs->meanVar, 3, STATER_F32, out); return STATER_OK; }
-    if (film && rest == "meanD") { fill_plane(bs, bs->meanD, 3, STATER_F32, out); return STATER_OK; }
+        if (rest == d.n) { fill_plane(bs, bs->per_set(d.b, s, 0), ch, STATER_F32, out); return STATER_OK; }
+    if (film && rest == "meanVar") { fill_plane(bs, bs->meanVar, ch, STATER_F32, out); return STATER_OK; }
+    if (film && rest == "meanD") { fill_plane(bs, bs->meanD, ch, STATER_F32, out); return STATER_OK; }
     return fail(STATER_E_NOTFOUND, "unknown buffer '" + nm + "'");
 }
 
@@ -535,8 +547,10 @@ int stater_denoise_vars(stater_bufset *bs, const stater_denoise_params *p, const
 
 static int check_film_plane(const stater_bufset *bs, const stater_plane *f, const char *what) {
     if (!f) return STATER_OK;
+    const bool il = f->layout == STATER_INTERLEAVED;
     if (!f->ptr || f->width != bs->W || f->height != bs->H || f->channels != 3 || f->dtype != STATER_F32 ||
-        f->pitch_bytes % 4 || f->channel_stride_bytes % 4)
+        (f->layout != STATER_PLANAR && !il) || f->pitch_bytes % 4 || (!il && f->channel_stride_bytes % 4) ||
+        f->pitch_bytes < (int64_t)bs->W * (il ? 12 : 4))
         return fail(STATER_E_ARG, std::string("stater_denoise_film: bad ") + what + " plane");
     return STATER_OK;
 }

@@ -708,7 +708,7 @@ __device__ __forceinline__ void welch_fast(const TCritPoly &tp, const float (&Mp
 __device__ __forceinline__ void welch_load(const WelchArgs &a, int xq, int yq, int s, int c, float &M, float &V) {
     const size_t pix = (size_t)yq * a.ld + xq;
     const int n = a.n[pix];
-    M = s == 0 ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 + c * a.film_cs)
+    M = s == 0 ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * a.film_xs + c * a.film_cs)
                : a.mean[(size_t)s * a.set_stride + c * a.plane + pix];
     float var;
     if (a.var_is_m2) var = n >= 2 ? a.var[(size_t)s * a.set_stride + c * a.plane + pix] / (float)(n - 1) : 0.f;
@@ -753,7 +753,7 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 slot[(LY::FILM + c) * HWC + col] =
-                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 +
+                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * a.film_xs +
                                           c * a.film_cs)
                        : 0.f;
             if (PLAIN)
@@ -925,7 +925,7 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
             }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * a.film_xs + c * a.film_cs);
             if (PLAIN) numV[c] = n >= 2 ? a.var[c * a.plane + pix] / (float)(n - 1) : 0.f;
         }
         if (a.dec) {
@@ -956,7 +956,7 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
     if (!active) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = num[c] / den[c];
+        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = num[c] / den[c];
         if (PLAIN) a.varW[c * a.plane + (size_t)y * a.ld + x] = numV[c] / den[c];
     }
 }
@@ -1092,7 +1092,7 @@ __device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int
     float film[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        film[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+        film[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * a.film_xs + c * a.film_cs);
     if (a.rec_head == 1) {
         rec[0] = make_float4(film[0], film[1], film[2], 0.f);
         return;
@@ -1190,7 +1190,7 @@ __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot,
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 slot[(LY::FILM + c) * HWC + col] =
-                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * 4 +
+                    in ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * a.film_xs +
                                           c * a.film_cs)
                        : 0.f;
         } else {
@@ -1285,7 +1285,7 @@ __device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, fl
             for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * 4 + c * a.film_cs);
+            num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * a.film_xs + c * a.film_cs);
         if (a.dec) {
             const int win = 2 * a.r + 1;
             const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
@@ -1305,7 +1305,7 @@ __device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, fl
     if (!active) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = num[c] / den[c];
+        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = num[c] / den[c];
 }
 
 template <bool HASW, int HWC, int TY>
@@ -1583,9 +1583,9 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         if (actA)
-            *(float *)((char *)a.out + (int64_t)yA * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) = A.num[c] / A.den[c];
+            *(float *)((char *)a.out + (int64_t)yA * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = A.num[c] / A.den[c];
         if (actB)
-            *(float *)((char *)a.out + (int64_t)(yA + 1) * a.out_pitch + (int64_t)x * 4 + c * a.out_cs) =
+            *(float *)((char *)a.out + (int64_t)(yA + 1) * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) =
                 B.num[c] / B.den[c];
     }
 }
@@ -1689,9 +1689,11 @@ static void fill_welch_args(stater_bufset *bs, const stater_denoise_params *p, c
     }
     a.film = bs->stat(0, ST_MEAN, 0);
     a.film_pitch = (int64_t)bs->ld * 4;
+    a.film_xs = 4;
     a.film_cs = (int64_t)bs->plane * 4;
     a.out = plain ? bs->meanD : bs->filmD;
     a.out_pitch = (int64_t)bs->ld * 4;
+    a.out_xs = 4;
     a.out_cs = (int64_t)bs->plane * 4;
     a.jb = bs->jb;
     a.discr = bs->discr;
@@ -1716,17 +1718,21 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
                         hipStream_t st, int ty) {
     WelchArgs a{};
     fill_welch_args(bs, p, g, plain, a);
+    // planar planes: channel c at c * channel_stride; interleaved (the
+    // reference's CV_32FC3 film, estimator.cpp:421-457): pixels 12 bytes apart
     if (film) {
-        if (film->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film must be planar");
+        const bool il = film->layout == STATER_INTERLEAVED;
         a.film = (const float *)film->ptr;
         a.film_pitch = film->pitch_bytes;
-        a.film_cs = film->channel_stride_bytes;
+        a.film_xs = il ? 12 : 4;
+        a.film_cs = il ? 4 : film->channel_stride_bytes;
     }
     if (film_out) {
-        if (film_out->layout != STATER_PLANAR) return fail(STATER_E_UNSUPPORTED, "denoise_film: film_out must be planar");
+        const bool il = film_out->layout == STATER_INTERLEAVED;
         a.out = (float *)film_out->ptr;
         a.out_pitch = film_out->pitch_bytes;
-        a.out_cs = film_out->channel_stride_bytes;
+        a.out_xs = il ? 12 : 4;
+        a.out_cs = il ? 4 : film_out->channel_stride_bytes;
     }
     a.dec = dec;
     a.bits = bs->welch_bits;

@@ -37,6 +37,7 @@ struct stater_bufset {
     int slot_set[STATER_MAX_LAMBDAS] = {};
     int mm = 5;
     unsigned flags = 0;
+    int stat_type = 0;   // the reference's StatTypeIndex the buffer names carry (t<type>-...)
     char *base = nullptr;
     size_t bytes = 0;
     // device planes

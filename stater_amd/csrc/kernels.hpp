@@ -58,10 +58,10 @@ struct WelchArgs {
     const float *var;      // variance fed to the test, [S][3][plane] (or m2 for the raw test)
     int var_is_m2;         // 1: var = m2/(n-1) computed on the fly from the statistics
     const float *film;     // filtered values and set-0 test mean (planar, 3 ch)
-    int64_t film_pitch, film_cs;
+    int64_t film_pitch, film_xs, film_cs;  // bytes: row, pixel (4 planar, 12 interleaved), channel
     const float *jb;
     float *out;            // filmD
-    int64_t out_pitch, out_cs;
+    int64_t out_pitch, out_xs, out_cs;
     float *discr;          // [S][3][plane]
     float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
     uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]

@@ -183,8 +183,9 @@ hipStream_t stream_of(void *stream) { return (hipStream_t)stream; }
 
 // The reference's buffer names a set holds (src/statistics/estimator.cpp:21-24,
 // 101-134; estimator.h:459-479): the film and its denoised form, then for
-// every lambda slot j the untransformed buffers t0-bj-film-<name> and, with
-// lambdas, the transformed buffers t0-bj-<name>.
+// every lambda slot j the untransformed buffers t<type>-bj-film-<name> and,
+// with lambdas, the transformed buffers t<type>-bj-<name> (type: the set's
+// stat type, 0 for radiance, 3..6 for the G-buffers).
 std::vector<std::string> set_names(const stater_bufset *bs) {
     static const char *base[] = {"mean", "m2", "m3", "m4", "c2X", "c2Y", "m1"};
     static const char *derived[] = {"jb", "lnS2", "se2", "m2C", "varD", "varCD", "discr"};
@@ -194,7 +195,7 @@ std::vector<std::string> set_names(const stater_bufset *bs) {
     if (der) out.push_back("filmD");
     const int nslots = std::max(1, bs->n_lambdas);
     for (int j = 0; j < nslots; ++j) {
-        const std::string pre = "t0-b" + std::to_string(j) + "-";
+        const std::string pre = "t" + std::to_string(bs->stat_type) + "-b" + std::to_string(j) + "-";
         for (auto n : base) out.push_back(pre + "film-" + n);
         if (der)
             for (auto n : derived) out.push_back(pre + "film-" + n);
@@ -323,7 +324,7 @@ int stater_bufset_read_pfm(stater_bufset *bs, const char *stem, uint64_t spp, vo
             continue;
         const std::string id = fn.substr(prefix.size(), fn.size() - prefix.size() - 4);
         // the film is the caller's plane (the reference reads it into its Film), other
-        // types (G-buffers t4/t5/t6) and unknown names are not statistics of this set
+        // other types (the G-buffers of other sets) and unknown names are not statistics of this set
         if (id == "film" || std::find(names.begin(), names.end(), id) == names.end()) continue;
         files.emplace_back(id, e.path().string());
     }

@@ -86,3 +86,32 @@ def test_frame_in_three_calls_equals_one_call(ctx, mode):
     f3 = hipmem.read_plane(three.view("film"))
     assert np.array_equal(_bits(f1), _bits(f3))
     assert np.array_equal(_bits(f1), _bits(oracle.film_image(oracle.film_add(smp, sanitize=True))))
+
+
+def test_welch_filter_reads_and_writes_interleaved_films(ctx):
+    """denoise_film on the reference's interleaved CV_32FC3 film, writing an
+    interleaved filmD, equals the planar run bit for bit (no conversion pass)."""
+    from stater_amd.workload import FEATURE_SDS
+    W, H = 88, 52
+    _, chunk, feats = generate(ctx, 2, 13, W, H, 32)
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5, flags=sa.STATER_BUFSET_FILM)
+    bs.accumulate(chunk.desc(sanitize=True))
+    bs.film_resolve()
+    p = sa.default_params(radius=7)
+    g = feats.gbuffers(FEATURE_SDS)
+    bs.denoise_vars(p, g)
+    bs.denoise_film(p, g, film=bs.view("film"))
+    ctx.synchronize()
+    want = hipmem.read_plane(bs.view("filmD"))
+    pitch = W * 12 + 64
+    fin, fout = hipmem.DeviceBuffer(pitch * H), hipmem.DeviceBuffer(pitch * H)
+    il_in = sa.plane(fin.ptr, W, H, 3, pitch, 4, layout=sa.STATER_INTERLEAVED)
+    il_out = sa.plane(fout.ptr, W, H, 3, pitch, 4, layout=sa.STATER_INTERLEAVED)
+    import ctypes as C
+    film = bs.view("film")
+    sa.check(ctx.lib.stater_convert_plane(ctx.h, C.byref(film), C.byref(il_in), None))
+    bs.denoise_film(p, g, film=il_in, film_out=il_out)
+    ctx.synchronize()
+    got = fout.download(np.uint8, pitch * H).reshape(H, pitch)[:, :W * 12].copy().view(np.float32)
+    got = got.reshape(H, W, 3).transpose(2, 0, 1)
+    assert np.array_equal(_bits(got), _bits(want))

@@ -43,18 +43,24 @@ def test_gbuffer_means_feed_the_filters(ctx):
     W, H, spp, r = 72, 40, 16, 5
     rng = np.random.default_rng(21)
     planes, ref_means, keep = [], [], []
-    for smp, ch in zip(_gbuffer_samples(rng, spp, H, W), (3, 3, 1)):
+    # albedo, normal and depth sets under the reference's stat types and names
+    # (StatAlbedo 6, StatNormal 5, StatDepth 4: t6/t5/t4-b0-film-mean)
+    for smp, ch, typ in zip(_gbuffer_samples(rng, spp, H, W), (3, 3, 1), (6, 5, 4)):
         chunk = SampleChunk(W, H, spp)
         chunk.from_numpy(smp)
-        gb = sa.BufferSet(ctx, W, H, (), max_moment=1)
+        gb = sa.BufferSet(ctx, W, H, (), max_moment=1, stat_type=typ)
         gb.accumulate(chunk.desc())
         ctx.synchronize()
-        n, st = read_stats(gb, ())
+        n = hipmem.read_plane(gb.view("n"))[0]
+        v = gb.view(f"t{typ}-b0-film-mean")
+        assert v.channels == ch
+        mean = hipmem.read_plane(v)
         n_ref, st_ref = oracle.accumulate(smp, (), 1)
         assert np.array_equal(n, n_ref)
-        assert np.array_equal(st[0, 0].view(np.uint32), st_ref[0, 0].view(np.uint32))  # the mean, bit for bit
-        v = gb.view("t0-b0-film-mean")
-        planes.append(sa.plane(v.ptr, W, H, ch, v.pitch_bytes, v.channel_stride_bytes))
+        assert np.array_equal(mean.view(np.uint32), st_ref[0, 0][:ch].view(np.uint32))  # the mean, bit for bit
+        with pytest.raises(sa.StaterError):
+            gb.view("t0-b0-film-mean")  # the radiance names are not this set's
+        planes.append(v)
         ref_means.append(st_ref[0, 0][:ch])
         keep += [chunk, gb]
     g = sa.gbuffers(planes, list(FEATURE_SDS))

@@ -67,3 +67,28 @@ def test_dump_reload_denoise_matches_in_memory(ctx, tmp_path):
     for k in names:
         assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
     np.testing.assert_array_equal(pfm.read(pfm.file_name(tmp_path / "out", spp, "filmD")), a["filmD"])
+
+
+def test_gbuffer_set_files_carry_their_stat_type(ctx, tmp_path):
+    """A normal G-buffer set (StatNormal, type 5) writes t5-b0-film-mean, the
+    file the reference's loader reads as a feature buffer, and a fresh
+    type-5 set reads it back bit for bit; a radiance set ignores the file."""
+    W, H, spp = 40, 24, 8
+    rng = np.random.default_rng(9)
+    smp = np.zeros((spp, 5, H, W), np.float32)
+    smp[:, :3] = rng.standard_normal((spp, 3, H, W)).astype(np.float32)
+    from stater_amd.workload import SampleChunk
+    chunk = SampleChunk(W, H, spp)
+    chunk.from_numpy(smp)
+    gb = sa.BufferSet(ctx, W, H, (), max_moment=1, flags=sa.STATER_BUFSET_NO_DERIVED, stat_type=5)
+    gb.accumulate(chunk.desc())
+    ctx.synchronize()
+    stem = tmp_path / "scene"
+    assert pfm.write_set(gb, stem, spp, "t5-b0-film-mean") == 1
+    assert pfm.file_name(stem, spp, "t5-b0-film-mean").exists()
+    back = sa.BufferSet(ctx, W, H, (), max_moment=1, flags=sa.STATER_BUFSET_NO_DERIVED, stat_type=5)
+    assert pfm.read_set(back, stem, spp) == 1
+    want = hipmem.read_plane(gb.view("t5-b0-film-mean"))
+    assert np.array_equal(hipmem.read_plane(back.view("t5-b0-film-mean")).view(np.uint32), want.view(np.uint32))
+    radiance = sa.BufferSet(ctx, W, H, (), max_moment=1, flags=sa.STATER_BUFSET_NO_DERIVED)
+    assert pfm.read_set(radiance, stem, spp) == 0
