@@ -218,13 +218,23 @@ def bench_c4(args) -> int:
             t.append(time.perf_counter() - t0)
         return min(t)
 
-    t_plain = timed(plain, max(1, args.steps // 2))
-    log(f"c4 plain: {t_plain * 1e3:.1f} ms")
+    t_plain = timed(plain, max(1, args.steps // 2)) if args.c4_mode != "batched" else None
+    if t_plain is not None:
+        log(f"c4 plain: {t_plain * 1e3:.1f} ms")
+    if args.c4_mode == "plain":  # the one-image-per-launch path alone (for a kernel trace of it)
+        print(json.dumps({"metric": "C4 denoised Mpixel/s, 48 uneven images, one image per launch",
+                          "value": round(P / t_plain / 1e6, 3), "unit": "Mpixel/s", "n_gpus": 1,
+                          "higher_is_better": True, "ms_plain": round(t_plain * 1e3, 2),
+                          "runs": 1 + max(1, args.steps // 2)}), flush=True)
+        ctx.close()
+        return 0
     batch = sa.Batch(ctx, [(bs, sc, feats.albedo, g, spp) for bs, sc, feats, g, spp, _ in imgs], p,
                      chunk_spp=chunk_spp)
     batch.tune()
     t_batch = timed(lambda: batch.run(use_graph=True), max(1, args.steps // 2))
     log(f"c4 batched: {t_batch * 1e3:.1f} ms")
+    if t_plain is None:
+        t_plain = float("nan")
     out = {
         "metric": "C4 denoised Mpixel/s, 48 uneven images (batched: 4 streams, tuned tiles, hipGraph)",
         "value": round(P / t_batch / 1e6, 3), "unit": "Mpixel/s", "n_gpus": 1, "higher_is_better": True,
@@ -362,6 +372,8 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=4, help="c5: jobs in flight")
     ap.add_argument("--chunk-spp", type=int, default=64, help="c4, c5: samples per pixel of one accumulation pass")
     ap.add_argument("--pass-spp", type=int, default=256, help="c3: samples per pixel of one accumulation pass")
+    ap.add_argument("--c4-mode", choices=["both", "plain", "batched"], default="both",
+                    help="c4: time the plain path, the batch, or both")
     args = ap.parse_args()
     if args.copy_probe > 0:
         from stater_amd import hipmem
