@@ -166,14 +166,16 @@ int64_t stater_bufset_bytes_for(int width, int height, int n_lambdas,
 
 /* ---- film (pbrt Film fed the same samples) ------------------------------
  * With STATER_BUFSET_FILM every stater_accumulate call also adds its samples
- * to the Film's pixel sums as the reference's render loop does: the tile sums
- * the (sanitised) RGB of the pixel's samples in order (FilmTile::AddSample,
- * box filter of radius 1/2: weight 1), and the merge adds their XYZ and the
- * weight count (Film::MergeFilmTile), one merge per call.  stater_film_resolve
- * is Film::UpdateImage (src/core/film.cpp:188-220, no splats): RGB =
+ * to the Film's pixel sums as the reference's render loop does: the film
+ * tiles live for the whole render (statpath.cpp:144), so each pixel's RGB
+ * contribSum and weight sum run over all its (sanitised) samples in order,
+ * across calls (FilmTile::AddSample, box filter of radius 1/2: weight 1).
+ * The reference clears the Film and merges each tile once per iteration,
+ * which leaves xyz = 0 + RGBToXYZ(contribSum).  stater_film_resolve is that
+ * and then Film::UpdateImage (src/core/film.cpp:188-220, no splats): RGB =
  * max(0, XYZToRGB(xyz) / weight sum) * scale, into out (NULL: the set's
  * "film" plane, which is then what denoise_film's film argument should be).
- * stater_bufset_reset clears the sums. */
+ * STATER_ACC_FRESH and stater_bufset_reset start the sums from zero. */
 int stater_film_resolve(stater_bufset *bs, float scale, const stater_plane *out, void *stream);
 
 /* ---- on-disk buffer interchange (PFM) ------------------------------------
