@@ -1005,11 +1005,16 @@ __device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot
 }
 
 // Pass 1, one window row dy >= 0 of the lane's pixel: dx from dx0 to r.
-template <int S, bool SMALLN, int HWC>
+// BANDED: bit i of the band's words is its offset band_base + i, and a
+// channel's band_count bits take bits_words words; otherwise the whole
+// forward half-window in kBitsWords words.
+template <int S, bool SMALLN, int HWC, bool BANDED>
 __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot, int tx, int dy, int dx0,
                                            const float (&Mp)[S][3], const float (&Vp)[S][3], float okp, float nm1p,
                                            float rp, uint32_t (&word)[3], size_t pix) {
-    const int nfwd = 2 * a.r * a.r + 2 * a.r;
+    const int base = BANDED ? a.band_base : 0;
+    const int count = BANDED ? a.band_count : 2 * a.r * a.r + 2 * a.r;
+    const int nwords = BANDED ? a.bits_words : kBitsWords;
     for (int dx = dx0; dx <= a.r; ++dx) {
         const float *q = slot + tx + a.r + dx;
         const float2 *pq = (const float2 *)(slot + 2 * HWC) + tx + a.r + dx;
@@ -1069,15 +1074,15 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
                 E[c] = pass ? -1.f : 1.f;
             }
         }
-        const int idx = fwd_index(dx, dy, a.r);
+        const int idx = fwd_index(dx, dy, a.r) - base;
 #pragma unroll
         for (int c = 0; c < 3; ++c) word[c] |= (okpq > 0.f && !(E[c] > 0.f) ? 1u : 0u) << (idx & 31);
-        if ((idx & 31) == 31 || idx == nfwd - 1) {
+        if ((idx & 31) == 31 || idx == count - 1) {
             if (a.rec_f4) {  // straight into the weighting pass's record, as one (c0, c1, c2, 0) group
                 ((uint4 *)(a.rec + pix * a.rec_f4 + a.rec_head))[idx >> 5] = make_uint4(word[0], word[1], word[2], 0u);
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) a.bits[((size_t)c * kBitsWords + (idx >> 5)) * a.plane + pix] = word[c];
+                for (int c = 0; c < 3; ++c) a.bits[((size_t)c * nwords + (idx >> 5)) * a.plane + pix] = word[c];
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) word[c] = 0u;
@@ -1085,15 +1090,15 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
     }
 }
 
-// The head of a pixel's weighting-pass record: film (3) and, with weights,
-// the 7 features: (film0, film1, film2, f0) (f1 .. f4) (f5, f6, 0, 0), or
-// (film0, film1, film2, 0) without.
-__device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int y, float4 *rec) {
+// The head of a pixel's weighting-pass record (head4 items): film (3) and,
+// with weights, the 7 features: (film0, film1, film2, f0) (f1 .. f4) (f5, f6,
+// 0, 0), or (film0, film1, film2, 0) without.
+__device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int y, float4 *rec, int head4) {
     float film[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
         film[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * a.film_xs + c * a.film_cs);
-    if (a.rec_head == 1) {
+    if (head4 == 1) {
         rec[0] = make_float4(film[0], film[1], film[2], 0.f);
         return;
     }
@@ -1105,7 +1110,7 @@ __device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int
     rec[2] = make_float4(f[5], f[6], 0.f, 0.f);
 }
 
-template <int S, int HWC, int TY>
+template <int S, int HWC, int TY, bool BANDED>
 __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, float *ring) {
     constexpr int SLOT = DecideLayout<S>::FLOATS * HWC;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -1130,37 +1135,40 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
-                a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
+                if (!BANDED || a.band_dy0 == 0) a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
             }
-        if (a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4);
+        if (!BANDED && a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4, a.rec_head);
+        if (BANDED && a.head) welch_record_head(a, x, y, a.head + pix * a.head4, a.head4);
     }
     const bool large_n = welch_window_large_n(a, x0, y0, TY);
 
-    for (int t = 0; t < TY; ++t) decide_stage_row<S, HWC>(a, ring + t * SLOT, y0 + t, x0);
+    // window rows dy = d0 .. d1; staged row j is image row y0 + d0 + j
+    const int d0 = BANDED ? a.band_dy0 : 0, d1 = BANDED ? a.band_dy1 : a.r;
+    for (int t = 0; t < TY; ++t) decide_stage_row<S, HWC>(a, ring + t * SLOT, y0 + d0 + t, x0);
     __syncthreads();
     uint32_t word[3] = {0u, 0u, 0u};
-    for (int k = 0; k <= a.r; ++k) {  // dy = k
-        const float *slot = ring + ((ty + k) % TY) * SLOT;
+    for (int k = d0; k <= d1; ++k) {  // dy = k
+        const float *slot = ring + ((ty + k - d0) % TY) * SLOT;
         const int dx0 = k == 0 ? 1 : -a.r;
         if (active) {
-            if (large_n) decide_row<S, false, HWC>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
-            else decide_row<S, true, HWC>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            if (large_n) decide_row<S, false, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            else decide_row<S, true, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
         }
-        if (k == a.r) break;
+        if (k == d1) break;
         __syncthreads();
-        decide_stage_row<S, HWC>(a, ring + (k % TY) * SLOT, y0 + k + TY, x0);
+        decide_stage_row<S, HWC>(a, ring + ((k - d0) % TY) * SLOT, y0 + k + TY, x0);
         __syncthreads();
     }
 }
 
-template <int S, int HWC, int TY>
+template <int S, int HWC, int TY, bool BANDED = false>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
     1, lds_waves_per_simd(TY * DecideLayout<S>::FLOATS * HWC * 4, 64 * TY)))) k_welch_decide(WelchArgs a) {
     extern __shared__ float ring[];
     const int tiles_x = (a.W + TX - 1) / TX;
     const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
     if (t < 0) return;
-    decide_tile<S, HWC, TY>(a, t % tiles_x, t / tiles_x, ring);
+    decide_tile<S, HWC, TY, BANDED>(a, t % tiles_x, t / tiles_x, ring);
 }
 
 // Pass 2 LDS slot: features, film[3], then the 3 x NW decision words of the
@@ -1195,7 +1203,7 @@ __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot,
                        : 0.f;
         } else {
             const int w = kind - 1, c = w / LY::NW, k = w - c * LY::NW;  // c * NW + k
-            words[w * HWC + col] = in ? a.bits[((size_t)c * kBitsWords + k) * a.plane + (size_t)yq * a.ld + xq] : 0u;
+            words[w * HWC + col] = in ? a.bits[((size_t)c * a.bits_words + k) * a.plane + (size_t)yq * a.ld + xq] : 0u;
         }
     }
 }
@@ -1219,9 +1227,9 @@ __device__ __forceinline__ void accum_row(const WelchArgs &a, const float *slot,
         if (a.r > 0)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                own0[c] = a.bits[((size_t)c * kBitsWords + k0) * a.plane + pix];
-                own1[c] = k1 > k0 ? a.bits[((size_t)c * kBitsWords + k0 + 1) * a.plane + pix] : 0u;
-                own2[c] = k1 > k0 + 1 ? a.bits[((size_t)c * kBitsWords + k0 + 2) * a.plane + pix] : 0u;
+                own0[c] = a.bits[((size_t)c * a.bits_words + k0) * a.plane + pix];
+                own1[c] = k1 > k0 ? a.bits[((size_t)c * a.bits_words + k0 + 1) * a.plane + pix] : 0u;
+                own2[c] = k1 > k0 + 1 ? a.bits[((size_t)c * a.bits_words + k0 + 2) * a.plane + pix] : 0u;
             }
     }
     for (int dx = -a.r; dx <= a.r; ++dx) {
@@ -1602,6 +1610,285 @@ static int accum2_launch(const WelchArgs &a, hipStream_t st) {
     return STATER_OK;
 }
 
+// ------------------------------------------------------------------------
+// Banded once-per-pair form, 10 < r <= 20.  The forward half-window is split
+// into two bands of rows, dy = 0 .. 10 and 11 .. r, whose decision words (at
+// most 14 per channel: 21 r + 10 <= 430 and 10 (2r + 1) <= 410 offsets) fit
+// the set's variance-filter record planes.  Each band runs the decision pass
+// over its rows and then a weighting pass over the pixel's pairs at dy in
+// [-d1, -d0] and [d0, d1].  The first band's sums (den, num per channel) carry
+// over to the second in planes of the set's decision-word area, which also
+// holds the weighting records (film, features) the first band's decision
+// pass writes.
+//
+// The weighting pass stages, per row step, only the word groups the step
+// reads: the 2r + 1 offsets of a window row span at most three words.
+// Wavefront ty (image row y0 + ty) at step dy reads the words of image row
+// y0 + ty + min(dy, 0), a backward neighbour's (dy < 0, or dy = 0 and dx < 0)
+// at the neighbour's column, its own (forward) at its own column, as
+// (c0, c1, c2, 0) groups; and the neighbour row's records from a ring of
+// TY + 1 rows.  The next step's words and ring row are loaded into registers
+// before the current step's pairs and written to LDS after them, so one
+// barrier per step separates the two.
+constexpr int kBandRows0 = 10;     // the first band holds dy = 0 .. 10
+constexpr int kBandWords = 14;     // words per channel of a band (r <= 20)
+constexpr int kBandWordSlots = 3;  // word groups of one window row
+static_assert(kBandRows0 == kRecBitsRadius, "radii above the record path's take two bands");
+static_assert(21 * 20 + 10 <= 32 * kBandWords && 10 * 41 <= 32 * kBandWords, "band words for r <= 20");
+static_assert(3 * kBandWords <= kVarRecFloats, "band words must fit the variance-filter record planes");
+static_assert(6 + 4 * 3 <= 3 * kBitsWords, "band sums and weighting records must fit the decision-word planes");
+
+template <bool HASW>
+constexpr int band_head4() { return HASW ? 3 : 1; }
+
+template <bool HASW, int HWC, int TY>
+struct BandStage {
+    static constexpr int F4 = band_head4<HASW>();
+    static constexpr int HITEMS = HWC * F4;                   // float4 items of a ring row
+    static constexpr int WITEMS = TY * kBandWordSlots * HWC;  // word groups of a step
+    static constexpr int HPER = (HITEMS + 64 * TY - 1) / (64 * TY);
+    static constexpr int WPER = (WITEMS + 64 * TY - 1) / (64 * TY);
+    static constexpr int LDS_BYTES = ((TY + 1) * HITEMS + 2 * WITEMS) * 16;
+    float4 h[HPER];
+    uint4 w[WPER];
+};
+
+// First word of the band that a step at window row dy reads.
+__device__ __forceinline__ int band_w0(const WelchArgs &a, int dy) {
+    const int ady = dy < 0 ? -dy : dy;
+    return ((ady == 0 ? 0 : fwd_index(-a.r, ady, a.r)) - a.band_base) >> 5;
+}
+
+template <bool HASW, int HWC, int TY>
+__device__ __forceinline__ void band_load_head(const WelchArgs &a, BandStage<HASW, HWC, TY> &S, int yq, int x0) {
+    using B = BandStage<HASW, HWC, TY>;
+    const bool row_in = yq >= 0 && yq < a.H;
+#pragma unroll
+    for (int i = 0; i < B::HPER; ++i) {
+        const int it = threadIdx.x + i * 64 * TY;
+        const int col = it / B::F4, item = it - col * B::F4;
+        const int xq = x0 - a.r + col;
+        const bool in = row_in && it < B::HITEMS && col < TX + 2 * a.r && xq >= 0 && xq < a.W;
+        S.h[i] = in ? a.head[((size_t)yq * a.ld + xq) * B::F4 + item] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+template <bool HASW, int HWC, int TY>
+__device__ __forceinline__ void band_store_head(const BandStage<HASW, HWC, TY> &S, float4 *slot) {
+    using B = BandStage<HASW, HWC, TY>;
+#pragma unroll
+    for (int i = 0; i < B::HPER; ++i) {
+        const int it = threadIdx.x + i * 64 * TY;
+        if (it < B::HITEMS) slot[it] = S.h[i];
+    }
+}
+
+template <bool HASW, int HWC, int TY>
+__device__ __forceinline__ void band_load_words(const WelchArgs &a, BandStage<HASW, HWC, TY> &S, int dy, int x0,
+                                                int y0) {
+    using B = BandStage<HASW, HWC, TY>;
+    const int w0 = band_w0(a, dy);
+#pragma unroll
+    for (int i = 0; i < B::WPER; ++i) {
+        const int it = threadIdx.x + i * 64 * TY;
+        const int t = it / (kBandWordSlots * HWC), rem = it - t * (kBandWordSlots * HWC);
+        const int j = rem / HWC, col = rem - j * HWC;
+        const int ys = y0 + t + (dy < 0 ? dy : 0), xq = x0 - a.r + col, wi = w0 + j;
+        const bool in = it < B::WITEMS && col < TX + 2 * a.r && ys >= 0 && ys < a.H && xq >= 0 && xq < a.W &&
+                        wi < a.bits_words;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (in) {
+            const uint32_t *b = a.bits + (size_t)wi * a.plane + (size_t)ys * a.ld + xq;
+            const size_t cs = (size_t)a.bits_words * a.plane;
+            v = make_uint4(b[0], b[cs], b[2 * cs], 0u);
+        }
+        S.w[i] = v;
+    }
+}
+
+template <bool HASW, int HWC, int TY>
+__device__ __forceinline__ void band_store_words(const BandStage<HASW, HWC, TY> &S, uint4 *buf) {
+    using B = BandStage<HASW, HWC, TY>;
+#pragma unroll
+    for (int i = 0; i < B::WPER; ++i) {
+        const int it = threadIdx.x + i * 64 * TY;
+        if (it < B::WITEMS) buf[it] = S.w[i];
+    }
+}
+
+// One window row dy of the lane's pixel: hrow holds the neighbour row's
+// records (column dx = -r at tx), wrow the wavefront's word groups of the
+// step (group j of column col at j * HWC + col).
+template <bool HASW, int HWC>
+__device__ __forceinline__ void band_row(const WelchArgs &a, const float4 *hrow, const uint4 *wrow, int tx, int x,
+                                         int y, int dy, const float (&gp)[kNF], float (&den)[3], float (&num)[3]) {
+    constexpr int F4 = band_head4<HASW>();
+    const int r = a.r, win = 2 * r + 1;
+    const int w0 = band_w0(a, dy);
+    const float spy = a.sp_scale * (float)(dy * dy);
+    for (int dx = -r; dx <= r; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const bool fwd = dy > 0 || (dy == 0 && dx > 0);
+        const int idx = (fwd ? fwd_index(dx, dy, r) : fwd_index(-dx, -dy, r)) - a.band_base;
+        const float4 *hq = hrow + (tx + dx + r) * F4;
+        const float4 h0 = hq[0];
+        const float4 h1 = HASW ? hq[1] : h0, h2 = HASW ? hq[2] : h0;
+        const uint4 wd = wrow[((idx >> 5) - w0) * HWC + tx + r + (fwd ? 0 : dx)];
+        __builtin_amdgcn_sched_barrier(0);
+        float w = 1.f;
+        if (HASW) {
+            const float fq[kNF] = {h0.w, h1.x, h1.y, h1.z, h1.w, h2.x, h2.y};
+            float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+#pragma unroll
+            for (int f = 0; f < kNF; ++f) {
+                const float d = gp[f] - fq[f];
+                lw = __builtin_fmaf(-d, d, lw);
+            }
+            w = __builtin_amdgcn_exp2f(lw);
+        }
+        const uint32_t q3[3] = {wd.x, wd.y, wd.z};
+        const float film[3] = {h0.x, h0.y, h0.z};
+        const int sh = idx & 31;
+        bool acc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            acc[c] = (q3[c] >> sh) & 1u;
+            const float wa = acc[c] ? w : 0.f;
+            den[c] += wa;
+            num[c] = __builtin_fmaf(wa, film[c], num[c]);
+        }
+        if (a.dec) {
+            const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + r) * win + (dx + r)) * 3;
+            for (int c = 0; c < 3; ++c) a.dec[kk + c] = acc[c] ? 1 : 0;
+        }
+    }
+}
+
+template <bool HASW, int HWC, int TY>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
+    1, lds_waves_per_simd(BandStage<HASW, HWC, TY>::LDS_BYTES, 64 * TY)))) k_welch_band(WelchArgs a) {
+    using B = BandStage<HASW, HWC, TY>;
+    constexpr int F4 = B::F4, NS = TY + 1;
+    extern __shared__ float4 bsh[];
+    float4 *ring = bsh;                                  // NS rows of HITEMS
+    uint4 *wbuf = (uint4 *)(bsh + NS * B::HITEMS);       // 2 steps of WITEMS
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    if (t < 0) return;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < a.W && y < a.H;
+    const size_t pix = (size_t)y * a.ld + x;
+    const bool first = a.band_dy0 == 0, last = a.band_dy1 == a.r;
+
+    float gp[kNF], den[3], num[3];
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) gp[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) den[c] = 1.f, num[c] = 0.f;
+    if (active) {
+        const float4 *h = a.head + pix * F4;
+        const float4 h0 = h[0];
+        if (HASW) {
+            const float4 h1 = h[1], h2 = h[2];
+            const float g[kNF] = {h0.w, h1.x, h1.y, h1.z, h1.w, h2.x, h2.y};
+#pragma unroll
+            for (int k = 0; k < kNF; ++k) gp[k] = g[k];
+        }
+        const float film[3] = {h0.x, h0.y, h0.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            den[c] = first ? 1.f : a.part[(size_t)c * a.plane + pix];           // the pixel itself, weight 1
+            num[c] = first ? film[c] : a.part[(size_t)(3 + c) * a.plane + pix];
+        }
+        if (a.dec && first) {
+            const int win = 2 * a.r + 1;
+            const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
+            a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
+        }
+    }
+
+    // sweeps of consecutive window rows: [-d1, d1] for the first band,
+    // [-d1, -d0] and [d0, d1] for the second
+    const int nsw = first ? 1 : 2;
+    for (int sw = 0; sw < nsw; ++sw) {
+        const int dya = sw == 0 ? -a.band_dy1 : a.band_dy0;
+        const int dyb = (first || sw == 1) ? a.band_dy1 : -a.band_dy0;
+        const int steps = dyb - dya + 1;
+        B S;
+        // ring row j is image row y0 + dya + j; wavefront ty reads row ty + k at step k
+        for (int j = 0; j < TY; ++j) {
+            band_load_head<HASW, HWC, TY>(a, S, y0 + dya + j, x0);
+            band_store_head<HASW, HWC, TY>(S, ring + (j % NS) * B::HITEMS);
+        }
+        band_load_words<HASW, HWC, TY>(a, S, dya, x0, y0);
+        band_store_words<HASW, HWC, TY>(S, wbuf);
+        __syncthreads();
+        for (int k = 0; k < steps; ++k) {
+            const int dy = dya + k;
+            const bool more = k + 1 < steps;
+            if (more) {
+                band_load_head<HASW, HWC, TY>(a, S, y0 + dya + k + TY, x0);
+                band_load_words<HASW, HWC, TY>(a, S, dy + 1, x0, y0);
+            }
+            if (active)
+                band_row<HASW, HWC>(a, ring + ((ty + k) % NS) * B::HITEMS, wbuf + (k & 1) * B::WITEMS +
+                                    ty * kBandWordSlots * HWC, tx, x, y, dy, gp, den, num);
+            if (more) {
+                // row k + TY replaces row k - 1, read by nobody since the last barrier
+                band_store_head<HASW, HWC, TY>(S, ring + ((k + TY) % NS) * B::HITEMS);
+                band_store_words<HASW, HWC, TY>(S, wbuf + ((k + 1) & 1) * B::WITEMS);
+            }
+            __syncthreads();
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (last) {
+            *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = num[c] / den[c];
+        } else {
+            a.part[(size_t)c * a.plane + pix] = den[c];
+            a.part[(size_t)(3 + c) * a.plane + pix] = num[c];
+        }
+    }
+}
+
+// Both bands: decision pass, weighting pass, decision pass, weighting pass.
+template <int S>
+static int welch_band_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
+    constexpr int HWC = TX + 40, DTY = kDecideTY, TY = kTY;
+    const size_t dlds = (size_t)DTY * DecideLayout<S>::FLOATS * HWC * 4;
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, DTY, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
+    const size_t blds = hasw ? BandStage<true, HWC, TY>::LDS_BYTES : BandStage<false, HWC, TY>::LDS_BYTES;
+    const void *bk = hasw ? (const void *)k_welch_band<true, HWC, TY> : (const void *)k_welch_band<false, HWC, TY>;
+    STATER_HIP(hipFuncSetAttribute(bk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds));
+    const dim3 dgrid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + DTY - 1) / DTY)));
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    WelchArgs b = a;
+    b.rec_f4 = b.rec_head = 0;
+    b.bits = (uint32_t *)a.rec;  // words: the variance-filter record planes
+    b.bits_words = kBandWords;
+    b.part = (float *)a.bits;    // sums, then weighting records: the decision-word planes
+    b.head4 = hasw ? 3 : 1;
+    for (int band = 0; band < 2; ++band) {
+        b.band_dy0 = band == 0 ? 0 : kBandRows0 + 1;
+        b.band_dy1 = band == 0 ? kBandRows0 : a.r;
+        b.band_base = band == 0 ? 0 : a.r + kBandRows0 * (2 * a.r + 1);  // fwd_index(-r, 11, r)
+        b.band_count = (band == 0 ? a.r + kBandRows0 * (2 * a.r + 1) : (a.r - kBandRows0) * (2 * a.r + 1));
+        b.head = band == 0 ? (float4 *)(b.part + 6 * a.plane) : nullptr;
+        hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY, true>), dgrid, dim3(64 * DTY), dlds, st, b);
+        STATER_HIP(hipGetLastError());
+        b.head = (float4 *)(b.part + 6 * a.plane);
+        if (hasw) hipLaunchKernelGGL((k_welch_band<true, HWC, TY>), grid, dim3(64 * TY), blds, st, b);
+        else hipLaunchKernelGGL((k_welch_band<false, HWC, TY>), grid, dim3(64 * TY), blds, st, b);
+        STATER_HIP(hipGetLastError());
+    }
+    return STATER_OK;
+}
+
 // HWC: halo row width of the radius class, TX + 20 (r <= 10, where the
 // weighting pass may take records) or TX + 40 (r <= 20, bit planes only).
 template <int S, int HWC>
@@ -1736,6 +2023,11 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
     }
     a.dec = dec;
     a.bits = bs->welch_bits;
+    a.bits_words = kBitsWords;
+    a.band_dy0 = 0;
+    a.band_dy1 = a.r;
+    a.band_base = 0;
+    a.band_count = 2 * a.r * a.r + 2 * a.r;
     a.rec = (float4 *)bs->var_rec;
     a.zero = (const float4 *)bs->var_zero;
     const int S = plain ? 1 : bs->S;
@@ -1750,6 +2042,16 @@ static int welch_common(stater_bufset *bs, const stater_denoise_params *p, const
             case 4: return welch_sym_launch<4, TX + 2 * kRecBitsRadius>(a, hasw, st);
             case 5: return welch_sym_launch<5, TX + 2 * kRecBitsRadius>(a, hasw, st);
             default: return welch_sym_launch<6, TX + 2 * kRecBitsRadius>(a, hasw, st);
+        }
+    }
+    if (!a.mode3 && a.bits && a.rec && a.r > kRecBitsRadius && a.r <= 20) {  // mode 2, wider windows: in two bands
+        switch (S) {
+            case 1: return welch_band_launch<1>(a, hasw, st);
+            case 2: return welch_band_launch<2>(a, hasw, st);
+            case 3: return welch_band_launch<3>(a, hasw, st);
+            case 4: return welch_band_launch<4>(a, hasw, st);
+            case 5: return welch_band_launch<5>(a, hasw, st);
+            default: return welch_band_launch<6>(a, hasw, st);
         }
     }
     switch (S) {

@@ -75,10 +75,9 @@ namespace stater {
 // forward neighbour offset (half of the (2r+1)^2 - 1 window) of every pixel:
 // 32-bit words per channel for a radius r.  The set's bit planes cover
 // r <= kBitsMaxRadius, as does the weighting pass that reads them from
-// per-pixel records (kRecBitsRadius).  Wider windows test every ordered pair
-// in one pass: their once-per-pair form, measured at r = 20 with bit planes
-// for 27 words and a plane-reading weighting pass, was slower (51.7 against
-// 32.7 ms at 1080p mt5).
+// per-pixel records (kRecBitsRadius).  Wider windows (r <= 20) split the
+// forward rows into two bands whose words fit the variance-filter record
+// planes (denoise.hip, banded form).
 constexpr int bits_words(int r) { return (2 * r * r + 2 * r + 31) / 32; }
 constexpr int kBitsMaxRadius = 10;
 constexpr int kBitsWords = bits_words(kBitsMaxRadius);  // 7

@@ -65,7 +65,13 @@ struct WelchArgs {
     float *discr;          // [S][3][plane]
     float *varW;           // plain method: mean of accepted s^2 (set 0) -> varD
     uint8_t *dec;          // optional per-pair decisions [P][(2r+1)^2][3]
-    uint32_t *bits;        // forward half-window decisions [3][kBitsWords][plane] (symmetric form)
+    uint32_t *bits;        // forward half-window decisions [3][bits_words][plane] (symmetric form)
+    int bits_words;        // words per channel in bits
+    int band_dy0, band_dy1;  // forward window rows of this pass pair (all of 0 .. r for r <= 10)
+    int band_base, band_count;  // forward index of the band's first offset, offsets in the band
+    float *part;           // banded form: [6][plane] den[3], num[3] carried from the first band to the second
+    float4 *head;          // banded form: per-pixel weighting records (film, features), head4 items each
+    int head4;
     float4 *rec;           // pass-2 records (the buffer set's variance-filter record planes)
     const float4 *zero;    // one all-zero record
     int rec_f4, rec_head;  // > 0: pass 1 writes the records (float4 items per record, head items)

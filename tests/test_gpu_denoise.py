@@ -263,8 +263,10 @@ def test_welch_filter_variants(ctx, cascade, use_levc):
     (77, 45, 3, False, 2),    # unweighted (no features, ds_factor 0)
     (130, 37, 7, True, 2),    # three 64-column tiles, the last one 2 wide
     (8, 5, 10, True, 2),      # image smaller than the window and than one tile
-    (33, 20, 12, True, 2),    # r > 10: every ordered pair tested on its own
+    (33, 20, 12, True, 2),    # r > 10: each pair tested once, in two bands of window rows
+    (70, 30, 11, True, 2),    # r = 11: a second band of one row
     (90, 50, 20, True, 2),    # r = 20, the reference's default radius
+    (200, 70, 20, True, 2),   # r = 20 over 4 x 18 tiles, the last column ragged
     (70, 45, 16, False, 2),   # r = 16 unweighted
     (70, 9, 0, True, 2),      # r = 0: the film itself
     (77, 45, 5, True, 3),     # mode 3 (the set with the smallest jb), ragged tiles
