@@ -104,14 +104,20 @@ __device__ __forceinline__ PreOut prepass_one(int n, float m2, float m3, float m
 // direct global->LDS loads issued by a dedicated loader wavefront, so the
 // loads of the next row are in flight while the other wavefronts test pairs.
 //
-// Each filtering lane handles two vertically adjacent pixels, A (row 2 ty of
-// the tile) and B (row 2 ty + 1): every staged neighbour row a wavefront
-// reads serves both, which halves the LDS reads per output.  Wavefront ty
-// reads row 2 ty + k at step k, so 2 TY - 1 consecutive rows are live; the
-// ring has 2 TY slots, the extra one receiving the next row.  In LDS a
-// record keeps its odd 16-byte stride, so the 16-byte reads of 8 (or 16)
-// consecutive lanes fall in distinct banks, and one base address serves all
-// of a pair's reads as immediate offsets.
+// The default filter (k_var_filter1) runs one pixel per lane and eight
+// filtering wavefronts (seven for r > 10) per workgroup, one tile row each:
+// the ring of their live rows plus the rows in flight fills one CU's LDS, and
+// two filtering wavefronts per SIMD hide each other's LDS and barrier waits
+// (one per SIMD issued VALU work on 0.70 of the cycles; LDS bandwidth is not
+// the limit: a CU serves about 210 bytes per clock of such reads, and the
+// filter asks for a third of that).  The 4-row tiles the batch tuner may pick
+// for small films run k_var_filter, whose lanes handle two vertically
+// adjacent pixels, A (row 2 ty of the tile) and B (row 2 ty + 1): every
+// staged neighbour row a wavefront reads serves both.  There wavefront ty
+// reads row 2 ty + k at step k, so 2 TY - 1 consecutive rows are live.  In
+// LDS a record keeps its odd 16-byte stride, so the 16-byte reads of 8 (or
+// 16) consecutive lanes fall in distinct banks, and one base address serves
+// all of a pair's reads as immediate offsets.
 constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
 
 // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
@@ -134,9 +140,10 @@ struct VarRec {
 
 // Ring slot: the halo row's HWC records, rounded up to whole 64-item loads
 // (the loader fills everything past the halo with zeros).  The ring holds the
-// 2 TY - 1 live rows and AHEAD rows in flight: two for r <= 10, one for the
-// wider windows, whose steps (2r + 1 columns) outlast a row's loads many
-// times over and whose wider halo rows would not leave room for 8-row tiles.
+// live rows (TY for one-row wavefronts, 2 TY - 1 for two-row ones) and AHEAD
+// rows in flight: two for r <= 10, one for the wider windows, whose steps
+// (2r + 1 columns) outlast a row's loads many times over and whose wider halo
+// rows leave room for seven one-row wavefronts only.
 constexpr int kVarAhead = 2;
 template <int G, bool HASW, int HWC>
 constexpr int var_loads_per_row() { return (HWC * VarRec<G, HASW>::FLOAT4S + 63) / 64; }
@@ -335,6 +342,40 @@ __device__ __forceinline__ void var_col_test(const VarArgs &a, const VarCol<G, H
         }
 }
 
+// One pixel per lane: its pair with neighbour column dx.
+template <int G, bool HASW>
+__device__ __forceinline__ void var_col_test1(const VarArgs &a, const VarCol<G, HASW> &c, float dx2, float lw,
+                                              VarPix<G> &A) {
+    const float fq[kNF] = {c.s0.y, c.s0.z, c.s0.w, c.s1.x, c.s1.y, c.s1.z, c.s1.w};
+    const float w = var_weight<HASW>(a.sp_scale, A.gp, c.s0.x, fq, lw, 1.f, dx2);
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) var_test(A.Lp[g][k], A.Zp[g][k], c.v[g][k], w, A.den[g][k], A.num[g][k], A.numC[g][k]);
+}
+
+// The 2r + 1 columns of one staged row for one pixel per lane, read ahead as
+// in var_cols.
+template <int G, bool HASW>
+__device__ __forceinline__ void var_cols1(const VarArgs &a, const float4 *slot, int tx, float lw, VarPix<G> &A) {
+    constexpr int F4 = VarRec<G, HASW>::FLOAT4S;
+    constexpr int ITEMS = VarRec<G, HASW>::ITEMS;
+    const float4 *q = slot + tx * F4;
+    VarCol<G, HASW> X, Y;
+    var_col_load<G, HASW>(q, X);
+    for (int dx = -a.r; dx < a.r; dx += 2, q += 2 * F4) {
+        var_col_load<G, HASW>(q + F4, Y);
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(ITEMS));
+        __builtin_amdgcn_sched_barrier(0);
+        var_col_test1<G, HASW>(a, X, (float)(dx * dx), lw, A);
+        var_col_load<G, HASW>(q + 2 * F4, X);
+        __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(ITEMS));
+        __builtin_amdgcn_sched_barrier(0);
+        var_col_test1<G, HASW>(a, Y, (float)((dx + 1) * (dx + 1)), lw, A);
+    }
+    var_col_test1<G, HASW>(a, X, (float)(a.r * a.r), lw, A);
+}
+
 // The 2r + 1 columns of one staged row: the LDS reads of column dx + 1 are
 // issued before the tests of column dx, in two register sets that
 // alternate; pairs of columns from dx = -r, then the last column alone.
@@ -410,6 +451,29 @@ __device__ __forceinline__ void var_own_fix(const VarArgs &a, size_t pix, int g,
 // set-channels, with plain per-pixel values (no selection between the two
 // pixels' state).
 template <int G, bool HASW>
+__device__ __forceinline__ void var_store1(const VarArgs &a, int x, int y, VarPix<G> &A) {
+    const size_t pA = (size_t)y * a.ld + x;
+    const int nA = a.n[pA];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t off = ((size_t)(a.g0 + g) * 3 + c) * a.plane;
+            const size_t m2off = (size_t)(a.g0 + g) * a.set_stride + c * a.plane;
+            if (nA >= 4 && var_own_suspect(A.Lp[g][c], A.Zp[g][c]))
+                var_own_fix<G, HASW>(a, pA, g, c, A.Lp[g][c], A.Zp[g][c], A.den[g][c], A.num[g][c], A.numC[g][c]);
+            float VA = A.num[g][c] / A.den[g][c], CA = A.numC[g][c] / A.den[g][c];
+            if (nA < 4) {
+                const float inv = nA >= 2 ? 1.f / (float)(nA - 1) : 0.f;
+                VA = nA >= 2 ? a.m2[m2off + pA] * inv : 0.f;
+                CA = nA >= 2 ? a.m2C[off + pA] * inv : 0.f;
+            }
+            a.varD[off + pA] = VA;
+            a.varCD[off + pA] = CA;
+        }
+}
+
+template <int G, bool HASW>
 __device__ __forceinline__ void var_store(const VarArgs &a, int x, int yA, bool actA, bool actB, VarPix<G> &A,
                                           VarPix<G> &B) {
     const size_t pA = (size_t)yA * a.ld + x, pB = pA + a.ld;
@@ -478,9 +542,8 @@ __device__ __forceinline__ void var_load_row(const VarArgs &a, float4 *dst, int 
 // (P + AHEAD).  Both roles pass the same 2r + 3 barriers: the first once
 // rows 0 .. P - 1 have landed, then one at the end of every step k, by which
 // row k + P has landed (it replaces nothing live: its slot held row k - 1 or
-// was never used).  Tiles of 4 and 8 rows are built for the r <= 10
-// feature-weighted filter (chosen per film by the batch tuner); everything
-// else runs 4 rows.
+// was never used).  Built for the 4-row tiles of the r <= 10
+// feature-weighted filter, which the batch tuner may choose per film.
 template <int G, bool HASW, int HWC, int TY, int AHEAD>
 __global__ void __launch_bounds__(64 * (TY + 1))
     __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(var_lds_bytes<G, HASW, HWC, TY, AHEAD>(),
@@ -529,6 +592,70 @@ __global__ void __launch_bounds__(64 * (TY + 1))
     var_store<G, HASW>(a, x, yA, actA, actB, A, B);
 }
 
+// One pixel per lane: TY filtering wavefronts of one row each (wavefront ty
+// reads row ty + k at step k, k = 0 .. 2r) and the loader.  Per output the
+// LDS reads double against the two-row lanes, but the ring of TY + AHEAD rows
+// holds eight filtering wavefronts in one CU's LDS, two per SIMD, so one
+// wavefront's waits are filled by the other's arithmetic.
+template <int G, bool HASW, int HWC, int TY, int AHEAD>
+__global__ void __launch_bounds__(64 * (TY + 1))
+    __attribute__((amdgpu_waves_per_eu(1, 2))) k_var_filter1(VarArgs a) {
+    extern __shared__ float4 vring[];
+    constexpr int P = TY, NS = TY + AHEAD;
+    constexpr int SLOT = var_slot_float4s<G, HASW, HWC>();
+    constexpr int AHEAD_LOADS = var_loads_per_row<G, HASW, HWC>() * (AHEAD - 1);
+    const int tiles_x = (a.W + TX - 1) / TX;
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
+    if (t < 0) return;
+    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int last = 2 * a.r;  // steps 0 .. last
+
+    if (wave == TY) {  // loader
+        for (int j = 0; j < P + AHEAD - 1; ++j)
+            var_load_row<G, HASW, HWC>(a, vring + j * SLOT, y0 - a.r + j, x0, lane);
+        __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // rows 0 .. P - 1 have landed
+        __builtin_amdgcn_s_barrier();
+        for (int k = 0; k <= last; ++k) {
+            const int j = k + P + AHEAD - 1;
+            var_load_row<G, HASW, HWC>(a, vring + (j % NS) * SLOT, y0 - a.r + j, x0, lane);
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));  // row k + P has landed
+            __builtin_amdgcn_s_barrier();
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        return;
+    }
+
+    const int x = x0 + lane, y = y0 + wave;
+    const bool act = x < a.W && y < a.H;
+    VarPix<G> A;
+    var_pix_init<G, HASW>(a, x, y, act, A);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    __syncthreads();
+    for (int k = 0; k <= last; ++k) {
+        const float4 *slot = vring + ((wave + k) % NS) * SLOT;
+        const int dy = k - a.r;
+        if (act) var_cols1<G, HASW>(a, slot, lane, a.sp_scale * (float)(dy * dy), A);
+        __syncthreads();
+    }
+    if (act) var_store1<G, HASW>(a, x, y, A);
+}
+
+template <int G, bool HASW, int HWC, int TY, int AHEAD>
+static int var_launch1(const VarArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
+                       const_cast<float4 *>(a.zero));
+    STATER_HIP(hipGetLastError());
+    constexpr size_t lds = (size_t)(TY + AHEAD) * var_slot_float4s<G, HASW, HWC>() * 16;
+    static_assert(lds <= 160 * 1024, "ring larger than a CU's LDS");
+    STATER_HIP(hipFuncSetAttribute((const void *)k_var_filter1<G, HASW, HWC, TY, AHEAD>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    hipLaunchKernelGGL((k_var_filter1<G, HASW, HWC, TY, AHEAD>), grid, dim3(64 * (TY + 1)), lds, st, a);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
 template <int G, bool HASW, int HWC, int TY, int AHEAD = 2>
 static int var_launch4(const VarArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
@@ -548,9 +675,9 @@ static int var_launch4(const VarArgs &a, hipStream_t st) {
 template <int G>
 static int var_launch(const VarArgs &a, bool hasw, int rows, hipStream_t st) {
     if (a.r > 10)
-        return hasw ? var_launch4<G, true, TX + 40, 4, 1>(a, st) : var_launch4<G, false, TX + 40, 4, 1>(a, st);
-    if (!hasw) return var_launch4<G, false, TX + 20, 2>(a, st);
-    return rows == 8 ? var_launch4<G, true, TX + 20, 4>(a, st) : var_launch4<G, true, TX + 20, 2>(a, st);
+        return hasw ? var_launch1<G, true, TX + 40, 7, 1>(a, st) : var_launch1<G, false, TX + 40, 7, 1>(a, st);
+    if (!hasw) return var_launch1<G, false, TX + 20, 8, 2>(a, st);
+    return rows == 8 ? var_launch1<G, true, TX + 20, 8, 2>(a, st) : var_launch4<G, true, TX + 20, 2>(a, st);
 }
 
 VarArgs make_var_args(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g, int g0) {
@@ -1332,10 +1459,12 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
 // (c0, c1, c2, 0) per group, so that one 16-byte read fetches a neighbour's
 // three channel words.  Pass 1 writes the records (the head when it loads the
 // pixel's own operands, each word group when it is complete); pass 2 stages
-// rows of them with a loader wavefront as in the variance filter.  Each lane weights two vertically adjacent pixels A and B.  A
-// forward offset's decision comes from the pixel's own words (in registers),
-// a backward offset's from the neighbour's record, at a word index that is
-// the same for the whole wavefront.
+// rows of them with a loader wavefront as in the variance filter, one pixel
+// per lane and eight filtering wavefronts (k_welch_accum1).  A forward
+// offset's decision comes from the pixel's own words (in registers), a
+// backward offset's from the neighbour's record, at a word index that is the
+// same for the whole wavefront.  The column code is written for two pixels
+// per lane (A, B) in modes of their own; the one-pixel pass runs B off.
 template <bool HASW>
 struct AccRec {
     static constexpr int HEAD4 = HASW ? 3 : 1;                  // float4 items of film (+ features)
@@ -1345,8 +1474,6 @@ struct AccRec {
 };
 template <bool HASW, int HWC>
 constexpr int acc_loads_per_row() { return (HWC * AccRec<HASW>::FLOAT4S + 63) / 64; }
-template <bool HASW, int HWC, int TY>
-constexpr int acc_lds_bytes() { return var_ring_slots<TY>() * acc_loads_per_row<HASW, HWC>() * 64 * 16; }
 
 template <bool HASW>
 struct AccPix {
@@ -1522,44 +1649,24 @@ __device__ __forceinline__ void acc_cols(const WelchArgs &a, const float4 *q, co
     acc_col_test<HASW, MA, MB>(a, X, r, RA, RB, A, B);
 }
 
-// Step k: the staged row at dy = k - r from A and k - r - 1 from B.
-template <bool HASW>
-__device__ __forceinline__ void acc_row(const WelchArgs &a, const float4 *slot, int tx, int k, AccPix<HASW> &A,
-                                        AccPix<HASW> &B, int &baseA, int &baseB) {
-    const int r = a.r;
-    const AccRowP RA = acc_row_p<HASW>(a, A, baseA, k - r, k > 2 * r);
-    const AccRowP RB = acc_row_p<HASW>(a, B, baseB, k - r - 1, k < 1);
-    const float4 *q = slot + tx * AccRec<HASW>::FLOAT4S;  // column of dx = -r
-    if (k == 0)
-        acc_cols<HASW, kAccBwd, kAccOff>(a, q, RA, RB, A, B);
-    else if (k < r)
-        acc_cols<HASW, kAccBwd, kAccBwd>(a, q, RA, RB, A, B);
-    else if (k == r)
-        acc_cols<HASW, kAccGen, kAccBwd>(a, q, RA, RB, A, B);
-    else if (k == r + 1)
-        acc_cols<HASW, kAccFwd, kAccGen>(a, q, RA, RB, A, B);
-    else if (k <= 2 * r)
-        acc_cols<HASW, kAccFwd, kAccFwd>(a, q, RA, RB, A, B);
-    else
-        acc_cols<HASW, kAccOff, kAccFwd>(a, q, RA, RB, A, B);
-}
-
+// One pixel per lane: TY filtering wavefronts of one row each and the loader,
+// as k_var_filter1; the ring of TY + 2 rows of records fits one CU's LDS with
+// eight filtering wavefronts, two per SIMD.  Wavefront ty reads row ty + k at
+// step k (dy = k - r): backward rows, the centre row, forward rows.
 template <bool HASW, int HWC, int TY>
-__global__ void __launch_bounds__(64 * (TY + 1))
-    __attribute__((amdgpu_waves_per_eu(1, lds_waves_per_simd(acc_lds_bytes<HASW, HWC, TY>(), 64 * (TY + 1)))))
-    k_welch_accum2(WelchArgs a) {
+__global__ void __launch_bounds__(64 * (TY + 1)) __attribute__((amdgpu_waves_per_eu(1, 2))) k_welch_accum1(WelchArgs a) {
     extern __shared__ float4 aring[];
     constexpr int F4 = AccRec<HASW>::FLOAT4S, IPR = acc_loads_per_row<HASW, HWC>();
-    constexpr int P = 2 * TY - 1, NS = var_ring_slots<TY>(), SLOT = IPR * 64;
+    constexpr int P = TY, NS = TY + kVarAhead, SLOT = IPR * 64;
     constexpr int AHEAD_LOADS = IPR * (kVarAhead - 1);
     const int tiles_x = (a.W + TX - 1) / TX;
-    const int t = xcd_tile(tiles_x * ((a.H + 2 * TY - 1) / (2 * TY)));
+    const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
     if (t < 0) return;
-    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * 2 * TY;
+    const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int last = 2 * a.r + 1;
+    const int last = 2 * a.r;
 
-    if (wave == TY) {  // loader, as in the variance filter
+    if (wave == TY) {  // loader
         for (int j = 0; j < P + kVarAhead - 1; ++j)
             ring_load_row<F4, IPR>(a.rec, a.zero, a.W, a.H, a.ld, a.r, aring + j * SLOT, y0 - a.r + j, x0, lane);
         __builtin_amdgcn_s_waitcnt(waitcnt_vm(AHEAD_LOADS));
@@ -1575,40 +1682,43 @@ __global__ void __launch_bounds__(64 * (TY + 1))
         return;
     }
 
-    const int x = x0 + lane, yA = y0 + 2 * wave;
-    const bool actA = x < a.W && yA < a.H, actB = x < a.W && yA + 1 < a.H;
-    AccPix<HASW> A, B;
-    acc_pix_init<HASW>(a, x, yA, actA, A);
-    acc_pix_init<HASW>(a, x, yA + 1, actB, B);
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the own records have landed: no counter waits in the loops
-    int baseA = 0, baseB = 0;  // own words shifted out so far
+    const int x = x0 + lane, y = y0 + wave;
+    const bool act = x < a.W && y < a.H;
+    AccPix<HASW> A;
+    acc_pix_init<HASW>(a, x, y, act, A);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    int base = 0;
     __syncthreads();
     for (int k = 0; k <= last; ++k) {
-        const float4 *slot = aring + ((2 * wave + k) % NS) * SLOT;
-        if (actA) acc_row<HASW>(a, slot, lane, k, A, B, baseA, baseB);
+        const float4 *q = aring + ((wave + k) % NS) * SLOT + lane * F4;  // column of dx = -r
+        if (act) {
+            const int dy = k - a.r;
+            const AccRowP R = acc_row_p<HASW>(a, A, base, dy, false);
+            if (dy < 0) acc_cols<HASW, kAccBwd, kAccOff>(a, q, R, R, A, A);
+            else if (dy == 0) acc_cols<HASW, kAccGen, kAccOff>(a, q, R, R, A, A);
+            else acc_cols<HASW, kAccFwd, kAccOff>(a, q, R, R, A, A);
+        }
         __syncthreads();
     }
+    if (!act) return;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (actA)
-            *(float *)((char *)a.out + (int64_t)yA * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = A.num[c] / A.den[c];
-        if (actB)
-            *(float *)((char *)a.out + (int64_t)(yA + 1) * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) =
-                B.num[c] / B.den[c];
-    }
+    for (int c = 0; c < 3; ++c)
+        *(float *)((char *)a.out + (int64_t)y * a.out_pitch + (int64_t)x * a.out_xs + c * a.out_cs) = A.num[c] / A.den[c];
 }
 
 template <bool HASW>
-static int accum2_launch(const WelchArgs &a, hipStream_t st) {
-    constexpr int HWC = TX + 2 * kRecBitsRadius, TY = 4;
-    const size_t lds = (size_t)acc_lds_bytes<HASW, HWC, TY>();
-    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum2<HASW, HWC, TY>,
+static int accum1_launch(const WelchArgs &a, hipStream_t st) {
+    constexpr int HWC = TX + 2 * kRecBitsRadius, TY = 8;
+    constexpr size_t lds = (size_t)(TY + kVarAhead) * acc_loads_per_row<HASW, HWC>() * 64 * 16;
+    static_assert(lds <= 160 * 1024, "ring larger than a CU's LDS");
+    STATER_HIP(hipFuncSetAttribute((const void *)k_welch_accum1<HASW, HWC, TY>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + 2 * TY - 1) / (2 * TY))));
-    hipLaunchKernelGGL((k_welch_accum2<HASW, HWC, TY>), grid, dim3(64 * (TY + 1)), lds, st, a);
+    const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
+    hipLaunchKernelGGL((k_welch_accum1<HASW, HWC, TY>), grid, dim3(64 * (TY + 1)), lds, st, a);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
+
 
 // ------------------------------------------------------------------------
 // Banded once-per-pair form, 10 < r <= 20.  The forward half-window is split
@@ -1905,7 +2015,7 @@ static int welch_sym_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
         b.rec_head = hasw ? AccRec<true>::HEAD4 : AccRec<false>::HEAD4;
         hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY>), dgrid, dim3(64 * DTY), lds, st, b);
         STATER_HIP(hipGetLastError());
-        return hasw ? accum2_launch<true>(b, st) : accum2_launch<false>(b, st);
+        return hasw ? accum1_launch<true>(b, st) : accum1_launch<false>(b, st);
     }
     hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY>), dgrid, dim3(64 * DTY), lds, st, a);
     STATER_HIP(hipGetLastError());

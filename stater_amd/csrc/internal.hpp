@@ -115,10 +115,11 @@ const TCritPoly &tcrit_for(stater_ctx *ctx, double alpha);
 // kernels (accumulate.hip, denoise.hip, synth.hip)
 int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipStream_t st);
 // The variance filter's pack pass also runs the prepass (lnS2, se2, m2C, jb).
-// ty: output rows per tile.  Variance filter: 4 or 8 (two rows per
-// filtering wavefront; 8 only for the r <= 10 feature-weighted filter,
-// otherwise 4).  Welch filter: 2, 4 or 8 (one row per wavefront; 2 and 8 only for the
-// r <= 10 feature-weighted mode-2 filter, otherwise 4).
+// ty: output rows per tile.  Variance filter: 8 (one row per filtering
+// wavefront) or, for the r <= 10 feature-weighted filter, 4 (two rows per
+// wavefront); r > 10 always runs 7.  Welch filter, ordered form (mode 3 and
+// the plain method): 2, 4 or 8 (one row per wavefront; 2 and 8 only for the
+// feature-weighted mode-2 filter at r <= 10, otherwise 4).
 constexpr int kVarTileRows = 8;
 int launch_var_filter(stater_bufset *bs, const stater_denoise_params *p, const stater_gbuffers *g,
                       hipStream_t st, int ty = kVarTileRows);
