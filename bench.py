@@ -33,7 +33,18 @@ PEAK_HBM = 8000.0      # GB/s
 WELCH_OPS_PER_TEST = 16
 VAR_OPS_PER_TEST = 8
 WEIGHT_OPS_PER_PAIR = 20
-ACC_OPS_PER_SET_CHANNEL = 22
+# Accumulation, per sample-pixel: the shared factors of the recurrence (count,
+# 1/n, sqrt(1 - 1/n), ...), set 0's reference FP32 update per channel (IEEE
+# division and square root included), the factored update per Box-Cox
+# set-channel, and the transforms: per channel ln x - L when some set takes
+# the log-based expm1 (per set-channel), and for lambda 1/4, 1/2, 3/4, 3 the
+# ratio forms u = x/x1 - 1 (3), rho^(1/2) - 1 (10), rho^(1/4) - 1 (10),
+# rho^(3/4) - 1 (2), rho^3 - 1 (4), one multiply per such set.
+ACC_SHARED_OPS = 16
+ACC_SET0_OPS_PER_CHANNEL = 40
+ACC_BC_OPS_PER_SET_CHANNEL = 15
+ACC_EXPM1_OPS = 12
+ACC_LOG_OPS = 7
 
 
 def log(msg: str) -> None:
@@ -47,12 +58,27 @@ def window_pairs(W: int, H: int, r: int) -> int:
     return sx * sy - W * H
 
 
-def kernel_work(W, H, spp, S, r):
+def acc_ops_per_sample(set_lambdas):
+    """Lane-ops of the accumulation per sample-pixel for the transformed sets' lambdas."""
+    kinds = [{0.25: "q", 0.5: "h", 0.75: "tq", 3.0: "c"}.get(float(l), "g") for l in set_lambdas]
+    per_channel = ACC_SET0_OPS_PER_CHANNEL + len(kinds) * ACC_BC_OPS_PER_SET_CHANNEL
+    if "g" in kinds:
+        per_channel += ACC_LOG_OPS + kinds.count("g") * ACC_EXPM1_OPS
+    power = [k for k in kinds if k != "g"]
+    if power:
+        per_channel += 3 + len(power)
+        per_channel += 10 if {"q", "h", "tq"} & set(power) else 0
+        per_channel += 10 if {"q", "tq"} & set(power) else 0
+        per_channel += 2 * power.count("tq") + 4 * power.count("c")
+    return ACC_SHARED_OPS + 3 * per_channel
+
+
+def kernel_work(W, H, spp, S, r, set_lambdas):
     P = W * H
     pairs = window_pairs(W, H, r)
     tests = pairs * S * 3
     T = S - 1
-    acc_ops = P * spp * (S * 3 * ACC_OPS_PER_SET_CHANNEL + 4 * (3 + 3 * T + 3 * S))
+    acc_ops = P * spp * acc_ops_per_sample(set_lambdas)
     return {
         # mode 2 tests each unordered pair once and weights every ordered pair
         "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
@@ -446,7 +472,7 @@ def main() -> int:
     value = P * args.steps / (t1 - t0) / 1e6
 
     tim = ctx.timing()
-    work = kernel_work(W, H, spp, S, r)
+    work = kernel_work(W, H, spp, S, r, [l for l in dict.fromkeys(float(x) for x in lambdas) if l != 1.0])
     kernels = {}
     for name, (cnt, ms) in tim.items():
         avg = ms / max(cnt, 1)

@@ -30,10 +30,12 @@
 //    (pow(x,l) - 1)/l, estimator.h:56,72-74, cancels for small l, and for
 //    l = 3 on dark pixels FP32 y = -1/3 + tiny loses the variation entirely).
 //    Central moments, c2X, c2Y and m1 do not depend on the shift; the stored
-//    mean is K + mean(y - K); mean(y - K) itself persists in an internal
-//    plane so that a later call continues the recurrence bit for bit.  One
-//    log per sample-channel is shared by all sets; expm1 is evaluated in base
-//    2 (wexpm1 below) and the log is the hardware log2.  The update is the
+//    mean is K + mean(y - K); mean(y - K) and x1 persist in internal planes
+//    so that a later call continues the recurrence bit for bit.  One log per
+//    sample-channel is shared by all sets; expm1 is evaluated in base 2
+//    (wexpm1 below) and the log is the hardware log2.  Sets with lambda 1/4,
+//    1/2, 3/4 or 3 take the same value from the ratio x / x1 without the log
+//    (bc_pow below).  The update is the
 //    reference recurrence in factored form,
 //        m2 += d^2 (1 - 1/n)
 //        m3 += -3 (d/n) m2 + d^3 (1 - 1/n^2)
@@ -116,6 +118,63 @@ __device__ __forceinline__ float wexpm1(float w, float Wt) {
 }
 
 constexpr float kLog2eAcc = 1.4426950408889634f;
+
+// Box-Cox sets whose lambda is 1/4, 1/2, 3/4 or 3 (every transformed set of
+// the reference's scene configurations but lambda = 0.01) take the shifted
+// value from rho = x / x1 without a logarithm,
+//     y - K = Wt (rho^l - 1),   Wt = x1^l / l,
+// evaluated from u = rho - 1 = (x - x1) / x1, which has the relative
+// precision of a quotient (x - x1 is exact near x1), so that every power
+// minus one keeps it too:
+//     rho^(1/2) - 1 = u / (rho^(1/2) + 1)                        = h2,
+//     rho^(1/4) - 1 = h2 / (rho^(1/4) + 1)                       = h4,
+//     rho^(3/4) - 1 = h2 + h4 + h2 h4,
+//     rho^3 - 1     = u (u (u + 3) + 3)   (Wt u taken first, so the
+//                                          product overflows only where x^3 does),
+// with the hardware square root and reciprocal (the square roots only in
+// the denominators).  About twenty instructions per sample-channel for the
+// four sets against twelve per set for the log-based expm1.  A sample <= 0 or
+// NaN counts as 0 (u = -1, y - K = -Wt), as in the log-based form.  KC packs
+// the kind of set s (1 .. T) in bits 3 (s - 1) .. 3 (s - 1) + 2; 0 is the
+// log-based form.
+enum : int { kBcGeneric = 0, kBcQuarter = 1, kBcHalf = 2, kBcThreeQuarter = 3, kBcCube = 4 };
+__host__ __device__ constexpr int bc_kind(int KC, int s) { return (KC >> (3 * (s - 1))) & 7; }
+__host__ __device__ constexpr bool bc_has(int KC, int T, int k0, int k1) {
+    for (int s = 1; s <= T; ++s)
+        if (bc_kind(KC, s) == k0 || bc_kind(KC, s) == k1) return true;
+    return false;
+}
+
+struct BcPow {
+    float u, h2, h4;  // rho - 1, rho^(1/2) - 1, rho^(1/4) - 1
+};
+
+template <int T, int KC>
+__device__ __forceinline__ BcPow bc_pow(float v, float x1, float ix1) {
+    BcPow p{0.f, 0.f, 0.f};
+    if (KC == 0) return p;
+    const float xv = v > 0.f ? v : 0.f;
+    p.u = (xv - x1) * ix1;
+    if (bc_has(KC, T, kBcHalf, kBcThreeQuarter) || bc_has(KC, T, kBcQuarter, kBcQuarter)) {
+        const float s2 = __builtin_amdgcn_sqrtf(p.u + 1.f);
+        p.h2 = p.u * __builtin_amdgcn_rcpf(s2 + 1.f);
+        if (bc_has(KC, T, kBcQuarter, kBcThreeQuarter)) {
+            const float s4 = __builtin_amdgcn_sqrtf(s2);
+            p.h4 = p.h2 * __builtin_amdgcn_rcpf(s4 + 1.f);
+        }
+    }
+    return p;
+}
+
+// The shifted transformed value t = y - K of set s.
+template <int KC>
+__device__ __forceinline__ float bc_shifted(int s, float lam, float dl, const BcPow &p, float Wt) {
+    const int kd = bc_kind(KC, s);
+    if (kd == kBcGeneric) return wexpm1(lam * kLog2eAcc * dl, Wt);
+    if (kd == kBcCube) return (Wt * p.u) * __builtin_fmaf(p.u, p.u + 3.f, 3.f);
+    const float h = kd == kBcQuarter ? p.h4 : kd == kBcHalf ? p.h2 : __builtin_fmaf(p.h2, p.h4, p.h2 + p.h4);
+    return Wt * h;
+}
 
 // Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
 struct NFac {
@@ -220,8 +279,9 @@ __device__ __forceinline__ void clear_slots(const AccArgs &a, int S, int c, int 
 // per-sample factors of the factored recurrence and the (u, v) loads are
 // done once per pixel-sample instead of once per channel, and each sample
 // costs five loads per pixel instead of fifteen.
-template <int T, int MM>
+template <int T, int MM, int KC>
 __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
+    constexpr bool GEN = KC == 0 || bc_has(KC, T, kBcGeneric, kBcGeneric);  // some set takes the log
     constexpr int NS = nstats(MM);
     if (x >= a.W) return;
     uint32_t pix = (uint32_t)y * (uint32_t)a.ld + (uint32_t)x;
@@ -252,7 +312,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             for (int k = 0; k < NS; ++k)
                 st[c][s][k] = a.fresh ? 0.f : a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
 
-    float L[3] = {0.f, 0.f, 0.f}, Wt[3][T + 1];
+    float L[3] = {0.f, 0.f, 0.f}, Wt[3][T + 1], x1[3] = {1.f, 1.f, 1.f}, ix1[3] = {1.f, 1.f, 1.f};
     if (T > 0) {
         float v0[3] = {0.f, 0.f, 0.f};
         if (n == 0) {
@@ -264,18 +324,20 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const size_t lp = (size_t)c * a.plane + pix;
+            // the reference point: the pixel-channel's first sample (1 when not positive and finite)
             if (n == 0) {
-                L[c] = (v0[c] > 0.f && __builtin_isfinite(v0[c])) ? __builtin_amdgcn_logf(v0[c]) * 0.69314718055994531f
-                                                                  : 0.f;
-                a.lref[lp] = L[c];
+                x1[c] = (v0[c] > 0.f && __builtin_isfinite(v0[c])) ? v0[c] : 1.f;
+                a.lref[lp] = x1[c];
             } else {
-                L[c] = a.lref[lp];
+                x1[c] = a.lref[lp];
             }
+            L[c] = __builtin_amdgcn_logf(x1[c]) * 0.69314718055994531f;
 #pragma unroll
             for (int s = 1; s <= T; ++s) {
                 Wt[c][s] = expf(a.lam[s] * L[c]) * a.inv_lam[s];
                 st[c][s][0] = n ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
             }
+            if (KC != 0) ix1[c] = 1.f / x1[c];
         }
     }
 
@@ -310,10 +372,11 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 // ln x - L; log of 0 (-inf) and of NaN become -1e30, where expm1 gives -1: y(0) = -1/l
-                const float dl = fmaxf(__builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c], -1e30f);
+                const float dl = GEN ? fmaxf(__builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c], -1e30f) : 0.f;
+                const BcPow pw = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
-                    meng_factored<MM>(st[c][s], wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[c][s]), f, xa, ya);
+                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[c][s]), f, xa, ya);
             }
         }
     }
@@ -357,8 +420,9 @@ __device__ __forceinline__ void merge64(double na, double *a, double nb, const d
 // moments in FP64 and rounded once; c2X, c2Y and m1, which depend on the
 // running mean of the whole sequence, continue sequentially on a running
 // mean carried from the resident state.
-template <int T, int MM>
+template <int T, int MM, int KC>
 __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
+    constexpr bool GEN = KC == 0 || bc_has(KC, T, kBcGeneric, kBcGeneric);
     constexpr int NS = nstats(MM);
     constexpr int NP = MM >= 4 ? 4 : MM;  // partial moments kept
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -388,21 +452,23 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
 #pragma unroll
         for (int k = 0; k < NS; ++k)
             st[s][k] = a.fresh ? 0.f : a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
-    float L = 0.f, Wt[T + 1];
+    float L = 0.f, Wt[T + 1], x1 = 1.f, ix1 = 1.f;
     if (T > 0) {
         const size_t lp = (size_t)c * a.plane + pix;
         if (n0 == 0) {
             const float v0 = load_sample(base, 0, a.comp, c, a.sanitize);
-            L = (v0 > 0.f && __builtin_isfinite(v0)) ? __builtin_amdgcn_logf(v0) * 0.69314718055994531f : 0.f;
-            a.lref[lp] = L;
+            x1 = (v0 > 0.f && __builtin_isfinite(v0)) ? v0 : 1.f;
+            a.lref[lp] = x1;
         } else {
-            L = a.lref[lp];
+            x1 = a.lref[lp];
         }
+        L = __builtin_amdgcn_logf(x1) * 0.69314718055994531f;
 #pragma unroll
         for (int s = 1; s <= T; ++s) {
             Wt[s] = expf(a.lam[s] * L) * a.inv_lam[s];
             st[s][0] = n0 ? a.mshift[((size_t)(s - 1) * 3 + c) * a.plane + pix] : 0.f;
         }
+        if (KC != 0) ix1 = 1.f / x1;
     }
     float pm[T + 1][NP];
     float mg[T + 1];  // running mean of the whole sequence (c2X, c2Y, m1)
@@ -426,10 +492,11 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         const NFac fr = nfac((float)(j + 1));
         NFac fg;
         if (MM >= 5) fg = nfac((float)(n0 + j + 1));
-        const float dl = T > 0 ? fmaxf(__builtin_amdgcn_logf(v) * 0.69314718055994531f - L, -1e30f) : 0.f;
+        const float dl = T > 0 && GEN ? fmaxf(__builtin_amdgcn_logf(v) * 0.69314718055994531f - L, -1e30f) : 0.f;
+        const BcPow pw = bc_pow<T, KC>(v, x1, ix1);
 #pragma unroll
         for (int s = 0; s <= T; ++s) {
-            const float t = s == 0 ? v : wexpm1(a.lam[s] * kLog2eAcc * dl, Wt[s]);
+            const float t = s == 0 ? v : bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[s]);
             meng_factored<NP>(pm[s], t, fr, 0.f, 0.f);
             if (MM >= 5) {
                 const float dg = t - mg[s];
@@ -470,13 +537,13 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
 // the store addresses rebuilt after the sample loop needs about 100.
 #define STATER_ACC_OCC __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3)))
 
-template <int T, int MM>
+template <int T, int MM, int KC>
 __global__ void __launch_bounds__(64) k_accumulate_px(AccArgs a) {
-    accp_tile<T, MM>(a, blockIdx.x * 64 + threadIdx.x, blockIdx.y);
+    accp_tile<T, MM, KC>(a, blockIdx.x * 64 + threadIdx.x, blockIdx.y);
 }
 
-template <int T, int MM>
-__global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM>(a, blockIdx.x, blockIdx.y); }
+template <int T, int MM, int KC>
+__global__ void STATER_ACC_OCC k_accumulate_merge(AccArgs a) { accm_tile<T, MM, KC>(a, blockIdx.x, blockIdx.y); }
 
 __global__ void k_zero(uint32_t *p, size_t words) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
@@ -535,21 +602,50 @@ int launch_zero(void *p, size_t bytes, hipStream_t st) {
     return STATER_OK;
 }
 
-template <int T, int MM>
+template <int T, int MM, int KC = 0>
 static void launch_tm(const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
-    if (merge) hipLaunchKernelGGL((k_accumulate_merge<T, MM>), grid, dim3(192), 0, st, a);
-    else hipLaunchKernelGGL((k_accumulate_px<T, MM>), grid, dim3(64), 0, st, a);
+    if (merge) hipLaunchKernelGGL((k_accumulate_merge<T, MM, KC>), grid, dim3(192), 0, st, a);
+    else hipLaunchKernelGGL((k_accumulate_px<T, MM, KC>), grid, dim3(64), 0, st, a);
 }
 
+// Kind codes of the reference's multi-transform scene configurations
+// (scenes/render-denoise-mt{1..5}.pbrt: lambdas [.5], [.5 .75], [.25 1],
+// [.25 .5 .75 1], [.01 .25 .5 .75 1 3]; a set per lambda other than 1, in
+// list order), built with max_moment 5; every other list runs the generic form.
+constexpr int kKcMt1 = kBcHalf;
+constexpr int kKcMt2 = kBcHalf | kBcThreeQuarter << 3;
+constexpr int kKcMt3 = kBcQuarter;
+constexpr int kKcMt4 = kBcQuarter | kBcHalf << 3 | kBcThreeQuarter << 6;
+constexpr int kKcMt5 = kBcGeneric | kBcQuarter << 3 | kBcHalf << 6 | kBcThreeQuarter << 9 | kBcCube << 12;
+
 template <int T>
-static void launch_t(int mm, const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
+static void launch_t(int mm, int kc, const AccArgs &a, bool merge, dim3 grid, hipStream_t st) {
     switch (mm) {
         case 1: launch_tm<T, 1>(a, merge, grid, st); break;
         case 2: launch_tm<T, 2>(a, merge, grid, st); break;
         case 3: launch_tm<T, 3>(a, merge, grid, st); break;
         case 4: launch_tm<T, 4>(a, merge, grid, st); break;
-        default: launch_tm<T, 5>(a, merge, grid, st); break;
+        default:
+            if (T == 1 && kc == kKcMt1) launch_tm<T, 5, (T == 1 ? kKcMt1 : 0)>(a, merge, grid, st);
+            else if (T == 1 && kc == kKcMt3) launch_tm<T, 5, (T == 1 ? kKcMt3 : 0)>(a, merge, grid, st);
+            else if (T == 2 && kc == kKcMt2) launch_tm<T, 5, (T == 2 ? kKcMt2 : 0)>(a, merge, grid, st);
+            else if (T == 3 && kc == kKcMt4) launch_tm<T, 5, (T == 3 ? kKcMt4 : 0)>(a, merge, grid, st);
+            else if (T == 5 && kc == kKcMt5) launch_tm<T, 5, (T == 5 ? kKcMt5 : 0)>(a, merge, grid, st);
+            else launch_tm<T, 5>(a, merge, grid, st);
+            break;
     }
+}
+
+// The kind code of a set's lambdas (sets 1 ..).
+static int bc_code(const stater_bufset *bs) {
+    int kc = 0;
+    for (int s = 1; s < bs->S; ++s) {
+        const float l = bs->set_lambda[s];
+        const int k = l == 0.25f ? kBcQuarter : l == 0.5f ? kBcHalf : l == 0.75f ? kBcThreeQuarter
+                                                                               : l == 3.f ? kBcCube : kBcGeneric;
+        kc |= k << (3 * (s - 1));
+    }
+    return kc;
 }
 
 AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s) {
@@ -588,13 +684,14 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
     dim3 grid((bs->W + 63) / 64, bs->H);
     const bool merge = (mode & ~STATER_ACC_FRESH) == STATER_ACC_MERGE64;
     LaunchTimer t(bs->ctx, st, merge ? "accumulate_merge64" : "accumulate");
+    const int kc = bc_code(bs);
     switch (bs->S - 1) {
-        case 0: launch_t<0>(bs->mm, a, merge, grid, st); break;
-        case 1: launch_t<1>(bs->mm, a, merge, grid, st); break;
-        case 2: launch_t<2>(bs->mm, a, merge, grid, st); break;
-        case 3: launch_t<3>(bs->mm, a, merge, grid, st); break;
-        case 4: launch_t<4>(bs->mm, a, merge, grid, st); break;
-        default: launch_t<5>(bs->mm, a, merge, grid, st); break;
+        case 0: launch_t<0>(bs->mm, kc, a, merge, grid, st); break;
+        case 1: launch_t<1>(bs->mm, kc, a, merge, grid, st); break;
+        case 2: launch_t<2>(bs->mm, kc, a, merge, grid, st); break;
+        case 3: launch_t<3>(bs->mm, kc, a, merge, grid, st); break;
+        case 4: launch_t<4>(bs->mm, kc, a, merge, grid, st); break;
+        default: launch_t<5>(bs->mm, kc, a, merge, grid, st); break;
     }
     STATER_HIP(hipGetLastError());
     return STATER_OK;

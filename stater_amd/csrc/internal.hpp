@@ -43,7 +43,7 @@ struct stater_bufset {
     // device planes
     int32_t *n = nullptr;
     float *stats = nullptr;  // [S][7][3][plane]
-    float *lref = nullptr;   // [3][plane] Box-Cox reference point ln x1 (sets 1..)
+    float *lref = nullptr;   // [3][plane] Box-Cox reference sample x1 of each pixel-channel (sets 1..)
     float *mshift = nullptr; // [S-1][3][plane] Box-Cox means relative to the reference point
     float *film_rgb = nullptr;  // STATER_BUFSET_FILM: [3][plane] resolved film
     float *film_sum = nullptr;  // [4][plane] the Film's per-pixel RGB contribSum and weight sum,

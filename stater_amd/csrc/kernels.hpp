@@ -18,7 +18,7 @@ struct AccArgs {
     int count_offset;
     int32_t *n;
     float *stats;
-    float *lref;    // [3][plane] ln of the reference sample of each pixel-channel
+    float *lref;    // [3][plane] the reference sample x1 of each pixel-channel (1 if not positive and finite)
     float *mshift;  // [S-1][3][plane] Box-Cox means relative to the reference point
     int ld;
     size_t plane;
