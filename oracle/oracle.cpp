@@ -248,9 +248,12 @@ inline float boxcox_ref(float x, float l) {
     if (l == 1.f) return x;
     return (std::pow(x, l) - 1.f) / l;
 }
-// The same transform evaluated in FP64 (the mathematical value).
+// The same transform evaluated in FP64 (the mathematical value): expm1 of
+// l ln x for x >= 0, and pow for a negative x, which is real for an integer
+// lambda only (NaN otherwise), as std::pow in the reference.
 inline double boxcox64(double x, double l) {
     if (l == 1.0) return x;
+    if (x < 0.0) return (std::pow(x, l) - 1.0) / l;
     return std::expm1(l * std::log(x)) / l;
 }
 

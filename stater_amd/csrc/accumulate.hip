@@ -93,11 +93,13 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float nf, float x
 //     Wt (2^w - 1) = (Wt 2^k)(2^f - 1) + (Wt 2^k - Wt),
 // so the scale Wt rides on the exponent and costs no multiply.  The relative
 // error of the result is at most 6.9e-7 (12 ulp) at every w, largest for
-// |w| < 1/2, far below the sampling noise of the moments it feeds; w below
-// -150 gives -Wt (Wt 2^k flushes to zero), the limit x -> 0.  The caller keeps w finite or +inf: a sample of 0
-// (or NaN) arrives as ln-ratio -1e30 instead of -inf, which drives k far
-// below -150; the conversion saturates (v_cvt_i32_f32 clamps) and the scale
-// flushes to zero, so the result is -Wt there too.
+// |w| < 1/2, far below the sampling noise of the moments it feeds.  w is
+// clamped at -160, where Wt 2^k is zero (or negligible against Wt): the limit
+// x -> 0, which a sample of 0 reaches as ln-ratio -1e30.  The clamp also
+// keeps f = w - k exact: without it the compiler may contract the multiply
+// that made w with this subtraction, fma(dl, l log2 e, -k), which for a
+// huge |w| is the product's rounding error, not 0 (a zero sample then gave
+// NaN).  NaN w gives -Wt here; the callers turn a NaN sample into NaN.
 __device__ __forceinline__ int cvt_i32_sat(float k) {
     int i;
     asm("v_cvt_i32_f32 %0, %1" : "=v"(i) : "v"(k));
@@ -105,6 +107,7 @@ __device__ __forceinline__ int cvt_i32_sat(float k) {
 }
 
 __device__ __forceinline__ float wexpm1(float w, float Wt) {
+    w = fmaxf(w, -160.f);
     const float k = __builtin_rintf(w);
     const float f = w - k;
     float p = 1.3333528768271208e-03f;
@@ -127,16 +130,18 @@ constexpr float kLog2eAcc = 1.4426950408889634f;
 // precision of a quotient (x - x1 is exact near x1), so that every power
 // minus one keeps it too:
 //     rho^(1/2) - 1 = u / (rho^(1/2) + 1)                        = h2,
+//                     (the root of rho = x / x1 itself, precise for small rho too)
 //     rho^(1/4) - 1 = h2 / (rho^(1/4) + 1)                       = h4,
 //     rho^(3/4) - 1 = h2 + h4 + h2 h4,
 //     rho^3 - 1     = u (u (u + 3) + 3)   (Wt u taken first, so the
 //                                          product overflows only where x^3 does),
 // with the hardware square root and reciprocal (the square roots only in
 // the denominators).  About twenty instructions per sample-channel for the
-// four sets against twelve per set for the log-based expm1.  A sample <= 0 or
-// NaN counts as 0 (u = -1, y - K = -Wt), as in the log-based form.  KC packs
-// the kind of set s (1 .. T) in bits 3 (s - 1) .. 3 (s - 1) + 2; 0 is the
-// log-based form.
+// four sets against twelve per set for the log-based expm1.  As pow(x, l)
+// in the reference's boxCox (estimator.h:56): x = 0 gives -1/l (u = -1,
+// y - K = -Wt), a negative x gives NaN for the roots and the exact cube for
+// l = 3, and NaN stays NaN.  KC packs the kind of set s (1 .. T) in bits
+// 3 (s - 1) .. 3 (s - 1) + 2; 0 is the log-based form.
 enum : int { kBcGeneric = 0, kBcQuarter = 1, kBcHalf = 2, kBcThreeQuarter = 3, kBcCube = 4 };
 __host__ __device__ constexpr int bc_kind(int KC, int s) { return (KC >> (3 * (s - 1))) & 7; }
 __host__ __device__ constexpr bool bc_has(int KC, int T, int k0, int k1) {
@@ -153,10 +158,9 @@ template <int T, int KC>
 __device__ __forceinline__ BcPow bc_pow(float v, float x1, float ix1) {
     BcPow p{0.f, 0.f, 0.f};
     if (KC == 0) return p;
-    const float xv = v > 0.f ? v : 0.f;
-    p.u = (xv - x1) * ix1;
+    p.u = (v - x1) * ix1;
     if (bc_has(KC, T, kBcHalf, kBcThreeQuarter) || bc_has(KC, T, kBcQuarter, kBcQuarter)) {
-        const float s2 = __builtin_amdgcn_sqrtf(p.u + 1.f);
+        const float s2 = __builtin_amdgcn_sqrtf(v * ix1);  // rho itself: u + 1 loses a small rho's precision
         p.h2 = p.u * __builtin_amdgcn_rcpf(s2 + 1.f);
         if (bc_has(KC, T, kBcQuarter, kBcThreeQuarter)) {
             const float s4 = __builtin_amdgcn_sqrtf(s2);
@@ -166,14 +170,29 @@ __device__ __forceinline__ BcPow bc_pow(float v, float x1, float ix1) {
     return p;
 }
 
-// The shifted transformed value t = y - K of set s.
+// The shifted transformed value t = y - K of set s.  The log-based form
+// takes dl = ln|x| - L: a negative x then follows pow's sign rule, NaN for a
+// non-integer lambda (par 0), x^l = |x|^l for an even one (y - K = Wt (r - 1),
+// r = (|x| / x1)^l) and -|x|^l for an odd one (y - K = -Wt (r + 1)).  The
+// reference lists' log-form set (lambda = 0.01) is known not to be an
+// integer.
 template <int KC>
-__device__ __forceinline__ float bc_shifted(int s, float lam, float dl, const BcPow &p, float Wt) {
+__device__ __forceinline__ float bc_shifted(int s, float lam, float dl, const BcPow &p, float Wt, float x, int par) {
     const int kd = bc_kind(KC, s);
-    if (kd == kBcGeneric) return wexpm1(lam * kLog2eAcc * dl, Wt);
+    if (kd == kBcGeneric) {
+        const float e = wexpm1(lam * kLog2eAcc * dl, Wt);
+        if (KC != 0 || !(x < 0.f)) return x >= 0.f ? e : __builtin_nanf("");  // NaN x: NaN
+        return par == 0 ? __builtin_nanf("") : par == 1 ? e : -e - 2.f * Wt;
+    }
     if (kd == kBcCube) return (Wt * p.u) * __builtin_fmaf(p.u, p.u + 3.f, 3.f);
     const float h = kd == kBcQuarter ? p.h4 : kd == kBcHalf ? p.h2 : __builtin_fmaf(p.h2, p.h4, p.h2 + p.h4);
     return Wt * h;
+}
+
+// ln|x| - L for the log-based form; x = 0 gives -1e30, where expm1 is -1 and
+// y(0) = -1/l (as pow(0, l) in estimator.h:56).
+__device__ __forceinline__ float log_ratio(float x, float L) {
+    return fmaxf(__builtin_amdgcn_logf(fabsf(x)) * 0.69314718055994531f - L, -1e30f);
 }
 
 // Per-sample factors of the factored recurrence, shared by all Box-Cox sets.
@@ -371,12 +390,12 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             const float xa = xD * f.a1, ya = yD * f.a1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                // ln x - L; log of 0 (-inf) and of NaN become -1e30, where expm1 gives -1: y(0) = -1/l
-                const float dl = GEN ? fmaxf(__builtin_amdgcn_logf(v[c]) * 0.69314718055994531f - L[c], -1e30f) : 0.f;
+                const float dl = GEN ? log_ratio(v[c], L[c]) : 0.f;
                 const BcPow pw = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
-                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[c][s]), f, xa, ya);
+                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[c][s], v[c], a.lam_par[s]), f,
+                                      xa, ya);
             }
         }
     }
@@ -492,11 +511,11 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         const NFac fr = nfac((float)(j + 1));
         NFac fg;
         if (MM >= 5) fg = nfac((float)(n0 + j + 1));
-        const float dl = T > 0 && GEN ? fmaxf(__builtin_amdgcn_logf(v) * 0.69314718055994531f - L, -1e30f) : 0.f;
+        const float dl = T > 0 && GEN ? log_ratio(v, L) : 0.f;
         const BcPow pw = bc_pow<T, KC>(v, x1, ix1);
 #pragma unroll
         for (int s = 0; s <= T; ++s) {
-            const float t = s == 0 ? v : bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[s]);
+            const float t = s == 0 ? v : bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[s], v, a.lam_par[s]);
             meng_factored<NP>(pm[s], t, fr, 0.f, 0.f);
             if (MM >= 5) {
                 const float dg = t - mg[s];
@@ -670,6 +689,8 @@ AccArgs make_acc_args(stater_bufset *bs, const stater_samples *s) {
     for (int k = 0; k < kMaxSets; ++k) {
         a.lam[k] = k < bs->S ? bs->set_lambda[k] : 1.f;
         a.inv_lam[k] = 1.f / a.lam[k];
+        const float l = a.lam[k];
+        a.lam_par[k] = l != std::floor(l) || std::fabs(l) > 16777216.f ? 0 : (std::fmod(l, 2.f) == 0.f ? 1 : 2);
     }
     a.film_sum = bs->film_sum;
     return a;

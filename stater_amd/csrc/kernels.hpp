@@ -25,6 +25,7 @@ struct AccArgs {
     int W, H;
     float lam[kMaxSets];
     float inv_lam[kMaxSets];
+    int lam_par[kMaxSets];  // 0: lambda not an integer, 1: even, 2: odd (the sign rule of pow for x < 0)
     float *film_sum;    // [4][plane] the Film's RGB contribSum and weight sum (continued), or null
     int fresh;          // start from an empty set: old n and statistics are neither read nor kept
 };

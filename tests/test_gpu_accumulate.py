@@ -93,6 +93,69 @@ def test_accumulate_mt5_m4c(ctx):
         assert fits.mean() > 0.9
 
 
+def _edge_samples(spp, W, seed):
+    """[spp, 5, 8, W] samples, one regime per row: ordinary, a tiny first
+    sample, a zero first sample, zeros and negatives among the samples, a
+    nearly constant pixel, large values, twenty decades of range, a huge
+    first sample."""
+    rng = np.random.default_rng(seed)
+    H = 8
+    x = rng.lognormal(0.0, 1.0, (spp, 3, H, W))
+    x[0, :, 1] = 1e-12
+    x[0, :, 2] = 0.0
+    x[:, :, 3] *= rng.random((spp, 3, W)) > 0.3
+    x[5, 0, 3] = -0.25                        # a negative channel sample
+    x[9, 2, 3, ::3] = -3.0
+    x[:, :, 4] = 1.0 + 1e-5 * rng.standard_normal((spp, 3, W))
+    x[:, :, 5] *= 1e10
+    x[:, :, 6] = 10.0 ** rng.uniform(-20, 10, (spp, 3, W))
+    x[0, :, 7] = 1e15
+    smp = np.zeros((spp, 5, H, W), np.float32)
+    smp[:, :3] = x
+    smp[:, 3:] = rng.random((spp, 2, H, W))
+    return smp
+
+
+@pytest.mark.parametrize("lambdas", [MT5_LAMBDAS, (0.5,), (0.5, 0.75), (0.25, 1.0), (0.25, 0.5, 0.75, 1.0),
+                                     (0.3, 2.0)])
+def test_boxcox_edge_samples_against_fp64(ctx, lambdas):
+    """Every transform form (the reference's lambda lists take the ratio form
+    for 1/4, 1/2, 3/4, 3 and the log form for 0.01; another list takes the
+    log form for all) on edge samples: the FP64 statistics wherever they fit
+    FP32, NaN where a negative sample makes pow NaN, set 0 bit for bit."""
+    W, spp = 64, 48
+    smp = _edge_samples(spp, W, 17)
+    chunk = SampleChunk(W, 8, spp)
+    chunk.from_numpy(smp)
+    bs = sa.BufferSet(ctx, W, 8, lambdas, max_moment=5)
+    bs.accumulate(chunk.desc())
+    ctx.synchronize()
+    n, st = read_stats(bs, lambdas)
+    n_ref, st32 = oracle.accumulate(smp, lambdas, 5)
+    _, st64 = oracle.accumulate(smp, lambdas, 5, fp64=True)
+    assert np.array_equal(n, n_ref)
+    assert np.array_equal(st[0], st32[0], equal_nan=True)
+    for s in range(1, st.shape[0]):
+        nan64 = np.isnan(st64[s])
+        fits = np.all(np.isfinite(st64[s]) & (np.abs(st64[s]) < 1e30), axis=0)
+        fits = np.broadcast_to(fits, st[s].shape)
+        assert np.all(np.isnan(st[s])[nan64]), s                  # the NaN pixels of pow ...
+        bad = fits & ~np.isfinite(st[s])
+        if bad.any():
+            idx = np.argwhere(bad)[:8]
+            print("non-finite where FP64 fits (stat, channel, row, column):", idx.tolist(),
+                  [float(st[s][tuple(i)]) for i in idx], [float(st64[s][tuple(i)]) for i in idx])
+        assert not bad.any(), s                                   # ... and every pixel that fits FP32 finite
+        lam = float(oracle.set_lambdas(lambdas)[s])
+        assert nan64.any() == (lam != round(lam))                 # the samples hold negatives
+        scale = _scales(np.where(np.isfinite(st64[s]), st64[s], 0.0), n)
+        err = np.abs(st[s].astype(np.float64) - st64[s]) / np.maximum(scale, 1e-300)
+        worst = float(np.max(np.where(fits, err, 0.0)))
+        rows = [float(np.max(np.where(fits, err, 0.0)[..., r, :])) for r in range(8)]
+        print(f"lambdas {lambdas} set {s}: worst error {worst:.2e}; per row " + " ".join(f"{v:.1e}" for v in rows))
+        assert worst <= BOXCOX_RTOL, (s, worst)
+
+
 def test_accumulate_in_chunks_continues_sequence(ctx):
     W, H = 70, 33
     sc, chunk, _ = generate(ctx, 2, 3, W, H, 40)

@@ -114,6 +114,19 @@ def test_boxcox_identity_and_limits():
     np.testing.assert_allclose(st[0, 0], x.mean(axis=0), rtol=1e-12)
 
 
+def test_boxcox_negative_samples_follow_pow():
+    """boxCox(x, l) = (pow(x, l) - 1)/l (estimator.h:56) for a negative x:
+    NaN for a non-integer lambda, the real power for an integer one."""
+    smp = _rand_samples(6, 1, 1, 3)
+    smp[2, 0, 0, 0] = -0.5
+    smp[4, 1, 0, 0] = -2.0
+    _, st = oracle.accumulate(smp, (0.5, 3.0), 1, fp64=True)
+    assert np.isnan(st[1, 0, 0, 0, 0]) and np.isnan(st[1, 0, 1, 0, 0])  # lambda 0.5: NaN
+    assert np.isfinite(st[1, 0, 2, 0, 0])
+    x = smp[:, :3].astype(np.float64)
+    np.testing.assert_allclose(st[2, 0], ((x ** 3 - 1) / 3).mean(axis=0), rtol=1e-12)
+
+
 def test_sanitize_counts_black_samples():
     smp = _rand_samples(4, 1, 3, 5)
     smp[1, 0, 0, 0] = np.nan
