@@ -84,7 +84,7 @@ def kernel_work(W, H, spp, S, r, set_lambdas):
         "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         # samples read once; a fresh call writes the statistics (n + 7 per set-channel), the
-        # reference logs and Box-Cox shifted means (3 + 3T planes) and the Film chunk (4 planes)
+        # reference samples and Box-Cox shifted means (3 + 3T planes) and the Film sums (4 planes)
         "accumulate": {"ops": acc_ops, "bytes": P * spp * 20 + P * (4 + S * 7 * 3 * 4) + P * 4 * (3 + 3 * T + 4),
                        "bound": "valu"},
     }
@@ -489,8 +489,10 @@ def main() -> int:
     dk = kernels[dom]
     traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / f"traffic_{dom}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    tdata = json.loads(tfile.read_text()) if tfile.exists() else {}
+    # the profile's figure only where it was taken on this workload
+    if tdata.get("workload") == {"width": W, "height": H, "spp": spp, "radius": r, "sets": S}:
+        traffic = tdata.get("bytes_per_launch")
         traffic_src = (f"profiles/{tfile.name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this command, scaled "
                        f"by the same counters on a copy of known size (bench.py --copy-probe); not measured in this "
                        f"run")
