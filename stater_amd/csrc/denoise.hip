@@ -1826,55 +1826,98 @@ __device__ __forceinline__ void band_store_words(const BandStage<HASW, HWC, TY> 
     }
 }
 
-// One window row dy of the lane's pixel: hrow holds the neighbour row's
-// records (column dx = -r at tx), wrow the wavefront's word groups of the
-// step (group j of column col at j * HWC + col).
-template <bool HASW, int HWC>
-__device__ __forceinline__ void band_row(const WelchArgs &a, const float4 *hrow, const uint4 *wrow, int tx, int x,
-                                         int y, int dy, const float (&gp)[kNF], float (&den)[3], float (&num)[3]) {
-    constexpr int F4 = band_head4<HASW>();
-    const int r = a.r, win = 2 * r + 1;
-    const int w0 = band_w0(a, dy);
-    const float spy = a.sp_scale * (float)(dy * dy);
-    for (int dx = -r; dx <= r; ++dx) {
-        if (dx == 0 && dy == 0) continue;
-        const bool fwd = dy > 0 || (dy == 0 && dx > 0);
-        const int idx = (fwd ? fwd_index(dx, dy, r) : fwd_index(-dx, -dy, r)) - a.band_base;
-        const float4 *hq = hrow + (tx + dx + r) * F4;
-        const float4 h0 = hq[0];
-        const float4 h1 = HASW ? hq[1] : h0, h2 = HASW ? hq[2] : h0;
-        const uint4 wd = wrow[((idx >> 5) - w0) * HWC + tx + r + (fwd ? 0 : dx)];
-        __builtin_amdgcn_sched_barrier(0);
-        float w = 1.f;
-        if (HASW) {
-            const float fq[kNF] = {h0.w, h1.x, h1.y, h1.z, h1.w, h2.x, h2.y};
-            float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
+// One column dx of a window row: the weight and the three channel bits
+// (bit 0 of q3[c] >> sh); DEC writes the pair's decisions.
+template <bool HASW, bool DEC>
+__device__ __forceinline__ void band_col(const WelchArgs &a, const float4 *hq, uint32_t q0, uint32_t q1, uint32_t q2,
+                                         int sh, int dx, int dy, int x, int y, float spy, const float (&gp)[kNF],
+                                         float (&den)[3], float (&num)[3]) {
+    const float4 h0 = hq[0];
+    const float4 h1 = HASW ? hq[1] : h0, h2 = HASW ? hq[2] : h0;
+    __builtin_amdgcn_sched_barrier(0);
+    float w = 1.f;
+    if (HASW) {
+        const float fq[kNF] = {h0.w, h1.x, h1.y, h1.z, h1.w, h2.x, h2.y};
+        float lw = __builtin_fmaf(a.sp_scale, (float)(dx * dx), spy);
 #pragma unroll
-            for (int f = 0; f < kNF; ++f) {
-                const float d = gp[f] - fq[f];
-                lw = __builtin_fmaf(-d, d, lw);
-            }
-            w = __builtin_amdgcn_exp2f(lw);
+        for (int f = 0; f < kNF; ++f) {
+            const float d = gp[f] - fq[f];
+            lw = __builtin_fmaf(-d, d, lw);
         }
-        const uint32_t q3[3] = {wd.x, wd.y, wd.z};
-        const float film[3] = {h0.x, h0.y, h0.z};
-        const int sh = idx & 31;
-        bool acc[3];
+        w = __builtin_amdgcn_exp2f(lw);
+    }
+    const uint32_t q3[3] = {q0, q1, q2};
+    const float film[3] = {h0.x, h0.y, h0.z};
+    bool acc[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            acc[c] = (q3[c] >> sh) & 1u;
-            const float wa = acc[c] ? w : 0.f;
-            den[c] += wa;
-            num[c] = __builtin_fmaf(wa, film[c], num[c]);
-        }
-        if (a.dec) {
-            const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + r) * win + (dx + r)) * 3;
-            for (int c = 0; c < 3; ++c) a.dec[kk + c] = acc[c] ? 1 : 0;
-        }
+    for (int c = 0; c < 3; ++c) {
+        acc[c] = (q3[c] >> sh) & 1u;
+        const float wa = acc[c] ? w : 0.f;
+        den[c] += wa;
+        num[c] = __builtin_fmaf(wa, film[c], num[c]);
+    }
+    if (DEC) {
+        const int win = 2 * a.r + 1;
+        const size_t kk = ((size_t)(y * a.W + x) * win * win + (size_t)(dy + a.r) * win + (dx + a.r)) * 3;
+        for (int c = 0; c < 3; ++c) a.dec[kk + c] = acc[c] ? 1 : 0;
     }
 }
 
-template <bool HASW, int HWC, int TY>
+// One window row dy of the lane's pixel, specialised by kind: all columns
+// backward (dy < 0: the neighbour's word group, at a bit index that falls by
+// one per column), the centre row (dy = 0) and all columns forward (dy > 0:
+// the pixel's own words, turned into two 32-bit masks per channel for the
+// row's 2r + 1 consecutive bits).  hrow holds the neighbour row's records
+// (column dx = -r at tx), wrow the wavefront's word groups of the step
+// (group j of column col at j * HWC + col).
+template <bool HASW, int HWC, bool DEC>
+__device__ __forceinline__ void band_row(const WelchArgs &a, const float4 *hrow, const uint4 *wrow, int tx, int x,
+                                         int y, int dy, const float (&gp)[kNF], float (&den)[3], float (&num)[3]) {
+    constexpr int F4 = band_head4<HASW>();
+    const int r = a.r;
+    const int w0 = band_w0(a, dy);
+    const float spy = a.sp_scale * (float)(dy * dy);
+    const float4 *hq = hrow + tx * F4;  // column dx = -r
+    if (dy < 0) {
+        const int i0 = fwd_index(r, -dy, r) - a.band_base;  // the neighbour's bit of column dx = -r
+        const uint4 *wq = wrow + tx;
+        for (int dx = -r; dx <= r; ++dx, hq += F4, ++wq) {
+            const int idx = i0 - (dx + r);
+            const uint4 wd = wq[((idx >> 5) - w0) * HWC];
+            band_col<HASW, DEC>(a, hq, wd.x, wd.y, wd.z, idx & 31, dx, dy, x, y, spy, gp, den, num);
+        }
+    } else if (dy == 0) {
+        const uint4 own = wrow[tx + r];
+        const uint4 *wq = wrow + tx;
+        for (int dx = -r; dx <= r; ++dx, hq += F4, ++wq) {
+            if (dx == 0) continue;
+            const uint4 wd = dx > 0 ? own : *wq;
+            band_col<HASW, DEC>(a, hq, wd.x, wd.y, wd.z, dx > 0 ? dx - 1 : -dx - 1, dx, dy, x, y, spy, gp, den,
+                                num);
+        }
+    } else {
+        const int lo = fwd_index(-r, dy, r) - a.band_base;  // own bit of column dx = -r
+        const int k0 = (lo >> 5) - w0, sh = lo & 31;
+        const uint4 o0 = wrow[k0 * HWC + tx + r];
+        const uint4 o1 = k0 + 1 < kBandWordSlots ? wrow[(k0 + 1) * HWC + tx + r] : make_uint4(0u, 0u, 0u, 0u);
+        const uint4 o2 = k0 + 2 < kBandWordSlots ? wrow[(k0 + 2) * HWC + tx + r] : make_uint4(0u, 0u, 0u, 0u);
+        // bits lo .. lo + 63 of the row: column dx = -r + j takes bit j
+        const uint32_t m0[3] = {__builtin_amdgcn_alignbit(o1.x, o0.x, (uint32_t)sh),
+                                __builtin_amdgcn_alignbit(o1.y, o0.y, (uint32_t)sh),
+                                __builtin_amdgcn_alignbit(o1.z, o0.z, (uint32_t)sh)};
+        const uint32_t m1[3] = {__builtin_amdgcn_alignbit(o2.x, o1.x, (uint32_t)sh),
+                                __builtin_amdgcn_alignbit(o2.y, o1.y, (uint32_t)sh),
+                                __builtin_amdgcn_alignbit(o2.z, o1.z, (uint32_t)sh)};
+        const int n0 = min(2 * r + 1, 32);
+        int dx = -r;
+        for (int j = 0; j < n0; ++j, ++dx, hq += F4)
+            band_col<HASW, DEC>(a, hq, m0[0], m0[1], m0[2], j, dx, dy, x, y, spy, gp, den, num);
+        for (int j = 32; j <= 2 * r; ++j, ++dx, hq += F4)
+            band_col<HASW, DEC>(a, hq, m1[0], m1[1], m1[2], j - 32, dx, dy, x, y, spy, gp, den, num);
+    }
+}
+
+template <bool HASW, int HWC, int TY, bool DEC>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
     1, lds_waves_per_simd(BandStage<HASW, HWC, TY>::LDS_BYTES, 64 * TY)))) k_welch_band(WelchArgs a) {
     using B = BandStage<HASW, HWC, TY>;
@@ -1912,7 +1955,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
             den[c] = first ? 1.f : a.part[(size_t)c * a.plane + pix];           // the pixel itself, weight 1
             num[c] = first ? film[c] : a.part[(size_t)(3 + c) * a.plane + pix];
         }
-        if (a.dec && first) {
+        if (DEC && first) {
             const int win = 2 * a.r + 1;
             const size_t k0 = ((size_t)(y * a.W + x) * win * win + (size_t)a.r * win + a.r) * 3;
             a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
@@ -1943,8 +1986,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
                 band_load_words<HASW, HWC, TY>(a, S, dy + 1, x0, y0);
             }
             if (active)
-                band_row<HASW, HWC>(a, ring + ((ty + k) % NS) * B::HITEMS, wbuf + (k & 1) * B::WITEMS +
-                                    ty * kBandWordSlots * HWC, tx, x, y, dy, gp, den, num);
+                band_row<HASW, HWC, DEC>(a, ring + ((ty + k) % NS) * B::HITEMS, wbuf + (k & 1) * B::WITEMS +
+                                         ty * kBandWordSlots * HWC, tx, x, y, dy, gp, den, num);
             if (more) {
                 // row k + TY replaces row k - 1, read by nobody since the last barrier
                 band_store_head<HASW, HWC, TY>(S, ring + ((k + TY) % NS) * B::HITEMS);
@@ -1973,7 +2016,9 @@ static int welch_band_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
     STATER_HIP(hipFuncSetAttribute((const void *)k_welch_decide<S, HWC, DTY, true>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)dlds));
     const size_t blds = hasw ? BandStage<true, HWC, TY>::LDS_BYTES : BandStage<false, HWC, TY>::LDS_BYTES;
-    const void *bk = hasw ? (const void *)k_welch_band<true, HWC, TY> : (const void *)k_welch_band<false, HWC, TY>;
+    const bool dec = a.dec != nullptr;
+    const void *bk = hasw ? (dec ? (const void *)k_welch_band<true, HWC, TY, true> : (const void *)k_welch_band<true, HWC, TY, false>)
+                          : (dec ? (const void *)k_welch_band<false, HWC, TY, true> : (const void *)k_welch_band<false, HWC, TY, false>);
     STATER_HIP(hipFuncSetAttribute(bk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds));
     const dim3 dgrid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + DTY - 1) / DTY)));
     const dim3 grid(xcd_grid(((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY)));
@@ -1992,8 +2037,10 @@ static int welch_band_launch(const WelchArgs &a, bool hasw, hipStream_t st) {
         hipLaunchKernelGGL((k_welch_decide<S, HWC, DTY, true>), dgrid, dim3(64 * DTY), dlds, st, b);
         STATER_HIP(hipGetLastError());
         b.head = (float4 *)(b.part + 6 * a.plane);
-        if (hasw) hipLaunchKernelGGL((k_welch_band<true, HWC, TY>), grid, dim3(64 * TY), blds, st, b);
-        else hipLaunchKernelGGL((k_welch_band<false, HWC, TY>), grid, dim3(64 * TY), blds, st, b);
+        if (hasw && dec) hipLaunchKernelGGL((k_welch_band<true, HWC, TY, true>), grid, dim3(64 * TY), blds, st, b);
+        else if (hasw) hipLaunchKernelGGL((k_welch_band<true, HWC, TY, false>), grid, dim3(64 * TY), blds, st, b);
+        else if (dec) hipLaunchKernelGGL((k_welch_band<false, HWC, TY, true>), grid, dim3(64 * TY), blds, st, b);
+        else hipLaunchKernelGGL((k_welch_band<false, HWC, TY, false>), grid, dim3(64 * TY), blds, st, b);
         STATER_HIP(hipGetLastError());
     }
     return STATER_OK;
