@@ -258,7 +258,10 @@ def denoise_vars(n, m2, lnS2, se2, m2C, radius, ds_factor, z, feats=None, nthrea
 
 def var_decisions(n, lnS2, se2, radius, z) -> np.ndarray:
     """The variance filter's decisions [S,3,H,W,win,win] (uint8, 1 = accepted)
-    for FP32 or FP64 lnS2/se2 [S,3,H,W]."""
+    for FP32 or FP64 lnS2/se2 [S,3,H,W]; FP32 inputs are decided with the FP32
+    path's own arithmetic (Z = z^2 se2 rounded, reject iff fma(dl, dl, -Z_q) >
+    Z_p), so a pair within rounding of the threshold goes the way the kernel
+    takes it."""
     S = lnS2.shape[0]
     H, W = n.shape
     win = 2 * radius + 1

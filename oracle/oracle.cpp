@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../include/stater.h"
@@ -791,8 +792,17 @@ void var_decisions_impl(int W, int H, int S, const int32_t *n, const T *lnS2, co
                                 const size_t q = (size_t)yq * W + xq;
                                 if (q == p) acc = true;
                                 else if (n[p] >= 4 && n[q] >= 4) {
-                                    const double dl = (double)lnS2[ip + p] - (double)lnS2[ip + q];
-                                    acc = !(dl * dl > z2 * ((double)se2[ip + p] + (double)se2[ip + q]));
+                                    if (std::is_same<T, float>::value) {
+                                        // the FP32 path's own arithmetic: Z = z^2 se2 rounded, reject iff
+                                        // fma(dl, dl, -Z_q) > Z_p (a NaN operand abstains: accept)
+                                        const float z2f = (float)z * (float)z;
+                                        const float Zp = z2f * (float)se2[ip + p], Zq = z2f * (float)se2[ip + q];
+                                        const float dl = (float)lnS2[ip + p] - (float)lnS2[ip + q];
+                                        acc = !(std::fma(dl, dl, -Zq) > Zp);
+                                    } else {
+                                        const double dl = (double)lnS2[ip + p] - (double)lnS2[ip + q];
+                                        acc = !(dl * dl > z2 * ((double)se2[ip + p] + (double)se2[ip + q]));
+                                    }
                                 }
                             }
                             o[(dy + radius) * win + dx + radius] = acc ? 1 : 0;

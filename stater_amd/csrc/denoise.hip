@@ -69,32 +69,6 @@ struct PreOut {
     float lnS2, se2, m2C, jbt;
 };
 
-__device__ __forceinline__ PreOut prepass_one(int n, float m2, float m3, float m4, float c2x, float c2y, float z,
-                                              bool lev) {
-    PreOut o{0.f, 0.f, 0.f, 0.f};
-    if (n < 4) return o;
-    const float nf = (float)n;
-    if (!(__builtin_isfinite(m2) && __builtin_isfinite(m4))) return PreOut{INFINITY, INFINITY, m2, INFINITY};
-    if (m2 > 0.f) {
-        const float sm = sqrtf(m2);
-        const float S3 = sqrtf(nf) * (m3 / m2) / sm;  // (m3/n)/(m2/n)^1.5
-        const float K = nf * (m4 / m2) / m2;          // (m4/n)/(m2/n)^2
-        o.jbt = S3 * S3 + 0.25f * (K - 3.f) * (K - 3.f);
-        const float cb = nf / (nf - z);
-        const float s2 = m2 / (nf - 1.f);
-        o.lnS2 = logf(fmaxf(cb * s2, 1e-30f));
-        o.se2 = cb * cb * (K - (nf - 3.f) / nf) / (nf - 1.f);
-        float m2c = m2;
-        if (lev) {
-            const float cx = c2x / sm, cy = c2y / sm;
-            m2c = m2 * fmaxf(0.f, 1.f - 12.f * (cx * cx + cy * cy) / nf);
-        }
-        o.m2C = m2c;
-    } else {
-        o.lnS2 = logf(1e-30f);
-    }
-    return o;
-}
 
 // ======================================================= variance filter
 // Two passes per group of up to 3 sets.  The pack pass writes one record per
@@ -161,88 +135,133 @@ constexpr int waitcnt_lgkm(int v) { return 15 | (3 << 14) | (7 << 4) | ((v > 15 
 
 __device__ __forceinline__ float finite_or_max(float v) { return __builtin_isfinite(v) ? v : FLT_MAX; }
 
-// Pack pass, with the prepass of the group's sets.  A workgroup handles 64
-// pixels of a row: it stages their inputs plane by plane (coalesced 256-byte
-// rows; one wavefront per plane): n, the 7 features, then (m2, m3, m4, c2X,
-// c2Y) per set-channel of the group; computes the prepass per pixel and
-// set-channel (lnS2, se2, m2C and the Jarque-Bera term, written to their
-// planes), then jb per pixel and set (the terms of R, G, B in order), and
-// writes the records item by item (consecutive threads, consecutive 16-byte
-// items).
-template <int G>
-constexpr int var_pack_planes() { return 8 + 15 * G; }
+// Pack pass, with the prepass of the group's sets: one thread per pixel,
+// four rows of 64 pixels per workgroup.  Each plane load of a wavefront is one
+// coalesced 256-byte row segment; the pixel's record goes out item by item
+// (the 16-byte stores of consecutive lanes fill consecutive records).  The
+// prepass's factors of n are formed once per pixel and its quotients, roots
+// and logarithm take the hardware reciprocal, reciprocal square root and
+// log2 (1 ulp; the prepass is held to FP64 within 2e-5 absolute for lnS2 and
+// 1e-4 relative for se2 and m2C), where the first version spent about 160
+// instructions per set-channel on IEEE quotients and roots.  The record's
+// s^2 and m2C/(n-1) keep the IEEE quotient 1/(n-1).
+struct PreN {
+    float nf, sqn, cs2, se2f, nm3n, in12, jbw;  // n, sqrt n, c/(n-1), c^2/(n-1), (n-3)/n, 12/n, n/6
+};
 
-template <int G, bool HASW>
-__device__ __forceinline__ float var_pack_input(const VarArgs &a, int pl, int x, int y) {
-    const size_t pix = (size_t)y * a.ld + x;
-    if (pl == 0) return __int_as_float(a.n[pix]);
-    if (pl < 8) return HASW && pl - 1 < a.f.nf ? feat_at(a.f, pl - 1, x, y) : 0.f;
-    const int sc = (pl - 8) / 5, kind = (pl - 8) - sc * 5;  // set-channel g*3 + c; m2, m3, m4, c2X, c2Y
-    const int g = sc / 3, c = sc - g * 3;
-    if (kind >= 3 && !a.lev) return 0.f;
-    const int st = kind == 0 ? ST_M2 : kind == 1 ? ST_M3 : kind == 2 ? ST_M4 : kind == 3 ? ST_C2X : ST_C2Y;
-    return a.stats[(((size_t)(a.g0 + g) * N_STATS + st) * 3 + c) * a.plane + pix];
+__device__ __forceinline__ PreN pre_n(int n, float z) {
+    PreN q;
+    q.nf = (float)n;
+    q.sqn = __builtin_amdgcn_sqrtf(q.nf);
+    const float cb = q.nf * __builtin_amdgcn_rcpf(q.nf - z);
+    const float inm1 = __builtin_amdgcn_rcpf(q.nf - 1.f);
+    q.cs2 = cb * inm1;
+    q.se2f = cb * cb * inm1;
+    const float in = __builtin_amdgcn_rcpf(q.nf);
+    q.nm3n = (q.nf - 3.f) * in;
+    q.in12 = 12.f * in;
+    q.jbw = q.nf / 6.f;
+    return q;
+}
+
+// Per set-channel: lnS2 = ln(c s^2), se2 = c^2 (K - (n-3)/n)/(n-1), the LEV
+// m2C and the Jarque-Bera term S^2 + (K-3)^2/4 (S = sqrt(n) m3/m2^1.5,
+// K = n m4/m2^2, scale-safe: m2 enters as 1/m2 and 1/sqrt(m2)).  n < 4 is
+// degenerate (zeros); a non-finite m2 or m4 gives lnS2 = se2 = +inf (the
+// tests abstain); m2 <= 0 gives lnS2 = ln 1e-30 and zeros.
+__device__ __forceinline__ PreOut prepass_px(int n, const PreN &q, float m2, float m3, float m4, float c2x, float c2y,
+                                             bool lev) {
+    PreOut o{0.f, 0.f, 0.f, 0.f};
+    if (n < 4) return o;
+    if (!(__builtin_isfinite(m2) && __builtin_isfinite(m4))) return PreOut{INFINITY, INFINITY, m2, INFINITY};
+    if (m2 > 0.f) {
+        const float im2 = __builtin_amdgcn_rcpf(m2), rsq = __builtin_amdgcn_rsqf(m2);
+        const float S3 = q.sqn * (m3 * im2) * rsq;
+        const float K = q.nf * (m4 * im2) * im2;
+        const float k3 = K - 3.f;
+        o.jbt = __builtin_fmaf(0.25f * k3, k3, S3 * S3);
+        o.lnS2 = __builtin_amdgcn_logf(fmaxf(q.cs2 * m2, 1e-30f)) * 0.69314718055994531f;
+        o.se2 = q.se2f * (K - q.nm3n);
+        float m2c = m2;
+        if (lev) {
+            const float cx = c2x * rsq, cy = c2y * rsq;
+            m2c = m2 * fmaxf(0.f, __builtin_fmaf(-q.in12, __builtin_fmaf(cx, cx, cy * cy), 1.f));
+        }
+        o.m2C = m2c;
+    } else {
+        o.lnS2 = -69.07755278982137f;  // ln 1e-30
+    }
+    return o;
 }
 
 template <int G, bool HASW>
 __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
     using R = VarRec<G, HASW>;
-    constexpr int NPL = var_pack_planes<G>();
-    __shared__ float in[NPL][64];
-    __shared__ float4 pre[3 * G][64];  // lnS2, se2, m2C, Jarque-Bera term
-    const int y = blockIdx.y, x0 = blockIdx.x * 64;
-    if (blockIdx.x == 0 && y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int i = threadIdx.x; i < NPL * 64; i += 256) {
-        const int pl = i >> 6, px = i & 63;
-        in[pl][px] = x0 + px < a.W ? var_pack_input<G, HASW>(a, pl, x0 + px, y) : 0.f;
-    }
-    __syncthreads();
-    const int npx = min(64, a.W - x0);
-    const size_t row = (size_t)y * a.ld + x0;
-    for (int i = threadIdx.x; i < 3 * G * 64; i += 256) {
-        const int sc = i >> 6, px = i & 63;
-        const float *v = &in[8 + 5 * sc][0];
-        const PreOut o = prepass_one(__float_as_int(in[0][px]), v[px], v[64 + px], v[128 + px], v[192 + px],
-                                     v[256 + px], a.z, a.lev != 0);
-        pre[sc][px] = make_float4(o.lnS2, o.se2, o.m2C, o.jbt);
-        if (px < npx) {
-            const size_t off = ((size_t)a.g0 * 3 + sc) * a.plane + row + px;
-            a.lnS2[off] = o.lnS2;
-            a.se2[off] = o.se2;
-            a.m2C[off] = o.m2C;
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < G * 64; i += 256) {
-        const int g = i >> 6, px = i & 63;
-        const int n = __float_as_int(in[0][px]);
-        float jbs = 0.f;
+    // each wavefront's 64 records pass through LDS, so the global stores are
+    // contiguous (one store at a 176-byte lane stride touches 64 lines)
+    __shared__ float4 recs[4][64 * R::FLOAT4S];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < R::FLOAT4S) zero[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 64, x = x0 + lane, y = blockIdx.y * 4 + wave;
+    if (y >= a.H) return;  // wavefront-uniform
+    const bool act = x < a.W;
+    const size_t pix = (size_t)y * a.ld + (act ? x : x0);
+    float4 *lrec = recs[wave] + lane * R::FLOAT4S;
+    if (act) {
+        const int n = a.n[pix];
+        float f[kNF];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) jbs += (float)n / 6.f * pre[g * 3 + c][px].w;
-        if (px < npx) a.jb[(size_t)(a.g0 + g) * a.plane + row + px] = n < 4 ? 0.f : jbs / 3.f;
-    }
-    float4 *rec = a.rec + row * R::FLOAT4S;
-    for (int j = threadIdx.x; j < npx * R::FLOAT4S; j += 256) {
-        const int px = j / R::FLOAT4S, item = j - px * R::FLOAT4S;
-        const int n = __float_as_int(in[0][px]);
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (item == 0) {
-            v.x = n >= 4 ? 1.f : 0.f;
-            if (HASW) { v.y = in[1][px]; v.z = in[2][px]; v.w = in[3][px]; }
-        } else if (HASW && item == 1) {
-            v = make_float4(in[4][px], in[5][px], in[6][px], in[7][px]);
-        } else if (item < R::ITEMS) {
-            const int sc = item - R::SINGLE4;
-            const float4 o = pre[sc][px];
-            if (n >= 4) { v.x = o.x; v.y = a.z2 * o.y; }
-            if (n >= 2) {
-                const float inv = 1.f / (float)(n - 1);
-                v.z = finite_or_max(in[8 + 5 * sc][px] * inv);
-                v.w = finite_or_max(o.z * inv);
+        for (int k = 0; k < kNF; ++k) f[k] = HASW && k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+        const PreN q = pre_n(n, a.z);
+        const float inv = n >= 2 ? 1.f / (float)(n - 1) : 0.f;
+        const bool lev = a.lev != 0;
+        // every input of the group first: the outputs could alias them, so loads
+        // after a store would each wait for the one before
+        float in[G][3][5];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const float *st = a.stats + ((size_t)(a.g0 + g) * N_STATS * 3) * a.plane + pix;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                in[g][c][0] = st[((size_t)ST_M2 * 3 + c) * a.plane];
+                in[g][c][1] = st[((size_t)ST_M3 * 3 + c) * a.plane];
+                in[g][c][2] = st[((size_t)ST_M4 * 3 + c) * a.plane];
+                in[g][c][3] = lev ? st[((size_t)ST_C2X * 3 + c) * a.plane] : 0.f;
+                in[g][c][4] = lev ? st[((size_t)ST_C2Y * 3 + c) * a.plane] : 0.f;
             }
         }
-        rec[j] = v;
+        lrec[0] = make_float4(n >= 4 ? 1.f : 0.f, f[0], f[1], f[2]);
+        if (HASW) lrec[1] = make_float4(f[3], f[4], f[5], f[6]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            float jbs = 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float m2 = in[g][c][0];
+                const PreOut o = prepass_px(n, q, m2, in[g][c][1], in[g][c][2], in[g][c][3], in[g][c][4], lev);
+                const size_t off = ((size_t)(a.g0 + g) * 3 + c) * a.plane + pix;
+                a.lnS2[off] = o.lnS2;
+                a.se2[off] = o.se2;
+                a.m2C[off] = o.m2C;
+                jbs = __builtin_fmaf(q.jbw, o.jbt, jbs);
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (n >= 4) { v.x = o.lnS2; v.y = a.z2 * o.se2; }
+                if (n >= 2) {
+                    v.z = finite_or_max(m2 * inv);
+                    v.w = finite_or_max(o.m2C * inv);
+                }
+                lrec[R::SINGLE4 + g * 3 + c] = v;
+            }
+            a.jb[(size_t)(a.g0 + g) * a.plane + pix] = n < 4 ? 0.f : jbs / 3.f;
+        }
+        if (R::FLOAT4S > R::ITEMS) lrec[R::ITEMS] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    // the wavefront's own records: no barrier beyond its own LDS writes
+    __builtin_amdgcn_s_waitcnt(waitcnt_lgkm(0));
+    __builtin_amdgcn_wave_barrier();
+    const int npx = min(64, a.W - x0);
+    float4 *grec = a.rec + ((size_t)y * a.ld + x0) * R::FLOAT4S;
+    for (int k = lane; k < npx * R::FLOAT4S; k += 64) grec[k] = recs[wave][k];
 }
 
 // One output pixel's state: its own statistics and the running sums.
@@ -643,7 +662,7 @@ __global__ void __launch_bounds__(64 * (TY + 1))
 
 template <int G, bool HASW, int HWC, int TY, int AHEAD>
 static int var_launch1(const VarArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, (a.H + 3) / 4), dim3(256), 0, st, a,
                        const_cast<float4 *>(a.zero));
     STATER_HIP(hipGetLastError());
     constexpr size_t lds = (size_t)(TY + AHEAD) * var_slot_float4s<G, HASW, HWC>() * 16;
@@ -658,7 +677,7 @@ static int var_launch1(const VarArgs &a, hipStream_t st) {
 
 template <int G, bool HASW, int HWC, int TY, int AHEAD = 2>
 static int var_launch4(const VarArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, a.H), dim3(256), 0, st, a,
+    hipLaunchKernelGGL((k_var_pack<G, HASW>), dim3((a.W + 63) / 64, (a.H + 3) / 4), dim3(256), 0, st, a,
                        const_cast<float4 *>(a.zero));
     STATER_HIP(hipGetLastError());
     const size_t lds = (size_t)var_lds_bytes<G, HASW, HWC, TY, AHEAD>();
