@@ -88,16 +88,18 @@ def test_frame_in_three_calls_equals_one_call(ctx, mode):
     assert np.array_equal(_bits(f1), _bits(oracle.film_image(oracle.film_add(smp, sanitize=True))))
 
 
-def test_welch_filter_reads_and_writes_interleaved_films(ctx):
+@pytest.mark.parametrize("r", [7, 20])
+def test_welch_filter_reads_and_writes_interleaved_films(ctx, r):
     """denoise_film on the reference's interleaved CV_32FC3 film, writing an
-    interleaved filmD, equals the planar run bit for bit (no conversion pass)."""
+    interleaved filmD, equals the planar run bit for bit (no conversion pass),
+    for the record-based filter (r <= 10) and the banded one (r = 20)."""
     from stater_amd.workload import FEATURE_SDS
     W, H = 88, 52
     _, chunk, feats = generate(ctx, 2, 13, W, H, 32)
     bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5, flags=sa.STATER_BUFSET_FILM)
     bs.accumulate(chunk.desc(sanitize=True))
     bs.film_resolve()
-    p = sa.default_params(radius=7)
+    p = sa.default_params(radius=r)
     g = feats.gbuffers(FEATURE_SDS)
     bs.denoise_vars(p, g)
     bs.denoise_film(p, g, film=bs.view("film"))
