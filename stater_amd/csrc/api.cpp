@@ -79,12 +79,12 @@ double t2_crit_host(double nu, double alpha) {
     return nu * (1.0 - x) / x;
 }
 
-TCritPoly make_tcrit_poly(double alpha) {
-    TCritPoly tp{};
-    constexpr int D = kTCritDeg;
-    const double umax = 0.0625;
-    // Chebyshev interpolation of g(u) = t2crit(1/u) on [0, umax] (nodes in
-    // s = 2u/umax - 1), then the monomial coefficients in u.
+// Fit g(u) = t2crit(1/u) on [0, umax] by Chebyshev interpolation of degree D
+// (nodes in s = 2u/umax - 1), convert to monomial coefficients in u, and
+// measure the FP32 Horner evaluation against the FP64 tail on a grid.
+// Returns the largest relative error; cmax gets the largest crit on the range.
+template <int D>
+static double fit_tcrit(double alpha, double umax, float (&c)[D + 1], double &cmax) {
     double gnode[D + 1], cheb[D + 1];
     for (int k = 0; k <= D; ++k) {
         const double s = std::cos(kPi * (k + 0.5) / (D + 1));
@@ -114,26 +114,36 @@ TCritPoly make_tcrit_poly(double alpha) {
             binom = binom * (i - m) / (m + 1);
         }
     }
-    for (int i = 0; i <= D; ++i) tp.c[i] = (float)mu[i];
-    tp.umax = (float)umax;
-    tp.alpha = alpha;
-    // Measure the FP32 evaluation error against the FP64 tail on a grid and
-    // size the exact-tail band well above it.
-    double maxerr = 0, cmax = 0;
+    for (int i = 0; i <= D; ++i) c[i] = (float)mu[i];
+    double maxerr = 0;
+    cmax = 0;
     for (int i = 0; i <= 1000; ++i) {
         const double u = umax * i / 1000.0;
         const float uf = (float)u;
-        float acc = tp.c[D];
-        for (int k = D - 1; k >= 0; --k) acc = std::fma(acc, uf, tp.c[k]);
+        float acc = c[D];
+        for (int k = D - 1; k >= 0; --k) acc = std::fma(acc, uf, c[k]);
         const double ref = u == 0.0 ? normal_t2_crit(alpha) : t2_crit_host(1.0 / u, alpha);
         maxerr = std::max(maxerr, std::fabs((double)acc / ref - 1.0));
         cmax = std::max(cmax, std::max((double)acc, ref));
     }
+    return maxerr;
+}
+
+TCritPoly make_tcrit_poly(double alpha) {
+    TCritPoly tp{};
+    double cmax, cmax_s;
+    const double maxerr = fit_tcrit<kTCritDeg>(alpha, kTCritUmax, tp.c, cmax);
+    const double maxerr_s = fit_tcrit<kTCritDegShort>(alpha, kTCritUmaxShort, tp.cs, cmax_s);
+    tp.umax = (float)kTCritUmax;
+    tp.alpha = alpha;
+    // The exact-tail band sits well above the fit's FP32 evaluation error.
     tp.band = (float)std::max(2e-6, 8.0 * maxerr + 2e-6);
     tp.neg_hi = -(1.f + tp.band);
     tp.neg_lo = -(1.f - tp.band);
     // crit rises with u and is flat between grid points to far below 1 %
     tp.band_abs = (float)((double)tp.band * cmax * 1.01);
+    const double band_s = std::max(2e-6, 8.0 * maxerr_s + 2e-6);
+    tp.band_abs_short = (float)(band_s * cmax_s * 1.01);
     return tp;
 }
 

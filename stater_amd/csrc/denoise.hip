@@ -803,15 +803,31 @@ __device__ __forceinline__ void welch_margin(const TCritPoly &tp, float mp, floa
     if (SMALLN) e = __builtin_fmaf(-kUBig, fmaxf(u - tp.umax, 0.f), e);
 }
 
-// Both margins of the large-n case as one number, g = t2 - crit(u): g >
-// band_abs (band times the largest crit on [0, umax]) implies e > 0 and g <
-// -band_abs implies f < 0.  Within a factor 2 of each other t2 and crit
-// subtract exactly, so g is exact wherever it is compared with +-band_abs.
-__device__ __forceinline__ float welch_gap(const TCritPoly &tp, float mp, float vp, float mq, float vq,
+// Both margins of the large-n case as one number, g = t2 - crit(u), with
+// the degree-D fit c valid on u <= 1/m where every n - 1 of the window is at
+// least m: g > band_abs (band times the largest crit of the fit's range)
+// implies e > 0 and g < -band_abs implies f < 0.  The fma forms t2 - crit
+// with one rounding, so g is exact to within an ulp of itself.  The weight
+// share ap = vp/V takes the legacy multiply (0 x inf = 0), so V = 0 (both
+// variances zero; they are never negative) gives u = rq, inside the fit's
+// range, without a clamp: t2 is then +inf (rejected) for unequal means and
+// NaN (skipped by the max reduction: accepted) for equal ones.
+// The legacy multiply (v_mul_legacy_f32: 0 x anything = 0), an LLVM
+// intrinsic without a clang builtin.
+extern "C" __device__ float stater_fmul_legacy(float, float) __asm("llvm.amdgcn.fmul.legacy");
+
+template <int D>
+__device__ __forceinline__ float welch_gap(const float (&c)[D + 1], float mp, float vp, float mq, float vq,
                                            const PairR &r) {
-    float t2, crit, u;
-    welch_t2_crit<false>(tp, mp, vp, mq, vq, r, t2, crit, u);
-    return t2 - crit;
+    const float V = vp + vq;
+    const float iV = __builtin_amdgcn_rcpf(V);
+    const float dm = mp - mq;
+    const float ap = stater_fmul_legacy(vp, iV);
+    const float u = __builtin_fmaf(__builtin_fmaf(r.A, ap, r.B), ap, r.C);
+    float crit = c[D];
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) crit = __builtin_fmaf(crit, u, c[k]);
+    return __builtin_fmaf(dm * iV, dm, -crit);
 }
 
 // The fast margins of one pair per channel, max-reduced over the NT tested
@@ -841,7 +857,7 @@ __device__ __forceinline__ void welch_fast(const TCritPoly &tp, const float (&Mp
         for (int s = 0; s < NT; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                E[c] = fmaxf(E[c], welch_gap(tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr));
+                E[c] = fmaxf(E[c], welch_gap<kTCritDeg>(tp.c, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             F[c] = E[c] + tp.band_abs;
@@ -920,20 +936,25 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
 
 // Below this n - 1 a pair may have nu < 1/umax (u <= max(1/(np-1), 1/(nq-1))).
 constexpr int kWelchLargeNm1 = 17;
+// From this n - 1 on, every pair has u <= kTCritUmaxShort.
+constexpr int kWelchShortNm1 = 128;
 
-// Whether every pixel of the tile's window region with n >= 2 has
-// n - 1 >= kWelchLargeNm1 (block-uniform).
-__device__ __forceinline__ bool welch_window_large_n(const WelchArgs &a, int x0, int y0, int TYv) {
+// The tile's window level (block-uniform), over the pixels of its window
+// region with n >= 2: 0 if some n - 1 < kWelchLargeNm1, 2 if every
+// n - 1 >= kWelchShortNm1, 1 otherwise.
+__device__ __forceinline__ int welch_window_level(const WelchArgs &a, int x0, int y0, int TYv) {
     const int xa = max(x0 - a.r, 0), xb = min(x0 + TX + a.r, a.W);
     const int ya = max(y0 - a.r, 0), yb = min(y0 + TYv + a.r, a.H);
     const int w = xb - xa, cnt = w * (yb - ya);
-    int small = 0;
+    int small = 0, mid = 0;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
         const int yy = ya + i / w, xx = xa + i % w;
         const int n = a.n[(size_t)yy * a.ld + xx];
         small |= (n >= 2 && n - 1 < kWelchLargeNm1) ? 1 : 0;
+        mid |= (n >= 2 && n - 1 < kWelchShortNm1) ? 1 : 0;
     }
-    return __syncthreads_or(small) == 0;
+    if (__syncthreads_or(small)) return 0;
+    return __syncthreads_or(mid) ? 1 : 2;
 }
 
 // One window row of the Welch filter for the pixel of this lane.
@@ -1080,7 +1101,7 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
             a.dec[k0] = a.dec[k0 + 1] = a.dec[k0 + 2] = 1;
         }
     }
-    const bool large_n = welch_window_large_n(a, x0, y0, TY);
+    const bool large_n = welch_window_level(a, x0, y0, TY) > 0;
 
     for (int t = 0; t < TY; ++t) welch_stage_row<S, HASW, PLAIN, HWC>(a, ring + t * SLOT, y0 - a.r + t, x0);
     __syncthreads();
@@ -1151,10 +1172,12 @@ __device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot
 }
 
 // Pass 1, one window row dy >= 0 of the lane's pixel: dx from dx0 to r.
-// BANDED: bit i of the band's words is its offset band_base + i, and a
-// channel's band_count bits take bits_words words; otherwise the whole
-// forward half-window in kBitsWords words.
-template <int S, bool SMALLN, int HWC, bool BANDED>
+// LEVEL is the tile's window level (welch_window_level): 0 tests both margins
+// per set with the range term, 1 one gap per set with the fit on u <= 1/16,
+// 2 with the short fit on u <= 1/128.  BANDED: bit i of the band's words is
+// its offset band_base + i, and a channel's band_count bits take bits_words
+// words; otherwise the whole forward half-window in kBitsWords words.
+template <int S, int LEVEL, int HWC, bool BANDED>
 __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot, int tx, int dy, int dx0,
                                            const float (&Mp)[S][3], const float (&Vp)[S][3], float okp, float nm1p,
                                            float rp, uint32_t (&word)[3], size_t pix) {
@@ -1177,13 +1200,13 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
         float E[3], F[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
-        if (SMALLN) {
+        if (LEVEL == 0) {
 #pragma unroll
             for (int s = 0; s < S; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float e, f;
-                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
+                    welch_margin<true>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
                     E[c] = fmaxf(E[c], e);
                     F[c] = fmaxf(F[c], f);
                 }
@@ -1192,11 +1215,15 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
             for (int s = 0; s < S; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    E[c] = fmaxf(E[c], welch_gap(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr));
+                    E[c] = fmaxf(E[c], LEVEL == 2 ? welch_gap<kTCritDegShort>(a.tp.cs, Mp[s][c], Vp[s][c], mv[s][c].x,
+                                                                               mv[s][c].y, pr)
+                                                  : welch_gap<kTCritDeg>(a.tp.c, Mp[s][c], Vp[s][c], mv[s][c].x,
+                                                                          mv[s][c].y, pr));
+            const float band = LEVEL == 2 ? a.tp.band_abs_short : a.tp.band_abs;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                F[c] = E[c] + a.tp.band_abs;
-                E[c] = E[c] - a.tp.band_abs;
+                F[c] = E[c] + band;
+                E[c] = E[c] - band;
             }
         }
         bool near = false;
@@ -1210,7 +1237,7 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
                 for (int s = 0; s < S; ++s) {
                     const float2 m = pq[(s * 3 + c) * HWC];
                     float e, f;
-                    welch_margin<SMALLN>(a.tp, Mp[s][c], Vp[s][c], m.x, m.y, pr, e, f);
+                    welch_margin<LEVEL == 0>(a.tp, Mp[s][c], Vp[s][c], m.x, m.y, pr, e, f);
                     if (e > 0.f) { pass = false; break; }
                     if (f >= 0.f && !welch_exact(Mp[s][c], Vp[s][c], nm1p, m.x, m.y, nm1q, a.alpha)) {
                         pass = false;
@@ -1286,7 +1313,7 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
         if (!BANDED && a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4, a.rec_head);
         if (BANDED && a.head) welch_record_head(a, x, y, a.head + pix * a.head4, a.head4);
     }
-    const bool large_n = welch_window_large_n(a, x0, y0, TY);
+    const int level = welch_window_level(a, x0, y0, TY);
 
     // window rows dy = d0 .. d1; staged row j is image row y0 + d0 + j
     const int d0 = BANDED ? a.band_dy0 : 0, d1 = BANDED ? a.band_dy1 : a.r;
@@ -1297,8 +1324,9 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
         const float *slot = ring + ((ty + k - d0) % TY) * SLOT;
         const int dx0 = k == 0 ? 1 : -a.r;
         if (active) {
-            if (large_n) decide_row<S, false, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
-            else decide_row<S, true, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            if (level == 2) decide_row<S, 2, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            else if (level == 1) decide_row<S, 1, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            else decide_row<S, 0, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
         }
         if (k == d1) break;
         __syncthreads();

@@ -58,8 +58,12 @@ __host__ __device__ inline double t2_tail(double t2, double nu) {
     return betainc_reg(0.5 * nu, 0.5, nu / (nu + t2));
 }
 
-// Degree of the critical-value fit.
+// Degree of the critical-value fit on u <= 1/16, and of the short fit on
+// u <= 1/128 that windows whose every n - 1 is at least 128 use.
 constexpr int kTCritDeg = 4;
+constexpr int kTCritDegShort = 2;
+constexpr double kTCritUmax = 1.0 / 16.0;
+constexpr double kTCritUmaxShort = 1.0 / 128.0;
 
 // Critical t^2 of the two-sided level-alpha test as a polynomial in u = 1/nu
 // on [0, umax], i.e. for nu >= 1/umax.  A fast test t2 > crit is trusted
@@ -71,11 +75,14 @@ struct TCritPoly {
     float band;              // relative half-width of the exact-tail band
     float neg_hi, neg_lo;    // -(1 + band), -(1 - band)
     float band_abs;          // band x the largest crit on [0, umax]: t2 - crit beyond +-band_abs is decided
+    float cs[kTCritDegShort + 1];  // the short fit on [0, kTCritUmaxShort]
+    float band_abs_short;          // its band_abs
     double alpha;
 };
 
-// Host: fit the polynomial for a level (Chebyshev interpolation on umax = 1/16,
-// i.e. nu >= 16; smaller nu takes the exact tail).
+// Host: fit the polynomials for a level (Chebyshev interpolation on
+// [0, 1/16], i.e. nu >= 16, where smaller nu takes the exact tail, and on
+// [0, 1/128]).
 TCritPoly make_tcrit_poly(double alpha);
 double t2_crit_host(double nu, double alpha);
 

@@ -221,6 +221,38 @@ def test_welch_filter_mt5(ctx, mt_mode, ragged):
     np.testing.assert_array_equal(discr, want)
 
 
+@pytest.mark.parametrize("r", [10, 20])
+def test_welch_filter_window_levels(ctx, r):
+    """Tiles of every window level of the decision pass in one film: some
+    n - 1 below 17 (x < 40), every n - 1 from 17 on (n = 64), and every
+    n - 1 from 128 on (n = 160 for x >= 100, which the 64-column tile at
+    x >= 128 sees alone)."""
+    W, H = 192, 48
+    counts = _ragged_counts(W, H)
+    counts[:, 100:] = 160
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=160, seed=7, counts=counts)
+    p = sa.default_params(radius=r, mt_mode=2)
+    g = feats.gbuffers()
+    bs.denoise_vars(p, g)
+    bs.denoise_film(p, g)
+    ctx.synchronize()
+    n, st = read_stats(bs, MT5_LAMBDAS)
+    assert np.array_equal(n, counts)
+    varCD = read_set_buf(bs, MT5_LAMBDAS, "varCD")
+    f = feats.to_numpy()
+    feats_list = [(f["albedo"], FEATURE_SDS[0]), (f["normal"], FEATURE_SDS[1]), (f["depth"], FEATURE_SDS[2])]
+    ref = oracle.denoise_film(n, st[:, 0].copy(), varCD, st[0, 0], r, p.ds_factor, p.alpha, mt_mode=2,
+                              feats=feats_list, want_decisions=True)
+    dec = _decisions(ctx, bs, p, g, False, W, H, r)
+    diff = dec != ref["decisions"]
+    for x0, x1 in ((0, 64), (64, 128), (128, W)):  # per tile column, each level on its own
+        part = diff[:, x0:x1]
+        assert np.count_nonzero(part) / part.size <= DECISION_MISMATCH_MAX, (x0, np.count_nonzero(part))
+    same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
+    filmD = hipmem.read_plane(bs.view("filmD"))
+    np.testing.assert_allclose(filmD[same], ref["filmD"][same], rtol=FILTER_RTOL)
+
+
 @pytest.mark.parametrize("cascade,use_levc", [(0, 0), (1, 0)])
 def test_welch_filter_variants(ctx, cascade, use_levc):
     """The reference's variant builds: "M" (CASCADE_MODE 0, USE_LEVC 0: the
