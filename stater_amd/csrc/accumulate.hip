@@ -50,29 +50,55 @@
 namespace stater {
 
 
-// Reference update, estimator.h:226-305, evaluated exactly as written.
+// The quotients d / n of a sample's three channels, correctly rounded, from
+// an FP64 reciprocal rn of n (within 2^-52 relative): RN32(d * rn) is
+// RN32(d / n) wherever the quotient is a normal float, because d / n lies at
+// least 2^-49 (relative) away from every midpoint of two adjacent floats
+// there: d carries 24 significant bits and n is an integer of at most 24
+// (times a power of two), so d - M n, for a midpoint M with 25 significant
+// bits, is a nonzero multiple of M's last bit's half.  A subnormal quotient
+// can fall on a midpoint exactly (d = 3 x 2^-149, n = 2), so when one lane
+// meets one, the three take the IEEE division instead (rare: |d| < n 2^-126).
+__device__ __forceinline__ void div_by_n(const float (&d)[3], double rn, float nf, float (&q)[3]) {
+#pragma clang fp contract(off)
+    bool sub = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        q[c] = (float)((double)d[c] * rn);
+        sub |= __builtin_amdgcn_classf(q[c], 0x90);  // negative or positive subnormal
+    }
+    if (__builtin_expect(sub, 0)) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[c] = d[c] / nf;
+    }
+}
+
+// 1/nf in FP64 from the FP32 reciprocal b = rcp(nf) (within a few ulp): two
+// Newton steps take its relative error e to e^2 and then below 2^-52.
+__device__ __forceinline__ double recip64(float nf, float b) {
+    const double n = (double)nf, y0 = (double)b;
+    const double y1 = __builtin_fma(y0, __builtin_fma(-n, y0, 1.0), y0);
+    return __builtin_fma(y1, __builtin_fma(-n, y1, 1.0), y1);
+}
+
+// Reference update, estimator.h:226-305, evaluated exactly as written, with
+// d = x - mean and dN = d / n formed by the caller for the three channels.
 template <int MM>
-__device__ __forceinline__ void meng_exact(float *st, float x, float nf, float xD, float yD) {
+__device__ __forceinline__ void meng_exact(float *st, float x, float d, float dN, float xD, float yD) {
 #pragma clang fp contract(off)
     float &mean = st[0];
     if (MM == 1) {
-        const float d = x - mean;
-        const float dN = d / nf;
         mean += dN;
         return;
     }
     if (MM == 2) {
-        const float d = x - mean;
-        const float dN = d / nf;
         mean += dN;
         st[1] += d * (d - dN);
         return;
     }
     float &m2 = st[1];
     float &m3 = st[2];
-    const float d = x - mean;
     const float d2 = d * d;
-    const float dN = d / nf;
     const float dN2 = dN * dN;
     mean += dN;
     m2 += d * (d - dN);
@@ -380,13 +406,19 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         if (a.sanitize && !keep_sample(v[0], v[1], v[2])) v[0] = v[1] = v[2] = 0.f;
         ++n;
         const float nf = (float)n;
+        const NFac f = nfac(nf);
+        {
+            float d[3], dN[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            fsum[c] += v[c];
-            meng_exact<MM>(st[c][0], v[c], nf, xD, yD);
+            for (int c = 0; c < 3; ++c) d[c] = v[c] - st[c][0][0];
+            div_by_n(d, recip64(nf, f.b), nf, dN);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                fsum[c] += v[c];
+                meng_exact<MM>(st[c][0], v[c], d[c], dN[c], xD, yD);
+            }
         }
         if (T > 0) {
-            const NFac f = nfac(nf);
             const float xa = xD * f.a1, ya = yD * f.a1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 
 #include "kernels.hpp"
@@ -941,20 +942,84 @@ constexpr int kWelchShortNm1 = 128;
 
 // The tile's window level (block-uniform), over the pixels of its window
 // region with n >= 2: 0 if some n - 1 < kWelchLargeNm1, 2 if every
-// n - 1 >= kWelchShortNm1, 1 otherwise.
-__device__ __forceinline__ int welch_window_level(const WelchArgs &a, int x0, int y0, int TYv) {
+// n - 1 >= kWelchShortNm1, 1 otherwise.  UNIFORM: level 3 instead of 2 when
+// all those pixels have the same n, which is returned in *nuni.
+template <bool UNIFORM = false>
+__device__ __forceinline__ int welch_window_level(const WelchArgs &a, int x0, int y0, int TYv, int *nuni = nullptr) {
+    __shared__ int s_lo, s_hi;
+    if (UNIFORM) {
+        if (threadIdx.x == 0) {
+            s_lo = INT_MAX;
+            s_hi = 0;
+        }
+        __syncthreads();
+    }
     const int xa = max(x0 - a.r, 0), xb = min(x0 + TX + a.r, a.W);
     const int ya = max(y0 - a.r, 0), yb = min(y0 + TYv + a.r, a.H);
     const int w = xb - xa, cnt = w * (yb - ya);
-    int small = 0, mid = 0;
+    int small = 0, mid = 0, lo = INT_MAX, hi = 0;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
         const int yy = ya + i / w, xx = xa + i % w;
         const int n = a.n[(size_t)yy * a.ld + xx];
         small |= (n >= 2 && n - 1 < kWelchLargeNm1) ? 1 : 0;
         mid |= (n >= 2 && n - 1 < kWelchShortNm1) ? 1 : 0;
+        if (UNIFORM && n >= 2) {
+            lo = min(lo, n);
+            hi = max(hi, n);
+        }
+    }
+    if (UNIFORM && lo <= hi) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
     }
     if (__syncthreads_or(small)) return 0;
-    return __syncthreads_or(mid) ? 1 : 2;
+    if (__syncthreads_or(mid)) return 1;
+    if (UNIFORM && s_lo == s_hi) {
+        *nuni = s_lo;
+        return 3;
+    }
+    return 2;
+}
+
+// The test on a window whose pixels (those with n >= 2) all have the same n,
+// n - 1 >= kWelchShortNm1.  With r = 1/(n - 1) and the weight shares
+// ap = vp/V, aq = vq/V (ap + aq = 1), 1/nu = r (ap^2 + aq^2) = r (1 - 2P)
+// with P = ap aq in [0, 1/4], so the short fit crit(u) = cs0 + cs1 u + cs2 u^2
+// is a quadratic a0 + a1 P + a2 P^2 in P.  Its minimax line on [0, 1/4]
+// (P^2 -> P/4 - 1/128, off by at most 1/128) gives
+//     g = t2 - crit ~ t2 - alpha - beta ap aq,
+// three instructions after t2 where the general form takes five; the band
+// widens by |a2|/128 (5e-5 at n = 256, against crit ~ 7.9) and two ulp of
+// alpha for its rounding.  nbeta = -beta.
+struct WelchUniform {
+    float alpha, nbeta, band;
+};
+
+static_assert(kTCritDegShort == 2, "the uniform-n test expands the short fit as a quadratic");
+
+__device__ __forceinline__ WelchUniform welch_uniform(const TCritPoly &tp, int n) {
+    const double r = 1.0 / (double)(n - 1);
+    const double c0 = tp.cs[0], c1 = tp.cs[1], c2 = tp.cs[2];
+    const double a0 = c0 + r * (c1 + r * c2), a1 = -2.0 * r * (c1 + 2.0 * r * c2), a2 = 4.0 * r * r * c2;
+    const double alpha = a0 - a2 / 128.0;
+    WelchUniform U;
+    U.alpha = (float)alpha;
+    U.nbeta = (float)-(a1 + 0.25 * a2);
+    U.band = (float)((double)tp.band_abs_short + fabs(a2) / 128.0 + fabs(alpha) * 0x1p-22);
+    return U;
+}
+
+// g of the uniform-n form; vpn = nbeta vp.  The legacy multiplies keep V = 0
+// (both variances zero) and V = inf as welch_gap takes them: the shares are 0
+// there, so g = t2 - alpha (+inf for unequal means with V = 0: rejected; NaN
+// for equal ones: skipped by the max reduction, accepted).
+__device__ __forceinline__ float welch_gap_uniform(const WelchUniform &U, float mp, float vp, float vpn, float mq,
+                                                   float vq) {
+    const float V = vp + vq;
+    const float iV = __builtin_amdgcn_rcpf(V);
+    const float dm = mp - mq;
+    const float h = __builtin_fmaf(dm * iV, dm, -U.alpha);
+    return __builtin_fmaf(stater_fmul_legacy(vpn, iV), stater_fmul_legacy(vq, iV), h);
 }
 
 // One window row of the Welch filter for the pixel of this lane.
@@ -1174,16 +1239,22 @@ __device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot
 // Pass 1, one window row dy >= 0 of the lane's pixel: dx from dx0 to r.
 // LEVEL is the tile's window level (welch_window_level): 0 tests both margins
 // per set with the range term, 1 one gap per set with the fit on u <= 1/16,
-// 2 with the short fit on u <= 1/128.  BANDED: bit i of the band's words is
+// 2 with the short fit on u <= 1/128, 3 with the uniform-n form of the short
+// fit (welch_uniform).  BANDED: bit i of the band's words is
 // its offset band_base + i, and a channel's band_count bits take bits_words
 // words; otherwise the whole forward half-window in kBitsWords words.
 template <int S, int LEVEL, int HWC, bool BANDED>
 __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot, int tx, int dy, int dx0,
                                            const float (&Mp)[S][3], const float (&Vp)[S][3], float okp, float nm1p,
-                                           float rp, uint32_t (&word)[3], size_t pix) {
+                                           float rp, const WelchUniform &U, uint32_t (&word)[3], size_t pix) {
     const int base = BANDED ? a.band_base : 0;
     const int count = BANDED ? a.band_count : 2 * a.r * a.r + 2 * a.r;
     const int nwords = BANDED ? a.bits_words : kBitsWords;
+    float Vn[S][3];  // level 3: nbeta vp
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Vn[s][c] = LEVEL == 3 ? U.nbeta * Vp[s][c] : 0.f;
     for (int dx = dx0; dx <= a.r; ++dx) {
         const float *q = slot + tx + a.r + dx;
         const float2 *pq = (const float2 *)(slot + 2 * HWC) + tx + a.r + dx;
@@ -1215,11 +1286,13 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
             for (int s = 0; s < S; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    E[c] = fmaxf(E[c], LEVEL == 2 ? welch_gap<kTCritDegShort>(a.tp.cs, Mp[s][c], Vp[s][c], mv[s][c].x,
-                                                                               mv[s][c].y, pr)
-                                                  : welch_gap<kTCritDeg>(a.tp.c, Mp[s][c], Vp[s][c], mv[s][c].x,
-                                                                          mv[s][c].y, pr));
-            const float band = LEVEL == 2 ? a.tp.band_abs_short : a.tp.band_abs;
+                    E[c] = fmaxf(E[c], LEVEL == 3   ? welch_gap_uniform(U, Mp[s][c], Vp[s][c], Vn[s][c], mv[s][c].x,
+                                                                        mv[s][c].y)
+                                       : LEVEL == 2 ? welch_gap<kTCritDegShort>(a.tp.cs, Mp[s][c], Vp[s][c],
+                                                                                mv[s][c].x, mv[s][c].y, pr)
+                                                    : welch_gap<kTCritDeg>(a.tp.c, Mp[s][c], Vp[s][c], mv[s][c].x,
+                                                                           mv[s][c].y, pr));
+            const float band = LEVEL == 3 ? U.band : LEVEL == 2 ? a.tp.band_abs_short : a.tp.band_abs;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 F[c] = E[c] + band;
@@ -1313,7 +1386,10 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
         if (!BANDED && a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4, a.rec_head);
         if (BANDED && a.head) welch_record_head(a, x, y, a.head + pix * a.head4, a.head4);
     }
-    const int level = welch_window_level(a, x0, y0, TY);
+    int nuni = 0;
+    const int level = welch_window_level<true>(a, x0, y0, TY, &nuni);
+    WelchUniform U{0.f, 0.f, 0.f};
+    if (level == 3) U = welch_uniform(a.tp, nuni);
 
     // window rows dy = d0 .. d1; staged row j is image row y0 + d0 + j
     const int d0 = BANDED ? a.band_dy0 : 0, d1 = BANDED ? a.band_dy1 : a.r;
@@ -1324,9 +1400,10 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
         const float *slot = ring + ((ty + k - d0) % TY) * SLOT;
         const int dx0 = k == 0 ? 1 : -a.r;
         if (active) {
-            if (level == 2) decide_row<S, 2, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
-            else if (level == 1) decide_row<S, 1, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
-            else decide_row<S, 0, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, word, pix);
+            if (level == 3) decide_row<S, 3, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
+            else if (level == 2) decide_row<S, 2, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
+            else if (level == 1) decide_row<S, 1, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
+            else decide_row<S, 0, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
         }
         if (k == d1) break;
         __syncthreads();

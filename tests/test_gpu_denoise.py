@@ -224,13 +224,17 @@ def test_welch_filter_mt5(ctx, mt_mode, ragged):
 @pytest.mark.parametrize("r", [10, 20])
 def test_welch_filter_window_levels(ctx, r):
     """Tiles of every window level of the decision pass in one film: some
-    n - 1 below 17 (x < 40), every n - 1 from 17 on (n = 64), and every
-    n - 1 from 128 on (n = 160 for x >= 100, which the 64-column tile at
-    x >= 128 sees alone)."""
-    W, H = 192, 48
+    n - 1 below 17 (x < 40), every n - 1 from 17 on (n = 64), every n - 1
+    from 128 on with one n (n = 160 for 100 <= x < 224, which the 64-column
+    tile at x >= 128 sees alone: the uniform-n form) and with several
+    (n = 150 and 170 in alternate columns from x = 224 on, which the tiles
+    from x = 192 on see)."""
+    W, H = 320, 48
     counts = _ragged_counts(W, H)
     counts[:, 100:] = 160
-    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=160, seed=7, counts=counts)
+    counts[:, 224::2] = 150
+    counts[:, 225::2] = 170
+    sc, chunk, feats, bs = _mt5_setup(ctx, W, H, spp=170, seed=7, counts=counts)
     p = sa.default_params(radius=r, mt_mode=2)
     g = feats.gbuffers()
     bs.denoise_vars(p, g)
@@ -245,7 +249,7 @@ def test_welch_filter_window_levels(ctx, r):
                               feats=feats_list, want_decisions=True)
     dec = _decisions(ctx, bs, p, g, False, W, H, r)
     diff = dec != ref["decisions"]
-    for x0, x1 in ((0, 64), (64, 128), (128, W)):  # per tile column, each level on its own
+    for x0, x1 in ((0, 64), (64, 128), (128, 192), (192, W)):  # per tile column, each level on its own
         part = diff[:, x0:x1]
         assert np.count_nonzero(part) / part.size <= DECISION_MISMATCH_MAX, (x0, np.count_nonzero(part))
     same = ~np.any(diff, axis=(2, 3)).transpose(2, 0, 1)
@@ -491,20 +495,24 @@ def test_variance_filter_edge_statistics(ctx):
     assert np.max(np.abs(gCD[ok] - vCD[ok]) / np.maximum(np.abs(vCD[ok]), 1e-30)) < FILTER_RTOL
 
 
-def test_welch_filter_edge_statistics(ctx):
+@pytest.mark.parametrize("n_bulk", [20, 200])
+def test_welch_filter_edge_statistics(ctx, n_bulk):
     """Welch tests on statistics written straight into a set: zero variances
     (equal means accept, different means reject), a NaN mean (abstains),
-    pixels with one or no sample, and a non-finite variance."""
+    pixels with one or no sample, and a non-finite variance; with n = 200 on
+    every pixel that has two samples or more, the uniform-n form decides."""
     W, H, r = 40, 24, 4
     rng = np.random.default_rng(23)
     S = len(oracle.set_lambdas(MT5_LAMBDAS))
-    n = np.full((H, W), 20, np.int32)
-    n[2, 2], n[2, 3], n[5, 7] = 0, 1, 2
+    n = np.full((H, W), n_bulk, np.int32)
+    n[2, 2], n[2, 3] = 0, 1
+    if n_bulk == 20:
+        n[5, 7] = 2
     st = np.zeros((S, 7, 3, H, W), np.float32)
     st[:, 0] = rng.normal(1.0, 0.05, (S, 3, H, W))
     st[:, 1] = rng.uniform(0.5, 2.0, (S, 3, H, W))
     st[:, 2] = rng.normal(0, 0.1, (S, 3, H, W))
-    st[:, 3] = rng.uniform(1.5, 4.0, (S, 3, H, W)) * st[:, 1] ** 2 / 20.0
+    st[:, 3] = rng.uniform(1.5, 4.0, (S, 3, H, W)) * st[:, 1] ** 2 / n_bulk
     st[:, 4:6] = rng.normal(0, 0.05, (S, 2, 3, H, W))
     st[:, 6] = rng.uniform(0.5, 1.0, (S, 3, H, W))
     st[:, 1, :, 8:10, 8:12] = 0.0           # zero variance, equal means in pairs of columns
