@@ -50,28 +50,15 @@
 namespace stater {
 
 
-// The quotients d / n of a sample's three channels, correctly rounded, from
-// an FP64 reciprocal rn of n (within 2^-52 relative): RN32(d * rn) is
-// RN32(d / n) wherever the quotient is a normal float, because d / n lies at
-// least 2^-49 (relative) away from every midpoint of two adjacent floats
-// there: d carries 24 significant bits and n is an integer of at most 24
-// (times a power of two), so d - M n, for a midpoint M with 25 significant
-// bits, is a nonzero multiple of M's last bit's half.  A subnormal quotient
-// can fall on a midpoint exactly (d = 3 x 2^-149, n = 2), so when one lane
-// meets one, the three take the IEEE division instead (rare: |d| < n 2^-126).
-__device__ __forceinline__ void div_by_n(const float (&d)[3], double rn, float nf, float (&q)[3]) {
-#pragma clang fp contract(off)
-    bool sub = false;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        q[c] = (float)((double)d[c] * rn);
-        sub |= __builtin_amdgcn_classf(q[c], 0x90);  // negative or positive subnormal
-    }
-    if (__builtin_expect(sub, 0)) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) q[c] = d[c] / nf;
-    }
-}
+// The quotient d / n, correctly rounded, from an FP64 reciprocal rn of n
+// (within 2^-52 relative): RN32(d * rn) is RN32(d / n) wherever the quotient
+// is a normal float, because d / n lies at least 2^-49 (relative) away from
+// every midpoint of two adjacent floats there: d carries 24 significant bits
+// and n is an integer of at most 24 (times a power of two), so d - M n, for
+// a midpoint M with 25 significant bits, is a nonzero multiple of half of
+// M's last bit.  A subnormal quotient can fall on a midpoint exactly
+// (d = 3 x 2^-149, n = 2); it needs |d| < n 2^-126, which exact_tiny covers.
+__device__ __forceinline__ float div_by_n(float d, double rn) { return (float)((double)d * rn); }
 
 // 1/nf in FP64 from the FP32 reciprocal b = rcp(nf) (within a few ulp): two
 // Newton steps take its relative error e to e^2 and then below 2^-52.
@@ -81,20 +68,44 @@ __device__ __forceinline__ double recip64(float nf, float b) {
     return __builtin_fma(y1, __builtin_fma(-n, y1, 1.0), y1);
 }
 
+// The correctly rounded square root of x >= 2^-104 (and of 0, inf, NaN): the
+// hardware root s is within an ulp, and the residuals x - s'' s of its two
+// neighbours s'' decide which of the three is the rounded root (the
+// expansion the compiler emits for sqrtf, without its scaling of tiny
+// inputs and its 0/inf select, neither needed on this range).  Checked
+// against sqrtf on all 2^32 floats: it differs only for 0 < x < 2^-104.
+__device__ __forceinline__ float sqrt_rn_large(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = s;
+    if (__builtin_fmaf(-sd, s, x) <= 0.f) r = sd;
+    if (__builtin_fmaf(-su, s, x) > 0.f) r = su;
+    return r;
+}
+
+// A deviation d = x - mean with 0 < |d| < 2^-40 (absurdly small against any
+// radiance) takes the IEEE division and sqrtf: everywhere else the quotient
+// d / n is a normal float or zero, and the radicand d (d - d/n) of m1 is 0
+// (n = 1) or at least 2^-81, where div_by_n and sqrt_rn_large are exact.
+__device__ __forceinline__ bool exact_tiny(float d) { return d != 0.f && fabsf(d) < 0x1p-40f; }
+
 // Reference update, estimator.h:226-305, evaluated exactly as written, with
-// d = x - mean and dN = d / n formed by the caller for the three channels.
+// d = x - mean and dN = d / n formed by the caller for the three channels;
+// for MM 5 it returns the radicand d (d - dN) of m1's increment, whose
+// square root the caller takes and adds (the update's last step).
 template <int MM>
-__device__ __forceinline__ void meng_exact(float *st, float x, float d, float dN, float xD, float yD) {
+__device__ __forceinline__ float meng_exact(float *st, float x, float d, float dN, float xD, float yD) {
 #pragma clang fp contract(off)
     float &mean = st[0];
     if (MM == 1) {
         mean += dN;
-        return;
+        return 0.f;
     }
     if (MM == 2) {
         mean += dN;
         st[1] += d * (d - dN);
-        return;
+        return 0.f;
     }
     float &m2 = st[1];
     float &m3 = st[2];
@@ -103,13 +114,13 @@ __device__ __forceinline__ void meng_exact(float *st, float x, float d, float dN
     mean += dN;
     m2 += d * (d - dN);
     m3 += -3.f * dN * m2 + d * (d2 - dN2);
-    if (MM == 3) return;
+    if (MM == 3) return 0.f;
     float &m4 = st[3];
     m4 += -4.f * dN * m3 - 6.f * dN2 * m2 + d * (d * d2 - dN * dN2);
-    if (MM == 4) return;
+    if (MM == 4) return 0.f;
     st[4] += (x - mean) * xD;
     st[5] += (x - mean) * yD;
-    st[6] += sqrtf(d * (d - dN));  // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
+    return d * (d - dN);
 }
 
 // Wt (e^z - 1) for z = w ln 2, evaluated in base 2: w = k + f with k the
@@ -407,16 +418,27 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         ++n;
         const float nf = (float)n;
         const NFac f = nfac(nf);
-        {
-            float d[3], dN[3];
+        // Set 0 takes div_by_n and sqrt_rn_large; a lane with a tiny
+        // deviation (never, in practice) redoes its quotients with the IEEE
+        // division before the update and its roots with sqrtf after it, so
+        // that the update itself stays one basic block.
+        float d[3], dN[3], p[3], r1[3];
+        const double rn = recip64(nf, f.b);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) d[c] = v[c] - st[c][0][0];
-            div_by_n(d, recip64(nf, f.b), nf, dN);
+        for (int c = 0; c < 3; ++c) {
+            d[c] = v[c] - st[c][0][0];
+            dN[c] = div_by_n(d[c], rn);
+        }
+        const bool tiny = exact_tiny(d[0]) | exact_tiny(d[1]) | exact_tiny(d[2]);
+        if (__builtin_expect(tiny, 0)) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                fsum[c] += v[c];
-                meng_exact<MM>(st[c][0], v[c], d[c], dN[c], xD, yD);
-            }
+            for (int c = 0; c < 3; ++c) dN[c] = d[c] / nf;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            fsum[c] += v[c];
+            p[c] = meng_exact<MM>(st[c][0], v[c], d[c], dN[c], xD, yD);
+            r1[c] = MM >= 5 ? sqrt_rn_large(p[c]) : 0.f;
         }
         if (T > 0) {
             const float xa = xD * f.a1, ya = yD * f.a1;
@@ -429,6 +451,14 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
                     meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[c][s], v[c], a.lam_par[s]), f,
                                       xa, ya);
             }
+        }
+        if (MM >= 5) {
+            if (__builtin_expect(tiny, 0)) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) r1[c] = sqrtf(p[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) st[c][0][6] += r1[c];
         }
     }
     asm volatile("" : "+v"(pix));

@@ -156,6 +156,56 @@ def test_boxcox_edge_samples_against_fp64(ctx, lambdas):
         assert worst <= BOXCOX_RTOL, (s, worst)
 
 
+def _extreme_set0_samples(spp, W, seed):
+    """Rows whose deviations x - mean reach every branch of set 0's exact
+    update: below 2^-40 (the IEEE division and sqrtf), subnormal quotients
+    and exact midpoints among them, radicands around 2^-104, deviations on
+    both sides of 2^-40, values whose squares overflow, and zeros."""
+    rng = np.random.default_rng(seed)
+    H = 8
+    tiny = np.float32(2.0 ** -149)
+    x = np.empty((spp, 3, H, W), np.float64)
+    x[:, :, 0] = 1e-14 * rng.lognormal(0.0, 1.0, (spp, 3, W))
+    x[:, :, 1] = 1e-30 * rng.lognormal(0.0, 1.0, (spp, 3, W))
+    x[:, :, 2] = 1e-40 * rng.lognormal(0.0, 1.0, (spp, 3, W))            # subnormal samples
+    x[:, :, 3] = tiny * rng.integers(0, 8, (spp, 3, W))                  # quotients on midpoints (3 x 2^-149 / 2)
+    x[:, :, 4] = 2.0 ** -52 * (1.0 + rng.uniform(-1.0, 1.0, (spp, 3, W)))  # radicands near 2^-104
+    x[:, :, 5] = 2.0 ** -39 * rng.uniform(0.0, 2.0, (spp, 3, W))        # deviations around 2^-40
+    x[:, :, 6] = 10.0 ** rng.uniform(18, 38, (spp, 3, W))                # d^2 overflows
+    x[:, :, 7] = rng.lognormal(0.0, 1.0, (spp, 3, W)) * (rng.random((spp, 3, W)) > 0.5)
+    smp = np.zeros((spp, 5, H, W), np.float32)
+    smp[:, :3] = x.astype(np.float32)
+    smp[:, 3:] = rng.random((spp, 2, H, W))
+    return smp
+
+
+@pytest.mark.parametrize("mm", [1, 2, 3, 4, 5])
+def test_set0_exact_at_extreme_deviations(ctx, mm):
+    """Set 0 takes d / n from an FP64 reciprocal of n and the root of m1's
+    increment from a neighbour-checked hardware square root, with the IEEE
+    division and sqrtf where a deviation is below 2^-40: bit for bit the
+    reference's FP32 update on samples that reach every one of those forms,
+    for every max_moment (estimator.h:226-305)."""
+    W, spp = 64, 40
+    smp = _extreme_set0_samples(spp, W, 31)
+    chunk = SampleChunk(W, 8, spp)
+    chunk.from_numpy(smp)
+    bs = sa.BufferSet(ctx, W, 8, (), max_moment=mm)
+    bs.accumulate(chunk.desc())
+    ctx.synchronize()
+    n, st = read_stats(bs, ())
+    n_ref, st_ref = oracle.accumulate(smp, (), mm)
+    assert np.array_equal(n, n_ref)
+    k = min(mm, 4) if mm < 5 else 7
+    got, want = st[0, :k], st_ref[0, :k]
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    if not same.all():
+        idx = np.argwhere(~same)[:8]
+        print("differing (stat, channel, row, column):", idx.tolist(),
+              [(float(got[tuple(i)]), float(want[tuple(i)])) for i in idx])
+    assert same.all()
+
+
 def test_accumulate_in_chunks_continues_sequence(ctx):
     W, H = 70, 33
     sc, chunk, _ = generate(ctx, 2, 3, W, H, 40)
