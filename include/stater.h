@@ -494,6 +494,15 @@ int  stater_server_destroy(stater_server *s);
 int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *p,
                                 const stater_gbuffers *gbufs, const stater_plane *film,
                                 int plain, uint8_t *dec, void *stream);
+/* Checks the fast exact forms of the accumulation's untransformed update
+ * against the IEEE ones over all 2^32 float bit patterns x (device counts[2],
+ * written by the call): counts[0] = non-negative x (-0 and NaN included)
+ * outside (0, 2^-104) whose neighbour-checked square root differs from
+ * sqrtf(x); counts[1] = x with
+ * |x| >= 2^-40 or x = 0 whose quotient by a pseudo-random n in [1, 2^24]
+ * through the FP64 reciprocal differs from x / n.  Both are 0 on a working
+ * build; no reference counterpart (it checks this build's arithmetic). */
+int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *counts, void *stream);
 
 #ifdef __cplusplus
 }

@@ -68,12 +68,13 @@ __device__ __forceinline__ double recip64(float nf, float b) {
     return __builtin_fma(y1, __builtin_fma(-n, y1, 1.0), y1);
 }
 
-// The correctly rounded square root of x >= 2^-104 (and of 0, inf, NaN): the
+// The correctly rounded square root of x >= 2^-104 (and of +-0, inf, NaN): the
 // hardware root s is within an ulp, and the residuals x - s'' s of its two
 // neighbours s'' decide which of the three is the rounded root (the
 // expansion the compiler emits for sqrtf, without its scaling of tiny
 // inputs and its 0/inf select, neither needed on this range).  Checked
-// against sqrtf on all 2^32 floats: it differs only for 0 < x < 2^-104.
+// against sqrtf on all 2^32 floats: among the non-negative ones it differs
+// only for 0 < x < 2^-104 (stater_debug_exact_forms).
 __device__ __forceinline__ float sqrt_rn_large(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u);
@@ -781,3 +782,46 @@ int launch_accumulate(stater_bufset *bs, const stater_samples *s, int mode, hipS
 }
 
 }  // namespace stater
+
+// ---------------------------------------------------------------- testing
+namespace stater {
+
+// Checks of set 0's fast exact forms against the IEEE ones, one float bit
+// pattern per thread over all 2^32: counts[0] counts the non-negative x
+// outside (0, 2^-104), -0 and NaN included, where sqrt_rn_large(x) and
+// sqrtf(x) differ (NaN equal to NaN; a radicand d (d - d/n) is never
+// negative: the hardware root of a negative subnormal is -0, not NaN),
+// counts[1] the d without exact_tiny(d) where div_by_n, with recip64 of a
+// pseudo-random n in [1, 2^24], differs from d / n.
+__global__ void __launch_bounds__(256) k_exact_forms(uint64_t base, unsigned long long *counts) {
+    const uint32_t i = (uint32_t)(base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const float x = __uint_as_float(i);
+    auto same = [](float a, float b) { return __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b); };
+    const bool in_range = x == 0.f || x >= 0x1p-104f || x != x;
+    if (in_range && !same(sqrt_rn_large(x), sqrtf(x))) atomicAdd(&counts[0], 1ull);
+    uint32_t h = i * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    const float nf = (float)(1u + (h & 0xFFFFFFu));
+    if (!exact_tiny(x) && !same(div_by_n(x, recip64(nf, __builtin_amdgcn_rcpf(nf))), x / nf))
+        atomicAdd(&counts[1], 1ull);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+}  // namespace stater
+
+extern "C" int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *counts, void *stream) {
+    using namespace stater;
+    if (!ctx || !counts) return fail(STATER_E_ARG, "stater_debug_exact_forms: NULL argument");
+    STATER_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    k_zero_u64<<<1, 64, 0, st>>>(counts, 2);
+    const uint64_t per = (uint64_t)1 << 28;  // 2^20 blocks of 256 threads per launch, 16 launches
+    for (uint64_t b = 0; b < ((uint64_t)1 << 32); b += per) k_exact_forms<<<(unsigned)(per / 256), 256, 0, st>>>(b, counts);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}

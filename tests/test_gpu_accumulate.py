@@ -5,6 +5,8 @@ restatement of StatTile (src/statistics/estimator.h:226-324) on the same
 samples; the Box-Cox statistics are compared with the FP64 evaluation of the
 same recurrence at the relative tolerance stated below.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -204,6 +206,18 @@ def test_set0_exact_at_extreme_deviations(ctx, mm):
         print("differing (stat, channel, row, column):", idx.tolist(),
               [(float(got[tuple(i)]), float(want[tuple(i)])) for i in idx])
     assert same.all()
+
+
+def test_exact_forms_on_every_float(ctx):
+    """The neighbour-checked square root equals sqrtf on every non-negative
+    float outside (0, 2^-104), -0 and NaN included, and the quotient through the FP64 reciprocal of n equals the
+    IEEE d / n for every float d outside 0 < |d| < 2^-40 (with a pseudo-random
+    n per d): the ranges on which set 0's update uses them."""
+    buf = hipmem.DeviceBuffer(16)
+    sa.check(sa.load_library().stater_debug_exact_forms(ctx.h, C.c_void_p(buf.ptr), None))
+    ctx.synchronize()
+    counts = buf.download(np.uint64, 2)
+    assert counts.tolist() == [0, 0], counts
 
 
 def test_accumulate_in_chunks_continues_sequence(ctx):
