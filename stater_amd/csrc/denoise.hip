@@ -31,6 +31,7 @@
 #include <climits>
 #include <cmath>
 
+#include "exact.hpp"
 #include "kernels.hpp"
 
 namespace stater {
@@ -1208,70 +1209,134 @@ __device__ __forceinline__ int fwd_index(int dx, int dy, int r) {
     return dy == 0 ? dx - 1 : r + (dy - 1) * (2 * r + 1) + dx + r;
 }
 
-// Pass 1 LDS slot: ok, n - 1, then (M, V) per set-channel.
+// Pass 1 LDS slot: ok (-1 where n >= 2, so that its sign bit is the flag; +1
+// otherwise), n - 1, then per column its 3S (M, V) pairs side by side, so a
+// lane reads a neighbour's operands at immediate offsets from one address
+// (a column is 24 S bytes: for S = 6, nine 16-byte units, an odd count, and
+// the lanes' reads are bank-conflict free).
 template <int S>
 struct DecideLayout {
     static constexpr int FLOATS = 2 + 6 * S;
 };
 
+// The decision pass's (M, V) of a pixel: welch_load with V = var/n through
+// the FP64 reciprocal rn of nf = max(n, 1), which gives the IEEE quotient bit
+// for bit (div_exact_by_n).
+__device__ __forceinline__ void decide_load(const WelchArgs &a, size_t pix, int xq, int yq, int s, int c, int n,
+                                            float nf, double rn, float &M, float &V) {
+    M = s == 0 ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * a.film_xs + c * a.film_cs)
+               : a.mean[(size_t)s * a.set_stride + c * a.plane + pix];
+    float var;
+    if (a.var_is_m2) var = n >= 2 ? a.var[(size_t)s * a.set_stride + c * a.plane + pix] / (float)(n - 1) : 0.f;
+    else var = a.var[((size_t)s * 3 + c) * a.plane + pix];
+    V = n >= 1 ? div_exact_by_n(var, rn, nf) : 0.f;
+    if (!(__builtin_isfinite(M) && __builtin_isfinite(V))) { M = NAN; V = 0.f; }
+}
+
+// Stages image row yq of the tile's halo: lane l takes columns l and l + 64,
+// wavefront w the set-channels w, w + (waves), ... of them, so n, its
+// reciprocal and the pixel address are formed once per column and the
+// global loads of a set-channel plane stay coalesced.
 template <int S, int HWC>
 __device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot, int yq, int x0) {
     const int HW = TX + 2 * a.r;
-    const int items = HW * (1 + 3 * S);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
     float2 *pairs = (float2 *)(slot + 2 * HWC);
-    for (int i = threadIdx.x; i < items; i += blockDim.x) {
-        const int kind = i / HW, col = i - kind * HW;
+    for (int col = lane; col < HW; col += 64) {
         const int xq = x0 - a.r + col;
         const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
-        if (kind == 0) {
-            const int n = in ? a.n[(size_t)yq * a.ld + xq] : 0;
-            slot[col] = n >= 2 ? 1.f : 0.f;
+        const size_t pix = in ? (size_t)yq * a.ld + xq : 0;
+        const int n = in ? a.n[pix] : 0;
+        if (wave == 0) {
+            slot[col] = n >= 2 ? -1.f : 1.f;
             slot[HWC + col] = n >= 2 ? (float)(n - 1) : 1.f;
-        } else {
-            const int sc = kind - 1, s = sc / 3, c = sc - s * 3;
+        }
+        const float nf = (float)max(n, 1);
+        const double rn = recip64(nf, __builtin_amdgcn_rcpf(nf));
+        for (int sc = wave; sc < 3 * S; sc += waves) {
+            const int s = sc / 3, c = sc - 3 * s;
             float M = 0.f, V = 0.f;
-            if (in) welch_load(a, xq, yq, s, c, M, V);
-            pairs[sc * HWC + col] = make_float2(M, V);
+            if (in) decide_load(a, pix, xq, yq, s, c, n, nf, rn, M, V);
+            pairs[col * (3 * S) + sc] = make_float2(M, V);
         }
     }
+}
+
+// Stores the word group that ends at forward index idx (bits shifted in
+// from below, the group's first offset highest): reversed and shifted down
+// so that bit i is the group's i-th offset, masked by both pixels' ok flags;
+// to the weighting pass's record as one (c0, c1, c2, 0) group, or to the bit
+// planes.
+template <bool BANDED>
+__device__ __forceinline__ void flush_words(const WelchArgs &a, int idx, bool okp, uint32_t (&word)[3],
+                                            uint32_t &okw, size_t pix) {
+    const uint32_t sh = 31u - (uint32_t)(idx & 31);
+    const uint32_t ok = okp ? __builtin_bitreverse32(okw) >> sh : 0u;
+    uint32_t w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c] = (__builtin_bitreverse32(word[c]) >> sh) & ok;
+    if (!BANDED && a.rec_f4) {
+        ((uint4 *)(a.rec + pix * a.rec_f4 + a.rec_head))[idx >> 5] = make_uint4(w[0], w[1], w[2], 0u);
+    } else {
+        const int nwords = BANDED ? a.bits_words : kBitsWords;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.bits[((size_t)c * nwords + (idx >> 5)) * a.plane + pix] = w[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) word[c] = 0u;
+    okw = 0u;
 }
 
 // Pass 1, one window row dy >= 0 of the lane's pixel: dx from dx0 to r.
 // LEVEL is the tile's window level (welch_window_level): 0 tests both margins
 // per set with the range term, 1 one gap per set with the fit on u <= 1/16,
 // 2 with the short fit on u <= 1/128, 3 with the uniform-n form of the short
-// fit (welch_uniform).  BANDED: bit i of the band's words is
-// its offset band_base + i, and a channel's band_count bits take bits_words
-// words; otherwise the whole forward half-window in kBitsWords words.
+// fit (welch_uniform).  Each column leaves per channel a number R whose sign
+// is the decision (negative: accepted): the reject margin for level 0, the
+// largest gap minus the band otherwise, +-1 where the exact path decided.
+// Its sign bit is shifted into word[c] from below (alignbit), and the
+// neighbour's ok flag (the sign of its staged ok) into okw, so a word holds
+// its offsets in reverse order until flush_words turns it round.
 template <int S, int LEVEL, int HWC, bool BANDED>
 __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot, int tx, int dy, int dx0,
-                                           const float (&Mp)[S][3], const float (&Vp)[S][3], float okp, float nm1p,
-                                           float rp, const WelchUniform &U, uint32_t (&word)[3], size_t pix) {
+                                           const float (&Mp)[S][3], const float (&Vp)[S][3], bool okp, float nm1p,
+                                           float rp, const WelchUniform &U, uint32_t (&word)[3], uint32_t &okw,
+                                           size_t pix) {
     const int base = BANDED ? a.band_base : 0;
     const int count = BANDED ? a.band_count : 2 * a.r * a.r + 2 * a.r;
-    const int nwords = BANDED ? a.bits_words : kBitsWords;
     float Vn[S][3];  // level 3: nbeta vp
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Vn[s][c] = LEVEL == 3 ? U.nbeta * Vp[s][c] : 0.f;
     for (int dx = dx0; dx <= a.r; ++dx) {
-        const float *q = slot + tx + a.r + dx;
-        const float2 *pq = (const float2 *)(slot + 2 * HWC) + tx + a.r + dx;
+        const int col = tx + a.r + dx;
+        const float2 *pq = (const float2 *)(slot + 2 * HWC) + col * (3 * S);
         float2 mv[S][3];
-        const float okq = q[0];
-        const float nm1q = q[HWC];
+        const float okq = slot[col];
+        const float nm1q = slot[HWC + col];
+        if ((3 * S) % 2 == 0) {  // whole 16-byte units per column (slot, 2 HWC floats and the column 16-byte aligned)
+            const float4 *p4 = (const float4 *)__builtin_assume_aligned(pq, 16);
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+            for (int k = 0; k < 3 * S / 2; ++k) {
+                const float4 t = p4[k];
+                mv[(2 * k) / 3][(2 * k) % 3] = make_float2(t.x, t.y);
+                mv[(2 * k + 1) / 3][(2 * k + 1) % 3] = make_float2(t.z, t.w);
+            }
+        } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) mv[s][c] = pq[(s * 3 + c) * HWC];
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) mv[s][c] = pq[s * 3 + c];
+        }
         __builtin_amdgcn_sched_barrier(0);
         const PairR pr = pair_r(rp, __builtin_amdgcn_rcpf(nm1q));
-        const float okpq = okq * okp;
-        float E[3], F[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
+        float R[3];
+        bool nc[3];
         if (LEVEL == 0) {
+            float E[3], F[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) E[c] = F[c] = -INFINITY;
 #pragma unroll
             for (int s = 0; s < S; ++s)
 #pragma unroll
@@ -1281,12 +1346,20 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
                     E[c] = fmaxf(E[c], e);
                     F[c] = fmaxf(F[c], f);
                 }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                nc[c] = !(E[c] > 0.f) && F[c] >= 0.f;
+                R[c] = E[c];  // E <= F: outside the band E > 0 rejects, E <= F < 0 accepts
+            }
         } else {  // one running maximum per channel, the band applied once
+            float G[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G[c] = -INFINITY;
 #pragma unroll
             for (int s = 0; s < S; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    E[c] = fmaxf(E[c], LEVEL == 3   ? welch_gap_uniform(U, Mp[s][c], Vp[s][c], Vn[s][c], mv[s][c].x,
+                    G[c] = fmaxf(G[c], LEVEL == 3   ? welch_gap_uniform(U, Mp[s][c], Vp[s][c], Vn[s][c], mv[s][c].x,
                                                                         mv[s][c].y)
                                        : LEVEL == 2 ? welch_gap<kTCritDegShort>(a.tp.cs, Mp[s][c], Vp[s][c],
                                                                                 mv[s][c].x, mv[s][c].y, pr)
@@ -1295,20 +1368,17 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
             const float band = LEVEL == 3 ? U.band : LEVEL == 2 ? a.tp.band_abs_short : a.tp.band_abs;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                F[c] = E[c] + band;
-                E[c] = E[c] - band;
+                nc[c] = fabsf(G[c]) <= band;
+                R[c] = G[c] - band;  // outside the band: > band rejects, < -band accepts
             }
         }
-        bool near = false;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) near |= !(E[c] > 0.f) && F[c] >= 0.f;
-        if (near && okpq > 0.f) {
+        if ((nc[0] || nc[1] || nc[2]) && okq < 0.f && okp) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (E[c] > 0.f || F[c] < 0.f) continue;
+                if (!nc[c]) continue;
                 bool pass = true;
                 for (int s = 0; s < S; ++s) {
-                    const float2 m = pq[(s * 3 + c) * HWC];
+                    const float2 m = pq[s * 3 + c];
                     float e, f;
                     welch_margin<LEVEL == 0>(a.tp, Mp[s][c], Vp[s][c], m.x, m.y, pr, e, f);
                     if (e > 0.f) { pass = false; break; }
@@ -1317,22 +1387,14 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
                         break;
                     }
                 }
-                E[c] = pass ? -1.f : 1.f;
+                R[c] = pass ? -1.f : 1.f;
             }
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) word[c] = __builtin_amdgcn_alignbit(word[c], __float_as_uint(R[c]), 31);
+        okw = __builtin_amdgcn_alignbit(okw, __float_as_uint(okq), 31);
         const int idx = fwd_index(dx, dy, a.r) - base;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) word[c] |= (okpq > 0.f && !(E[c] > 0.f) ? 1u : 0u) << (idx & 31);
-        if ((idx & 31) == 31 || idx == count - 1) {
-            if (a.rec_f4) {  // straight into the weighting pass's record, as one (c0, c1, c2, 0) group
-                ((uint4 *)(a.rec + pix * a.rec_f4 + a.rec_head))[idx >> 5] = make_uint4(word[0], word[1], word[2], 0u);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) a.bits[((size_t)c * nwords + (idx >> 5)) * a.plane + pix] = word[c];
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) word[c] = 0u;
-        }
+        if ((idx & 31) == 31 || idx == count - 1) flush_words<BANDED>(a, idx, okp, word, okw, pix);
     }
 }
 
@@ -1366,21 +1428,24 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
     const size_t pix = (size_t)y * a.ld + x;
 
     float Mp[S][3], Vp[S][3];
-    float okp = 0.f, nm1p = 1.f, rp = 0.f;
+    bool okp = false;
+    float nm1p = 1.f, rp = 0.f;
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Mp[s][c] = Vp[s][c] = 0.f;
     if (active) {
         const int n = a.n[pix];
-        okp = n >= 2 ? 1.f : 0.f;
+        okp = n >= 2;
         nm1p = n >= 2 ? (float)(n - 1) : 1.f;
         rp = __builtin_amdgcn_rcpf(nm1p);
+        const float nf = (float)max(n, 1);
+        const double rn = recip64(nf, __builtin_amdgcn_rcpf(nf));
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                welch_load(a, x, y, s, c, Mp[s][c], Vp[s][c]);
+                decide_load(a, pix, x, y, s, c, n, nf, rn, Mp[s][c], Vp[s][c]);
                 if (!BANDED || a.band_dy0 == 0) a.discr[((size_t)s * 3 + c) * a.plane + pix] = Vp[s][c];
             }
         if (!BANDED && a.rec_f4) welch_record_head(a, x, y, a.rec + pix * a.rec_f4, a.rec_head);
@@ -1395,15 +1460,19 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
     const int d0 = BANDED ? a.band_dy0 : 0, d1 = BANDED ? a.band_dy1 : a.r;
     for (int t = 0; t < TY; ++t) decide_stage_row<S, HWC>(a, ring + t * SLOT, y0 + d0 + t, x0);
     __syncthreads();
-    uint32_t word[3] = {0u, 0u, 0u};
+    uint32_t word[3] = {0u, 0u, 0u}, okw = 0u;
     for (int k = d0; k <= d1; ++k) {  // dy = k
         const float *slot = ring + ((ty + k - d0) % TY) * SLOT;
         const int dx0 = k == 0 ? 1 : -a.r;
         if (active) {
-            if (level == 3) decide_row<S, 3, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
-            else if (level == 2) decide_row<S, 2, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
-            else if (level == 1) decide_row<S, 1, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
-            else decide_row<S, 0, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, pix);
+            if (level == 3)
+                decide_row<S, 3, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, okw, pix);
+            else if (level == 2)
+                decide_row<S, 2, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, okw, pix);
+            else if (level == 1)
+                decide_row<S, 1, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, okw, pix);
+            else
+                decide_row<S, 0, HWC, BANDED>(a, slot, tx, k, dx0, Mp, Vp, okp, nm1p, rp, U, word, okw, pix);
         }
         if (k == d1) break;
         __syncthreads();
@@ -1415,11 +1484,11 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
 template <int S, int HWC, int TY, bool BANDED = false>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(
     1, lds_waves_per_simd(TY * DecideLayout<S>::FLOATS * HWC * 4, 64 * TY)))) k_welch_decide(WelchArgs a) {
-    extern __shared__ float ring[];
+    extern __shared__ float4 ring4[];  // float4: the dynamic ring starts 16-byte aligned after the static LDS
     const int tiles_x = (a.W + TX - 1) / TX;
     const int t = xcd_tile(tiles_x * ((a.H + TY - 1) / TY));
     if (t < 0) return;
-    decide_tile<S, HWC, TY, BANDED>(a, t % tiles_x, t / tiles_x, ring);
+    decide_tile<S, HWC, TY, BANDED>(a, t % tiles_x, t / tiles_x, (float *)ring4);
 }
 
 // Pass 2 LDS slot: features, film[3], then the 3 x NW decision words of the
