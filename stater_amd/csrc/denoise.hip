@@ -1794,15 +1794,18 @@ __device__ __forceinline__ void acc_pix_col(const WelchArgs &a, const AccCol<HAS
     const int j = dx + a.r;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        uint32_t bit;
+        // the decision bit sign-extended to a mask (all ones: accepted), and
+        // the weight masked with it: two instructions where a compare and a
+        // select take three
+        int mask;
         if (M == kAccFwd) {
-            bit = (R.m[ch] >> j) & 1u;
+            mask = __builtin_amdgcn_sbfe((int)R.m[ch], j, 1);
         } else if (M == kAccBwd) {
-            bit = (q3[ch] >> sh) & 1u;
+            mask = __builtin_amdgcn_sbfe((int)q3[ch], sh, 1);
         } else {  // centre row: dx > 0 own index dx - 1 (word 0), dx < 0 the neighbour's
-            bit = dx > 0 ? (R.o0[ch] >> (dx - 1)) & 1u : (q3[ch] >> sh) & 1u;
+            mask = dx > 0 ? __builtin_amdgcn_sbfe((int)R.o0[ch], dx - 1, 1) : __builtin_amdgcn_sbfe((int)q3[ch], sh, 1);
         }
-        const float wa = bit ? w : 0.f;
+        const float wa = __uint_as_float(__float_as_uint(w) & (uint32_t)mask);
         P.den[ch] += wa;
         P.num[ch] = __builtin_fmaf(wa, film[ch], P.num[ch]);
     }
