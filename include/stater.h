@@ -503,6 +503,13 @@ int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *
  * through the FP64 reciprocal differs from x / n.  Both are 0 on a working
  * build; no reference counterpart (it checks this build's arithmetic). */
 int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *counts, void *stream);
+/* VALU issue probe: blocks x 256 threads, each running 16 independent FP32
+ * FMA chains (register operands only) for iters steps (16 x iters FMAs per
+ * thread); out (device,
+ * blocks x 256 floats) receives the chains' sums.  Timing it measures the
+ * non-packed FP32 instruction rate the kernels' rooflines are priced
+ * against; no reference counterpart. */
+int stater_debug_valu_rate(stater_ctx *ctx, int blocks, int iters, float *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -786,3 +786,36 @@ extern "C" int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *cou
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }
+
+namespace stater {
+
+// VALU issue probe: 16 independent FP32 FMA chains per lane, a <- a a + a
+// (one v_fmac with register operands only: an SGPR or literal operand, a
+// compare or a select issue at a lower rate), so a launch of many waves
+// measures the rate at which the SIMDs take non-packed FP32 instructions.
+// Starting in (-1, 0) every chain decays towards 0 (no inf or NaN).
+__global__ void __launch_bounds__(256) k_valu_rate(float *out, int iters, float c1, float c2) {
+    float a[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a[i] = c1 + c2 * (float)i - 1e-6f * (float)threadIdx.x;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] = __builtin_fmaf(a[i], a[i], a[i]);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += a[i];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace stater
+
+extern "C" int stater_debug_valu_rate(stater_ctx *ctx, int blocks, int iters, float *out, void *stream) {
+    using namespace stater;
+    if (!ctx || !out || blocks <= 0 || iters <= 0) return fail(STATER_E_ARG, "stater_debug_valu_rate: bad argument");
+    STATER_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    k_valu_rate<<<blocks, 256, 0, st>>>(out, iters, -0.5f, 1e-3f);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}

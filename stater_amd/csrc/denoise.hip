@@ -1309,6 +1309,16 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Vn[s][c] = LEVEL == 3 ? U.nbeta * Vp[s][c] : 0.f;
+    // The fit's coefficients and bands in VGPRs for this row (kernel arguments
+    // sit in SGPRs, and an FMA that reads an SGPR issues at half the rate).
+    TCritPoly tp = a.tp;
+    if (LEVEL < 3) {
+#pragma unroll
+        for (int k = 0; k <= kTCritDeg; ++k) asm volatile("" : "+v"(tp.c[k]));
+#pragma unroll
+        for (int k = 0; k <= kTCritDegShort; ++k) asm volatile("" : "+v"(tp.cs[k]));
+        asm volatile("" : "+v"(tp.umax), "+v"(tp.neg_hi), "+v"(tp.neg_lo), "+v"(tp.band_abs), "+v"(tp.band_abs_short));
+    }
     for (int dx = dx0; dx <= a.r; ++dx) {
         const int col = tx + a.r + dx;
         const float2 *pq = (const float2 *)(slot + 2 * HWC) + col * (3 * S);
@@ -1342,7 +1352,7 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float e, f;
-                    welch_margin<true>(a.tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
+                    welch_margin<true>(tp, Mp[s][c], Vp[s][c], mv[s][c].x, mv[s][c].y, pr, e, f);
                     E[c] = fmaxf(E[c], e);
                     F[c] = fmaxf(F[c], f);
                 }
@@ -1361,11 +1371,11 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
                 for (int c = 0; c < 3; ++c)
                     G[c] = fmaxf(G[c], LEVEL == 3   ? welch_gap_uniform(U, Mp[s][c], Vp[s][c], Vn[s][c], mv[s][c].x,
                                                                         mv[s][c].y)
-                                       : LEVEL == 2 ? welch_gap<kTCritDegShort>(a.tp.cs, Mp[s][c], Vp[s][c],
+                                       : LEVEL == 2 ? welch_gap<kTCritDegShort>(tp.cs, Mp[s][c], Vp[s][c],
                                                                                 mv[s][c].x, mv[s][c].y, pr)
-                                                    : welch_gap<kTCritDeg>(a.tp.c, Mp[s][c], Vp[s][c], mv[s][c].x,
+                                                    : welch_gap<kTCritDeg>(tp.c, Mp[s][c], Vp[s][c], mv[s][c].x,
                                                                            mv[s][c].y, pr));
-            const float band = LEVEL == 3 ? U.band : LEVEL == 2 ? a.tp.band_abs_short : a.tp.band_abs;
+            const float band = LEVEL == 3 ? U.band : LEVEL == 2 ? tp.band_abs_short : tp.band_abs;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 nc[c] = fabsf(G[c]) <= band;
