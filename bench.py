@@ -27,24 +27,23 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-PEAK_LANE_OPS = 39.3   # T lane-ops/s: 256 CU x 64 lanes x 2.4 GHz, one FP32 op per lane per clock
+# VALU peak: MI355X's published FP32 vector rate, 157.3 TFLOP/s, is 78.6 T
+# lane-ops/s with an FMA counted as one lane-op (256 CU x 4 SIMD x 32 lanes x
+# 2.4 GHz).  Non-packed FP32 reaches it: bench.py --valu-probe (16 independent
+# FMA chains per lane, many waves) measures about 75 T.
+PEAK_LANE_OPS = 78.6   # T lane-ops/s
 PEAK_HBM = 8000.0      # GB/s
-# Algorithmic work constants (lane-ops; a transcendental counts 4).
-WELCH_OPS_PER_TEST = 16
-VAR_OPS_PER_TEST = 8
-WEIGHT_OPS_PER_PAIR = 20
-# Accumulation, per sample-pixel: the shared factors of the recurrence (count,
-# 1/n, sqrt(1 - 1/n), ...), set 0's reference FP32 update per channel (IEEE
-# division and square root included), the factored update per Box-Cox
-# set-channel, and the transforms: per channel ln x - L when some set takes
-# the log-based expm1 (per set-channel), and for lambda 1/4, 1/2, 3/4, 3 the
-# ratio forms u = x/x1 - 1 (3), rho^(1/2) - 1 (10), rho^(1/4) - 1 (10),
-# rho^(3/4) - 1 (2), rho^3 - 1 (4), one multiply per such set.
-ACC_SHARED_OPS = 16
-ACC_SET0_OPS_PER_CHANNEL = 40
-ACC_BC_OPS_PER_SET_CHANNEL = 15
-ACC_EXPM1_OPS = 12
-ACC_LOG_OPS = 7
+# Algorithmic work per unit (lane-ops), the per-unit figures of the path's
+# scope table: a transcendental (rcp, sqrt, log2, exp2) counts 1, the rate at
+# which it issues beside other VALU work on MI355X (it co-issues; on its own
+# it takes about 2.7 issue slots).
+WELCH_OPS_PER_TEST = 13    # dm, V, dm^2, shares, 1/nu, rcp, crit(u) in 6 FMAs, compare and reduce
+VAR_OPS_PER_TEST = 8       # diff, square, add, multiply, compare, select, two accumulations
+WEIGHT_OPS_PER_PAIR = 17   # spatial and 7 feature range terms, exp2 (once per ordered pair)
+# Accumulation, per sample-pixel: the M4C update of one set-channel is 22
+# FP32 lane-ops and one square root (m1); per channel one log2 for the
+# Box-Cox transforms and one exp2 per transformed set.
+ACC_OPS_PER_SET_CHANNEL = 22 + 1
 
 
 def log(msg: str) -> None:
@@ -59,18 +58,10 @@ def window_pairs(W: int, H: int, r: int) -> int:
 
 
 def acc_ops_per_sample(set_lambdas):
-    """Lane-ops of the accumulation per sample-pixel for the transformed sets' lambdas."""
-    kinds = [{0.25: "q", 0.5: "h", 0.75: "tq", 3.0: "c"}.get(float(l), "g") for l in set_lambdas]
-    per_channel = ACC_SET0_OPS_PER_CHANNEL + len(kinds) * ACC_BC_OPS_PER_SET_CHANNEL
-    if "g" in kinds:
-        per_channel += ACC_LOG_OPS + kinds.count("g") * ACC_EXPM1_OPS
-    power = [k for k in kinds if k != "g"]
-    if power:
-        per_channel += 3 + len(power)
-        per_channel += 10 if {"q", "h", "tq"} & set(power) else 0
-        per_channel += 10 if {"q", "tq"} & set(power) else 0
-        per_channel += 2 * power.count("tq") + 4 * power.count("c")
-    return ACC_SHARED_OPS + 3 * per_channel
+    """Algorithmic lane-ops of the M4C accumulation per sample-pixel with T = len(set_lambdas)
+    Box-Cox sets: 3 channels x ((1 + T) set updates + a log2 + T exp2)."""
+    T = len(set_lambdas)
+    return 3 * ((1 + T) * ACC_OPS_PER_SET_CHANNEL + (1 if T else 0) + T)
 
 
 def kernel_work(W, H, spp, S, r, set_lambdas):
@@ -80,7 +71,8 @@ def kernel_work(W, H, spp, S, r, set_lambdas):
     T = S - 1
     acc_ops = P * spp * acc_ops_per_sample(set_lambdas)
     return {
-        # mode 2 tests each unordered pair once and weights every ordered pair
+        # mode 2 tests each unordered pair once and weights every ordered pair; the variance
+        # filter tests every ordered pair, and a pair's weight is the same for all its sets
         "welch_filter": {"ops": tests // 2 * WELCH_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         "var_filter": {"ops": tests * VAR_OPS_PER_TEST + pairs * WEIGHT_OPS_PER_PAIR, "bound": "valu"},
         # samples read once; a fresh call writes the statistics (n + 7 per set-channel), the
@@ -386,6 +378,9 @@ def main() -> int:
     ap.add_argument("--cpu-region", default="768,432,384,216", help="x0,y0,w,h of the CPU baseline's crop")
     ap.add_argument("--error-region", default="900,500,112,96",
                     help="x0,y0,w,h of the crop checked against the FP64 pipeline")
+    ap.add_argument("--valu-probe", type=int, default=0,
+                    help="only time this many launches of the VALU issue probe (stater_debug_valu_rate: 16 "
+                         "independent FP32 FMA chains per lane) and print its rate")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="GiB: only copy a device buffer of this size three times (hipMemcpy, device to device), "
                          "the known-size copy that calibrates the HBM traffic counters")
@@ -401,6 +396,33 @@ def main() -> int:
     ap.add_argument("--c4-mode", choices=["both", "plain", "batched"], default="both",
                     help="c4: time the plain path, the batch, or both")
     args = ap.parse_args()
+    if args.valu_probe > 0:
+        import ctypes as C
+
+        import stater_amd as sa
+        from stater_amd import hipmem
+
+        ctx = sa.Context(0)
+        blocks, iters = 256 * 64, 4096
+        out = hipmem.DeviceBuffer(blocks * 256 * 4)
+        lib = sa.load_library()
+
+        def launch():
+            sa.check(lib.stater_debug_valu_rate(ctx.h, blocks, iters, C.c_void_p(out.ptr), None))
+
+        launch()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.valu_probe):
+            launch()
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        lane_ops = blocks * 256 * iters * 16 * args.valu_probe
+        print(json.dumps({"valu_probe_tlane_op_s": round(lane_ops / dt / 1e12, 2), "launches": args.valu_probe,
+                          "ms_per_launch": round(dt * 1e3 / args.valu_probe, 3), "peak_used": PEAK_LANE_OPS}),
+              flush=True)
+        ctx.close()
+        return 0
     if args.copy_probe > 0:
         from stater_amd import hipmem
 
