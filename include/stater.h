@@ -495,14 +495,25 @@ int stater_debug_film_decisions(stater_bufset *bs, const stater_denoise_params *
                                 const stater_gbuffers *gbufs, const stater_plane *film,
                                 int plain, uint8_t *dec, void *stream);
 /* Checks the fast exact forms of the accumulation's untransformed update
- * against the IEEE ones over all 2^32 float bit patterns x (device counts[2],
+ * against the IEEE ones over all 2^32 float bit patterns x (device counts[3],
  * written by the call): counts[0] = non-negative x (-0 and NaN included)
  * outside (0, 2^-104) whose neighbour-checked square root differs from
- * sqrtf(x); counts[1] = x with
- * |x| >= 2^-40 or x = 0 whose quotient by a pseudo-random n in [1, 2^24]
- * through the FP64 reciprocal differs from x / n.  Both are 0 on a working
- * build; no reference counterpart (it checks this build's arithmetic). */
+ * sqrtf(x); counts[1] = finite x with |x| >= 2^-40 or x = 0 whose quotient
+ * by a pseudo-random n in [1, 2^24] through the correctly rounded FP32
+ * reciprocal differs from x / n; counts[2] = floats m in [1, 2^32] whose
+ * reciprocal (one Newton step from the hardware one) differs from 1.f / m.
+ * All three are 0 on a working build; no reference counterpart (it checks
+ * this build's arithmetic). */
 int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *counts, void *stream);
+/* Checks the accumulation's quotient form (a product with the correctly
+ * rounded reciprocal and one residual correction) against the IEEE x / m for
+ * every significand x in [1, 2) and every integer m in [m_lo, m_hi)
+ * (1 <= m_lo < m_hi <= 2^24 + 1): device counts[0] receives the number of
+ * differences, 0 on a working build.  m_lo = 2^23, m_hi = 2^24 + 1 covers
+ * every pair of significands (2^46 quotients, some seconds); no reference
+ * counterpart. */
+int stater_debug_quotient_sweep(stater_ctx *ctx, uint32_t m_lo, uint32_t m_hi, unsigned long long *counts,
+                                void *stream);
 /* VALU issue probe: blocks x 256 threads, each running 16 independent FP32
  * FMA chains (register operands only) for iters steps (16 x iters FMAs per
  * thread); out (device,

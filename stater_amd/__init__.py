@@ -152,6 +152,7 @@ SIGNATURES = [
     ("stater_debug_film_decisions", C.c_int, [_P, C.POINTER(DenoiseParams), C.POINTER(GBuffers),
                                               C.POINTER(Plane), C.c_int, _P, _P]),
     ("stater_debug_exact_forms", C.c_int, [_P, _P, _P]),
+    ("stater_debug_quotient_sweep", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     ("stater_debug_valu_rate", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     ("stater_server_default_config", None, [C.POINTER(ServerConfig)]),
     ("stater_server_create", C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),

@@ -135,7 +135,8 @@ constexpr float kLog2eAcc = 1.4426950408889634f;
 //     rho^3 - 1     = u (u (u + 3) + 3)   (Wt u taken first, so the
 //                                          product overflows only where x^3 does),
 // with the hardware square root and reciprocal (the square roots only in
-// the denominators).  About twenty instructions per sample-channel for the
+// the denominators; one reciprocal of (rho^(1/2) + 1)(rho^(1/4) + 1) serves
+// both quotients).  About twenty instructions per sample-channel for the
 // four sets against twelve per set for the log-based expm1.  As pow(x, l)
 // in the reference's boxCox (estimator.h:56): x = 0 gives -1/l (u = -1,
 // y - K = -Wt), a negative x gives NaN for the roots and the exact cube for
@@ -160,10 +161,14 @@ __device__ __forceinline__ BcPow bc_pow(float v, float x1, float ix1) {
     p.u = (v - x1) * ix1;
     if (bc_has(KC, T, kBcHalf, kBcThreeQuarter) || bc_has(KC, T, kBcQuarter, kBcQuarter)) {
         const float s2 = __builtin_amdgcn_sqrtf(v * ix1);  // rho itself: u + 1 loses a small rho's precision
-        p.h2 = p.u * __builtin_amdgcn_rcpf(s2 + 1.f);
         if (bc_has(KC, T, kBcQuarter, kBcThreeQuarter)) {
+            // one reciprocal for both denominators: h4 = u / ((s2 + 1)(s4 + 1)), h2 = h4 (s4 + 1)
             const float s4 = __builtin_amdgcn_sqrtf(s2);
-            p.h4 = p.h2 * __builtin_amdgcn_rcpf(s4 + 1.f);
+            const float t4 = s4 + 1.f;
+            p.h4 = p.u * __builtin_amdgcn_rcpf(__builtin_fmaf(s2, t4, t4));
+            p.h2 = __builtin_fmaf(p.h4, s4, p.h4);
+        } else {
+            p.h2 = p.u * __builtin_amdgcn_rcpf(s2 + 1.f);
         }
     }
     return p;
@@ -380,18 +385,19 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         ++n;
         const float nf = (float)n;
         const NFac f = nfac(nf);
-        // Set 0 takes div_by_n and sqrt_rn_large; a lane with a tiny
-        // deviation (never, in practice) redoes its quotients with the IEEE
-        // division before the update and its roots with sqrtf after it, so
-        // that the update itself stays one basic block.
+        // Set 0 takes div_rn and sqrt_rn_large; a lane with a tiny or an
+        // infinite deviation (never, in practice) redoes its quotients with
+        // the IEEE division before the update and its roots with sqrtf after
+        // it, so that the update itself stays one basic block.
         float d[3], dN[3], p[3], r1[3];
-        const double rn = recip64(nf, f.b);
+        const float y = recip_rn(nf);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             d[c] = v[c] - st[c][0][0];
-            dN[c] = div_by_n(d[c], rn);
+            dN[c] = div_rn(d[c], y, nf);
         }
-        const bool tiny = exact_tiny(d[0]) | exact_tiny(d[1]) | exact_tiny(d[2]);
+        const bool tiny = exact_tiny(d[0]) | exact_tiny(d[1]) | exact_tiny(d[2]) |
+                          (fmaxf(fmaxf(fabsf(d[0]), fabsf(d[1])), fabsf(d[2])) == __builtin_inff());
         if (__builtin_expect(tiny, 0)) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) dN[c] = d[c] / nf;
@@ -752,10 +758,13 @@ namespace stater {
 // outside (0, 2^-104), -0 and NaN included, where sqrt_rn_large(x) and
 // sqrtf(x) differ (NaN equal to NaN; a radicand d (d - d/n) is never
 // negative: the hardware root of a negative subnormal is -0, not NaN),
-// counts[1] the d without exact_tiny(d) where div_by_n, with recip64 of a
-// pseudo-random n in [1, 2^24], differs from d / n.
+// counts[1] the finite d without exact_tiny(d) where div_rn, with recip_rn of
+// a pseudo-random n in [1, 2^24], differs from d / n, and counts[2] the
+// floats nf in [1, 2^32] (the first 2^28 + 1 threads) where recip_rn(nf)
+// differs from 1.f / nf.
 __global__ void __launch_bounds__(256) k_exact_forms(uint64_t base, unsigned long long *counts) {
-    const uint32_t i = (uint32_t)(base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const uint64_t g = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = (uint32_t)g;
     const float x = __uint_as_float(i);
     auto same = [](float a, float b) { return __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b); };
     const bool in_range = x == 0.f || x >= 0x1p-104f || x != x;
@@ -765,8 +774,27 @@ __global__ void __launch_bounds__(256) k_exact_forms(uint64_t base, unsigned lon
     h *= 0x85EBCA77u;
     h ^= h >> 13;
     const float nf = (float)(1u + (h & 0xFFFFFFu));
-    if (!exact_tiny(x) && !same(div_by_n(x, recip64(nf, __builtin_amdgcn_rcpf(nf))), x / nf))
+    if (!exact_tiny(x) && fabsf(x) != __builtin_inff() && !same(div_rn(x, recip_rn(nf), nf), x / nf))
         atomicAdd(&counts[1], 1ull);
+    if (g < ((uint64_t)1 << 28)) {
+        const float m = __uint_as_float(0x3f800000u + i);  // 1 <= m < 2^32
+        if (!same(recip_rn(m), 1.f / m)) atomicAdd(&counts[2], 1ull);
+        if (g == 0 && !same(recip_rn(0x1p32f), 1.f / 0x1p32f)) atomicAdd(&counts[2], 1ull);
+    }
+}
+
+// div_rn(d, recip_rn(m), m) against the IEEE d / m for every significand
+// d in [1, 2) (2^23 values, thread t takes t, t + 256, ...) and every
+// integer m in [m0, m0 + gridDim.x): counts[0] counts the differences.
+__global__ void __launch_bounds__(256) k_quotient_sweep(uint32_t m0, unsigned long long *counts) {
+    const float m = (float)(m0 + blockIdx.x);
+    const float y = recip_rn(m);
+    uint32_t bad = 0;
+    for (uint32_t j = threadIdx.x; j < (1u << 23); j += 256) {
+        const float d = __uint_as_float(0x3f800000u | j);
+        bad += __float_as_uint(div_rn(d, y, m)) != __float_as_uint(d / m);
+    }
+    if (bad) atomicAdd(&counts[0], (unsigned long long)bad);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -780,9 +808,24 @@ extern "C" int stater_debug_exact_forms(stater_ctx *ctx, unsigned long long *cou
     if (!ctx || !counts) return fail(STATER_E_ARG, "stater_debug_exact_forms: NULL argument");
     STATER_HIP(hipSetDevice(ctx->device));
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    k_zero_u64<<<1, 64, 0, st>>>(counts, 2);
+    k_zero_u64<<<1, 64, 0, st>>>(counts, 3);
     const uint64_t per = (uint64_t)1 << 28;  // 2^20 blocks of 256 threads per launch, 16 launches
     for (uint64_t b = 0; b < ((uint64_t)1 << 32); b += per) k_exact_forms<<<(unsigned)(per / 256), 256, 0, st>>>(b, counts);
+    STATER_HIP(hipGetLastError());
+    return STATER_OK;
+}
+
+extern "C" int stater_debug_quotient_sweep(stater_ctx *ctx, uint32_t m_lo, uint32_t m_hi, unsigned long long *counts,
+                                           void *stream) {
+    using namespace stater;
+    if (!ctx || !counts) return fail(STATER_E_ARG, "stater_debug_quotient_sweep: NULL argument");
+    if (m_lo < 1 || m_hi <= m_lo || m_hi > (1u << 24) + 1)
+        return fail(STATER_E_ARG, "stater_debug_quotient_sweep: need 1 <= m_lo < m_hi <= 2^24 + 1");
+    STATER_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    k_zero_u64<<<1, 64, 0, st>>>(counts, 1);
+    const uint32_t per = 4096;  // 2^35 pairs per launch
+    for (uint32_t m = m_lo; m < m_hi; m += per) k_quotient_sweep<<<min(per, m_hi - m), 256, 0, st>>>(m, counts);
     STATER_HIP(hipGetLastError());
     return STATER_OK;
 }

@@ -1220,10 +1220,10 @@ struct DecideLayout {
 };
 
 // The decision pass's (M, V) of a pixel: welch_load with V = var/n through
-// the FP64 reciprocal rn of nf = max(n, 1), which gives the IEEE quotient bit
-// for bit (div_exact_by_n).
+// the correctly rounded reciprocal rn of nf = max(n, 1), which gives the IEEE
+// quotient bit for bit (div_exact_by_n).
 __device__ __forceinline__ void decide_load(const WelchArgs &a, size_t pix, int xq, int yq, int s, int c, int n,
-                                            float nf, double rn, float &M, float &V) {
+                                            float nf, float rn, float &M, float &V) {
     M = s == 0 ? *(const float *)((const char *)a.film + (int64_t)yq * a.film_pitch + (int64_t)xq * a.film_xs + c * a.film_cs)
                : a.mean[(size_t)s * a.set_stride + c * a.plane + pix];
     float var;
@@ -1252,7 +1252,7 @@ __device__ __forceinline__ void decide_stage_row(const WelchArgs &a, float *slot
             slot[HWC + col] = n >= 2 ? (float)(n - 1) : 1.f;
         }
         const float nf = (float)max(n, 1);
-        const double rn = recip64(nf, __builtin_amdgcn_rcpf(nf));
+        const float rn = recip_rn(nf);
         for (int sc = wave; sc < 3 * S; sc += waves) {
             const int s = sc / 3, c = sc - 3 * s;
             float M = 0.f, V = 0.f;
@@ -1450,7 +1450,7 @@ __device__ __forceinline__ void decide_tile(const WelchArgs a, int bx, int by, f
         nm1p = n >= 2 ? (float)(n - 1) : 1.f;
         rp = __builtin_amdgcn_rcpf(nm1p);
         const float nf = (float)max(n, 1);
-        const double rn = recip64(nf, __builtin_amdgcn_rcpf(nf));
+        const float rn = recip_rn(nf);
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll

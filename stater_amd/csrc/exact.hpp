@@ -7,22 +7,33 @@
 
 namespace stater {
 
-// The quotient d / n, correctly rounded, from an FP64 reciprocal rn of n
-// (within 2^-52 relative): RN32(d * rn) is RN32(d / n) wherever the quotient
-// is a normal float, because d / n lies at least 2^-49 (relative) away from
-// every midpoint of two adjacent floats there: d carries 24 significant bits
-// and n is an integer of at most 24 (times a power of two), so d - M n, for
-// a midpoint M with 25 significant bits, is a nonzero multiple of half of
-// M's last bit.  A subnormal quotient can fall on a midpoint exactly
-// (d = 3 x 2^-149, n = 2); it needs |d| < n 2^-126, which exact_tiny covers.
-__device__ __forceinline__ float div_by_n(float d, double rn) { return (float)((double)d * rn); }
+// 1/nf correctly rounded, for every float nf in [1, 2^32]: one Newton step
+// y0 + y0 (1 - nf y0) from the hardware reciprocal (checked against the IEEE
+// 1.f / nf on all 2^28 + 1 such floats by stater_debug_exact_forms).
+__device__ __forceinline__ float recip_rn(float nf) {
+    const float y0 = __builtin_amdgcn_rcpf(nf);
+    return __builtin_fmaf(__builtin_fmaf(-nf, y0, 1.f), y0, y0);
+}
 
-// 1/nf in FP64 from the FP32 reciprocal b = rcp(nf) (within a few ulp): two
-// Newton steps take its relative error e to e^2 and then below 2^-52.
-__device__ __forceinline__ double recip64(float nf, float b) {
-    const double n = (double)nf, y0 = (double)b;
-    const double y1 = __builtin_fma(y0, __builtin_fma(-n, y0, 1.0), y0);
-    return __builtin_fma(y1, __builtin_fma(-n, y1, 1.0), y1);
+// d / n correctly rounded, from y = recip_rn(nf), by Markstein's correction
+// step: q = d y is close to the quotient, the residual n q - d is exact in
+// the FMA, and q - (n q - d) y rounds to the IEEE quotient.  Three FP32
+// instructions where a correctly rounded FP64 product took three at half
+// rate (two conversions and a multiply).  Exact for every d with
+// 2^-40 <= |d| < inf (and d = 0 or NaN) and every float nf in [1, 2^32]:
+// the steps scale exactly with powers of two of d and of nf on that range
+// (no subnormal intermediate; recip_rn(2^k m) = 2^-k recip_rn(m), both being
+// correctly rounded), so the pairs of a significand of d in [1, 2) and an
+// integer m in [2^23, 2^24] stand for all of them, and
+// stater_debug_quotient_sweep checks those 2^46 pairs against the IEEE
+// division.  d = +-inf would give NaN, and a tiny d can have a subnormal
+// quotient that falls on a midpoint exactly (d = 3 x 2^-149, n = 2): callers
+// send both to the IEEE division.
+__device__ __forceinline__ float div_rn(float d, float y, float nf) {
+    const float q = d * y;
+    // q - (n q - d) y: the residual taken as n q - d keeps the sign of a zero
+    // d (d - n q would turn -0 / n into +0)
+    return __builtin_fmaf(-__builtin_fmaf(q, nf, -d), y, q);
 }
 
 // The correctly rounded square root of x >= 2^-104 (and of +-0, inf, NaN): the
@@ -45,15 +56,16 @@ __device__ __forceinline__ float sqrt_rn_large(float x) {
 // A deviation d = x - mean with 0 < |d| < 2^-40 (absurdly small against any
 // radiance) takes the IEEE division and sqrtf: everywhere else the quotient
 // d / n is a normal float or zero, and the radicand d (d - d/n) of m1 is 0
-// (n = 1) or at least 2^-81, where div_by_n and sqrt_rn_large are exact.
+// (n = 1) or at least 2^-81, where div_rn and sqrt_rn_large are exact.
 __device__ __forceinline__ bool exact_tiny(float d) { return d != 0.f && fabsf(d) < 0x1p-40f; }
 
 // x / n correctly rounded for any x and an integer-valued float n >= 1 with
-// FP64 reciprocal rn: div_by_n, except where the quotient could be subnormal
-// (0 < |x| < 2^-100), which takes the IEEE division.
-__device__ __forceinline__ float div_exact_by_n(float x, double rn, float nf) {
-    if (__builtin_expect(x != 0.f && fabsf(x) < 0x1p-100f, 0)) return x / nf;
-    return div_by_n(x, rn);
+// y = recip_rn(nf): div_rn, except where the quotient could be subnormal
+// (0 < |x| < 2^-100) or x is infinite, which take the IEEE division.
+__device__ __forceinline__ float div_exact_by_n(float x, float y, float nf) {
+    const float ax = fabsf(x);
+    if (__builtin_expect((x != 0.f && ax < 0x1p-100f) || ax == __builtin_inff(), 0)) return x / nf;
+    return div_rn(x, y, nf);
 }
 
 }  // namespace stater

@@ -210,14 +210,37 @@ def test_set0_exact_at_extreme_deviations(ctx, mm):
 
 def test_exact_forms_on_every_float(ctx):
     """The neighbour-checked square root equals sqrtf on every non-negative
-    float outside (0, 2^-104), -0 and NaN included, and the quotient through the FP64 reciprocal of n equals the
-    IEEE d / n for every float d outside 0 < |d| < 2^-40 (with a pseudo-random
-    n per d): the ranges on which set 0's update uses them."""
-    buf = hipmem.DeviceBuffer(16)
+    float outside (0, 2^-104), -0 and NaN included; the quotient through the
+    correctly rounded reciprocal of n equals the IEEE d / n for every finite
+    float d outside 0 < |d| < 2^-40 (with a pseudo-random n per d); and that
+    reciprocal equals 1.f / m for every float m in [1, 2^32]: the ranges on
+    which set 0's update uses them."""
+    buf = hipmem.DeviceBuffer(24)
     sa.check(sa.load_library().stater_debug_exact_forms(ctx.h, C.c_void_p(buf.ptr), None))
     ctx.synchronize()
-    counts = buf.download(np.uint64, 2)
-    assert counts.tolist() == [0, 0], counts
+    counts = buf.download(np.uint64, 3)
+    assert counts.tolist() == [0, 0, 0], counts
+
+
+def test_quotient_on_every_pair_of_significands(ctx):
+    """The quotient form equals the IEEE x / m for every significand x in
+    [1, 2) and every integer m in [2^23, 2^24]: with the reciprocal check
+    above and the exact scaling by powers of two, every quotient set 0's
+    update forms."""
+    lib = sa.load_library()
+    buf = hipmem.DeviceBuffer(8)
+    # argument checks
+    assert lib.stater_debug_quotient_sweep(ctx.h, 0, 5, C.c_void_p(buf.ptr), None) != 0
+    assert lib.stater_debug_quotient_sweep(ctx.h, 9, 9, C.c_void_p(buf.ptr), None) != 0
+    assert lib.stater_debug_quotient_sweep(ctx.h, 1, (1 << 24) + 2, C.c_void_p(buf.ptr), None) != 0
+    step = 1 << 20
+    for lo in range(1 << 23, (1 << 24) + 1, step):
+        hi = min(lo + step, (1 << 24) + 1)
+        sa.check(lib.stater_debug_quotient_sweep(ctx.h, lo, hi, C.c_void_p(buf.ptr), None))
+        ctx.synchronize()
+        bad = int(buf.download(np.uint64, 1)[0])
+        print(f"m in [{lo}, {hi}): {bad} differing", flush=True)
+        assert bad == 0
 
 
 def test_accumulate_in_chunks_continues_sequence(ctx):
