@@ -203,6 +203,7 @@ __device__ __forceinline__ float log_ratio(float x, float L) {
 struct NFac {
     float b, a1, a2, a3, b3, b4, b6, sa1;
 };
+constexpr int kFacSlots = 10;  // NFac, recip_rn(nf), nf
 
 __device__ __forceinline__ NFac nfac(float nf) {
     NFac f;
@@ -373,7 +374,9 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
     };
     float nx[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     load(0, nx);
-    for (int j = 0; j < cnt; ++j) {
+    // Sample j of the call with the per-sample factors of count nf (f, and
+    // y = recip_rn(nf) for set 0's quotients).
+    auto sample = [&](int j, const NFac &f, float nf, float y) {
         float v[3] = {nx[0], nx[1], nx[2]};
         float xD = 0.f, yD = 0.f;
         if (MM >= 5) {
@@ -382,15 +385,11 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         }
         if (j + 1 < cnt) load(j + 1, nx);
         if (a.sanitize && !keep_sample(v[0], v[1], v[2])) v[0] = v[1] = v[2] = 0.f;
-        ++n;
-        const float nf = (float)n;
-        const NFac f = nfac(nf);
         // Set 0 takes div_rn and sqrt_rn_large; a lane with a tiny or an
         // infinite deviation (never, in practice) redoes its quotients with
         // the IEEE division before the update and its roots with sqrtf after
         // it, so that the update itself stays one basic block.
         float d[3], dN[3], p[3], r1[3];
-        const float y = recip_rn(nf);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             d[c] = v[c] - st[c][0][0];
@@ -428,6 +427,43 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) st[c][0][6] += r1[c];
         }
+    };
+    // Where every lane of the wavefront starts from the same count (a fresh
+    // call, or one after uniform calls), sample j has the same count in every
+    // lane, and the factors of 64 consecutive counts are computed once, one
+    // count per lane, into LDS; each sample then reads its own from there (a
+    // broadcast read) instead of computing them: two transcendentals and a
+    // dozen other instructions per sample.  The block is one wavefront, so
+    // the LDS accesses are in program order; the lanes still active fill the
+    // 64 slots between them (all of them at once in a full wavefront).
+    __shared__ float fac[kFacSlots][64];
+    auto put = [&](int k, float nk) {
+        const NFac fk = nfac(nk);
+        fac[0][k] = fk.b, fac[1][k] = fk.a1, fac[2][k] = fk.a2, fac[3][k] = fk.a3;
+        fac[4][k] = fk.b3, fac[5][k] = fk.b4, fac[6][k] = fk.b6, fac[7][k] = fk.sa1;
+        fac[8][k] = recip_rn(nk), fac[9][k] = nk;
+    };
+    const bool uni = __all(n == __builtin_amdgcn_readfirstlane(n));
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < cnt; ++j) {
+        int k = lane;
+        if (uni) {
+            if ((j & 63) == 0) {  // slots i = 0 .. 63 <- count n + 1 + i, shared by the active lanes
+                const uint64_t act = __ballot(1);
+                const int m = __popcll(act);
+                const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                for (int i = r; i < 64; i += m) put(i, (float)(n + 1u + (uint32_t)i));
+            }
+            k = j & 63;
+        } else {
+            put(lane, (float)(n + 1u));  // own slot, own count
+        }
+        __builtin_amdgcn_wave_barrier();
+        const NFac f{fac[0][k], fac[1][k], fac[2][k], fac[3][k], fac[4][k], fac[5][k], fac[6][k], fac[7][k]};
+        ++n;
+        sample(j, f, fac[9][k], fac[8][k]);
+        __builtin_amdgcn_wave_barrier();
     }
     asm volatile("" : "+v"(pix));
 #pragma unroll

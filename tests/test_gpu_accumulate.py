@@ -265,6 +265,52 @@ def test_accumulate_in_chunks_continues_sequence(ctx):
     assert not n0.any() and not st0.any()
 
 
+def test_count_factor_paths(ctx):
+    """The per-sample factors of the sample count come from a table shared by
+    the wavefront where every lane starts from the same count, and from each
+    lane's own count otherwise.  A fresh call with ragged counts (pixels
+    without samples, a partial last wavefront, more than 64 samples, so that
+    the table is refilled by fewer lanes each time) takes the first; a call
+    continuing those unequal counts takes the second.  Set 0 stays bit-exact
+    with the oracle and the Box-Cox sets within their tolerance of FP64
+    through both."""
+    W, H, spp1, spp2 = 100, 3, 150, 70
+    rng = np.random.default_rng(23)
+
+    def samples(spp):
+        smp = rng.lognormal(0.0, 1.0, (spp, 5, H, W)).astype(np.float32)
+        smp[:, 3:] = rng.random((spp, 2, H, W), dtype=np.float32)
+        return smp
+
+    smp1, smp2 = samples(spp1), samples(spp2)
+    counts = rng.integers(-1, spp1 + 1, (H, W)).astype(np.int32)
+    counts[:, 10:20] = 0
+    counts[0] = spp1
+    c1, c2 = SampleChunk(W, H, spp1), SampleChunk(W, H, spp2)
+    c1.from_numpy(smp1)
+    c2.from_numpy(smp2)
+    cbuf = hipmem.DeviceBuffer(W * H * 4)
+    cplane = sa.plane(cbuf.ptr, W, H, 1, W * 4, W * H * 4, sa.STATER_I32)
+    hipmem.write_plane(cplane, counts[None])
+    bs = sa.BufferSet(ctx, W, H, MT5_LAMBDAS, 5)
+    ref32 = ref64 = None
+    for smp, desc in ((smp1, c1.desc(counts=cplane)), (smp2, c2.desc())):
+        bs.accumulate(desc)
+        ctx.synchronize()
+        n, st = read_stats(bs, MT5_LAMBDAS)
+        kw = {"counts": counts} if ref32 is None else {}
+        ref32 = oracle.accumulate(smp, MT5_LAMBDAS, 5, state=ref32, **kw)
+        ref64 = oracle.accumulate(smp, MT5_LAMBDAS, 5, fp64=True, state=ref64, **kw)
+        assert np.array_equal(n, ref32[0])
+        assert np.array_equal(st[0].view(np.uint32), ref32[1][0].view(np.uint32))
+        for s in range(1, st.shape[0]):
+            scale = _scales(ref64[1][s], n)
+            err = np.abs(st[s].astype(np.float64) - ref64[1][s]) / np.maximum(scale, 1e-30)
+            worst = float(np.max(np.where(n[None, None] > 0, err, 0.0)))
+            assert worst <= BOXCOX_RTOL, (s, worst)
+    assert len(set(ref32[0][0])) == 1 and len(set(ref32[0][1])) > 10  # uniform, then unequal counts
+
+
 def test_sanitize_and_counts(ctx):
     W, H, spp = 40, 24, 16
     rng = np.random.default_rng(5)
