@@ -385,6 +385,17 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         }
         if (j + 1 < cnt) load(j + 1, nx);
         if (a.sanitize && !keep_sample(v[0], v[1], v[2])) v[0] = v[1] = v[2] = 0.f;
+        // The Box-Cox sets' transforms first: they need no count factors, so
+        // they cover the latency of the factors' reads.
+        float dl[3] = {0.f, 0.f, 0.f};
+        BcPow pw[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            dl[c] = GEN ? log_ratio(v[c], L[c]) : 0.f;
+            pw[c] = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
+            // kept here, ahead of set 0's branches (the compiler would sink them past them)
+            if (T > 0) asm volatile("" : "+v"(dl[c]), "+v"(pw[c].u), "+v"(pw[c].h2), "+v"(pw[c].h4));
+        }
         // Set 0 takes div_rn and sqrt_rn_large; a lane with a tiny or an
         // infinite deviation (never, in practice) redoes its quotients with
         // the IEEE division before the update and its roots with sqrtf after
@@ -410,14 +421,11 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         if (T > 0) {
             const float xa = xD * f.a1, ya = yD * f.a1;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float dl = GEN ? log_ratio(v[c], L[c]) : 0.f;
-                const BcPow pw = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
-                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[c][s], v[c], a.lam_par[s]), f,
-                                      xa, ya);
-            }
+                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl[c], pw[c], Wt[c][s], v[c], a.lam_par[s]),
+                                      f, xa, ya);
         }
         if (MM >= 5) {
             if (__builtin_expect(tiny, 0)) {
