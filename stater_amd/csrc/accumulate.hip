@@ -394,7 +394,8 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             dl[c] = GEN ? log_ratio(v[c], L[c]) : 0.f;
             pw[c] = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
             // kept here, ahead of set 0's branches (the compiler would sink them past them)
-            if (T > 0) asm volatile("" : "+v"(dl[c]), "+v"(pw[c].u), "+v"(pw[c].h2), "+v"(pw[c].h4));
+            if (T > 0 && GEN) asm volatile("" : "+v"(dl[c]));
+            if (T > 0 && KC != 0) asm volatile("" : "+v"(pw[c].u), "+v"(pw[c].h2), "+v"(pw[c].h4));
         }
         // Set 0 takes div_rn and sqrt_rn_large; a lane with a tiny or an
         // infinite deviation (never, in practice) redoes its quotients with
