@@ -55,12 +55,25 @@ FeatArgs make_feats(const stater_gbuffers *g) {
     return f;
 }
 
-__device__ __forceinline__ float feat_at(const FeatArgs &f, int k, int x, int y) {
-    const char *row = (const char *)f.ptr[k] + (int64_t)y * f.pitch[k];
-    float v;
-    if (f.bf16 >> k & 1u) v = __uint_as_float((uint32_t)*(const uint16_t *)(row + (int64_t)x * 2) << 16);
-    else v = *(const float *)(row + (int64_t)x * 4);
-    return v * f.scale[k];
+constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
+
+// The kNF features of pixel (x, y), 0 beyond the set's nf.  Every load is
+// issued before any value is used (a bf16 value is widened afterwards), so
+// a pixel's features cost one memory latency, not one per feature.
+__device__ __forceinline__ void feats_at(const FeatArgs &f, int x, int y, float (&out)[kNF]) {
+    uint32_t raw[kNF];
+#pragma unroll
+    for (int k = 0; k < kNF; ++k) {
+        raw[k] = 0u;
+        if (k < f.nf) {
+            const char *row = (const char *)f.ptr[k] + (int64_t)y * f.pitch[k];
+            raw[k] = (f.bf16 >> k & 1u) ? (uint32_t)*(const uint16_t *)(row + (int64_t)x * 2)
+                                        : *(const uint32_t *)(row + (int64_t)x * 4);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kNF; ++k)
+        out[k] = __uint_as_float((f.bf16 >> k & 1u) ? raw[k] << 16 : raw[k]) * f.scale[k];
 }
 
 // ================================================================ prepass
@@ -94,7 +107,6 @@ struct PreOut {
 // LDS a record keeps its odd 16-byte stride, so the 16-byte reads of 8 (or
 // 16) consecutive lanes fall in distinct banks, and one base address serves
 // all of a pair's reads as immediate offsets.
-constexpr int kNF = 7;  // feature channels per pair weight (unused ones stay zero)
 
 // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
 // rounded up.  The filter kernels size their dynamic LDS at compile time, so
@@ -211,9 +223,8 @@ __global__ void __launch_bounds__(256) k_var_pack(VarArgs a, float4 *zero) {
     float4 *lrec = recs[wave] + lane * R::FLOAT4S;
     if (act) {
         const int n = a.n[pix];
-        float f[kNF];
-#pragma unroll
-        for (int k = 0; k < kNF; ++k) f[k] = HASW && k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+        float f[kNF] = {};
+        if (HASW) feats_at(a.f, x, y, f);
         const PreN q = pre_n(n, a.z);
         const float inv = n >= 2 ? 1.f / (float)(n - 1) : 0.f;
         const bool lev = a.lev != 0;
@@ -910,10 +921,12 @@ __device__ __forceinline__ void welch_stage_row(const WelchArgs &a, float *slot,
             }
             slot[col] = ok;
             slot[HWC + col] = nm1;
-            if (HASW)
+            if (HASW) {
+                float fv[kNF] = {};
+                if (in) feats_at(a.f, xq, yq, fv);
 #pragma unroll
-                for (int k = 0; k < kNF; ++k)
-                    slot[(LY::FEAT + k) * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
+                for (int k = 0; k < kNF; ++k) slot[(LY::FEAT + k) * HWC + col] = fv[k];
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 slot[(LY::FILM + c) * HWC + col] =
@@ -1136,9 +1149,7 @@ __device__ __forceinline__ void welch_tile(const WelchArgs a, int bx, int by, fl
         okp = n >= 2 ? 1.f : 0.f;
         nm1p = n >= 2 ? (float)(n - 1) : 1.f;
         rp = __builtin_amdgcn_rcpf(nm1p);
-        if (HASW)
-#pragma unroll
-            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+        if (HASW) feats_at(a.f, x, y, gp);
         if (MODE3) {
             float bj = a.jb[pix];
             for (int s = 1; s < S; ++s) {
@@ -1421,8 +1432,7 @@ __device__ __forceinline__ void welch_record_head(const WelchArgs &a, int x, int
         return;
     }
     float f[kNF];
-#pragma unroll
-    for (int k = 0; k < kNF; ++k) f[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+    feats_at(a.f, x, y, f);
     rec[0] = make_float4(film[0], film[1], film[2], f[0]);
     rec[1] = make_float4(f[1], f[2], f[3], f[4]);
     rec[2] = make_float4(f[5], f[6], 0.f, 0.f);
@@ -1522,9 +1532,12 @@ __device__ __forceinline__ void accum_stage_row(const WelchArgs &a, float *slot,
         const int xq = x0 - a.r + col;
         const bool in = xq >= 0 && xq < a.W && yq >= 0 && yq < a.H;
         if (kind == 0) {
-            if (HASW)
+            if (HASW) {
+                float fv[kNF] = {};
+                if (in) feats_at(a.f, xq, yq, fv);
 #pragma unroll
-                for (int k = 0; k < kNF; ++k) slot[k * HWC + col] = in && k < a.f.nf ? feat_at(a.f, k, xq, yq) : 0.f;
+                for (int k = 0; k < kNF; ++k) slot[k * HWC + col] = fv[k];
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 slot[(LY::FILM + c) * HWC + col] =
@@ -1618,9 +1631,7 @@ __device__ __forceinline__ void accum_tile(const WelchArgs a, int bx, int by, fl
 #pragma unroll
     for (int c = 0; c < 3; ++c) den[c] = 1.f, num[c] = 0.f;  // the pixel itself, weight 1
     if (active) {
-        if (HASW)
-#pragma unroll
-            for (int k = 0; k < kNF; ++k) gp[k] = k < a.f.nf ? feat_at(a.f, k, x, y) : 0.f;
+        if (HASW) feats_at(a.f, x, y, gp);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             num[c] = *(const float *)((const char *)a.film + (int64_t)y * a.film_pitch + (int64_t)x * a.film_xs + c * a.film_cs);
