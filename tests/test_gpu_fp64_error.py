@@ -8,7 +8,7 @@ Welch filters, FP64 pbrt Film) on the same samples, per output buffer:
 
 Bars (north star): <= 0.01 % of the accept/reject decisions differ, and every
 pixel-channel whose decisions all agree is within 1e-4 relative of FP64.
-The figures are printed (pytest -s) and recorded in DESIGN.md."""
+The figures are printed (pytest -s)."""
 import ctypes as C
 import json
 
