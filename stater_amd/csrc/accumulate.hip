@@ -121,6 +121,25 @@ __device__ __forceinline__ float wexpm1(float w, float Wt) {
 
 constexpr float kLog2eAcc = 1.4426950408889634f;
 
+// Wt (e^z - 1) for |z| <= 1/2 as (Wt z) p(z), p a degree-5
+// least-relative-error fit to (e^z - 1)/z on [-1/2, 1/2] (relative error
+// 9.6e-8 in exact arithmetic, 2.3e-7 evaluated in FP32): full-rate
+// instructions only, where wexpm1 needs a clamp, a rounding, a conversion
+// and an ldexp.  With z = l (ln x - L) and l = 0.01 (the log-based set of the
+// reference's lambda lists) it covers every ratio x / x1 in [e^-50, e^50]; a
+// zero, negative, infinite or NaN sample gives |z| > 1/2 or a NaN z, and its
+// lane takes the general form instead.
+constexpr float kExpm1SmallZ = 0.5f;
+__device__ __forceinline__ float wexpm1_small(float z, float Wt) {
+    float p = 1.3888798421248794e-03f;
+    p = __builtin_fmaf(p, z, 8.407683111727238e-03f);
+    p = __builtin_fmaf(p, z, 4.166870191693306e-02f);
+    p = __builtin_fmaf(p, z, 1.6665974259376526e-01f);
+    p = __builtin_fmaf(p, z, 4.999998211860657e-01f);
+    p = __builtin_fmaf(p, z, 1.0000001192092896f);
+    return (Wt * z) * p;
+}
+
 // Box-Cox sets whose lambda is 1/4, 1/2, 3/4 or 3 (every transformed set of
 // the reference's scene configurations but lambda = 0.01) take the shifted
 // value from rho = x / x1 without a logarithm,
@@ -148,6 +167,19 @@ __host__ __device__ constexpr bool bc_has(int KC, int T, int k0, int k1) {
     for (int s = 1; s <= T; ++s)
         if (bc_kind(KC, s) == k0 || bc_kind(KC, s) == k1) return true;
     return false;
+}
+// The one log-based set of a list whose other sets are all ratio sets (the
+// reference's multi-transform lists), or 0: that set takes wexpm1_small
+// where it can.
+__host__ __device__ constexpr int bc_single_generic(int KC, int T) {
+    if (KC == 0) return 0;
+    int sg = 0;
+    for (int s = 1; s <= T; ++s)
+        if (bc_kind(KC, s) == kBcGeneric) {
+            if (sg) return 0;
+            sg = s;
+        }
+    return sg;
 }
 
 struct BcPow {
@@ -306,6 +338,7 @@ __device__ __forceinline__ void clear_slots(const AccArgs &a, int S, int c, int 
 template <int T, int MM, int KC>
 __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
     constexpr bool GEN = KC == 0 || bc_has(KC, T, kBcGeneric, kBcGeneric);  // some set takes the log
+    constexpr int SG = bc_single_generic(KC, T);  // the log-based set taking wexpm1_small, or 0
     constexpr int NS = nstats(MM);
     if (x >= a.W) return;
     uint32_t pix = (uint32_t)y * (uint32_t)a.ld + (uint32_t)x;
@@ -337,6 +370,9 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
                 st[c][s][k] = a.fresh ? 0.f : a.stats[(((size_t)s * N_STATS + k) * 3 + c) * a.plane + pix];
 
     float L[3] = {0.f, 0.f, 0.f}, Wt[3][T + 1], x1[3] = {1.f, 1.f, 1.f}, ix1[3] = {1.f, 1.f, 1.f};
+    // z = l ln2 log2(x) - l L of set SG; l ln2 kept in a VGPR (an FMA reading an SGPR issues at half rate)
+    float gz0[3] = {0.f, 0.f, 0.f}, gz1 = SG ? a.lam[SG] * 0.69314718055994531f : 0.f;
+    if (SG) asm volatile("" : "+v"(gz1));
     if (T > 0) {
         float v0[3] = {0.f, 0.f, 0.f};
         if (n == 0) {
@@ -356,6 +392,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
                 x1[c] = a.lref[lp];
             }
             L[c] = __builtin_amdgcn_logf(x1[c]) * 0.69314718055994531f;
+            if (SG) gz0[c] = -a.lam[SG] * L[c];
 #pragma unroll
             for (int s = 1; s <= T; ++s) {
                 Wt[c][s] = expf(a.lam[s] * L[c]) * a.inv_lam[s];
@@ -391,11 +428,30 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
         BcPow pw[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            dl[c] = GEN ? log_ratio(v[c], L[c]) : 0.f;
+            dl[c] = GEN && !SG ? log_ratio(v[c], L[c]) : 0.f;
             pw[c] = bc_pow<T, KC>(v[c], x1[c], ix1[c]);
             // kept here, ahead of set 0's branches (the compiler would sink them past them)
-            if (T > 0 && GEN) asm volatile("" : "+v"(dl[c]));
+            if (T > 0 && GEN && !SG) asm volatile("" : "+v"(dl[c]));
             if (T > 0 && KC != 0) asm volatile("" : "+v"(pw[c].u), "+v"(pw[c].h2), "+v"(pw[c].h4));
+        }
+        // Set SG's shifted values: the small-z form, or the general one on a
+        // lane with a channel outside it.
+        float tg[3] = {0.f, 0.f, 0.f};
+        if (SG) {
+            bool wide = false;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float z = __builtin_fmaf(__builtin_amdgcn_logf(v[c]), gz1, gz0[c]);
+                tg[c] = wexpm1_small(z, Wt[c][SG]);
+                wide |= !(fabsf(z) <= kExpm1SmallZ);
+            }
+            if (__builtin_expect(wide, 0)) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    tg[c] = bc_shifted<KC>(SG, a.lam[SG], log_ratio(v[c], L[c]), pw[c], Wt[c][SG], v[c], a.lam_par[SG]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(tg[c]));
         }
         // Set 0 takes div_rn and sqrt_rn_large; a lane with a tiny or an
         // infinite deviation (never, in practice) redoes its quotients with
@@ -425,7 +481,9 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int s = 1; s <= T; ++s)
-                    meng_factored<MM>(st[c][s], bc_shifted<KC>(s, a.lam[s], dl[c], pw[c], Wt[c][s], v[c], a.lam_par[s]),
+                    meng_factored<MM>(st[c][s],
+                                      s == SG ? tg[c]
+                                              : bc_shifted<KC>(s, a.lam[s], dl[c], pw[c], Wt[c][s], v[c], a.lam_par[s]),
                                       f, xa, ya);
         }
         if (MM >= 5) {
@@ -517,6 +575,7 @@ __device__ __forceinline__ void merge64(double na, double *a, double nb, const d
 template <int T, int MM, int KC>
 __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
     constexpr bool GEN = KC == 0 || bc_has(KC, T, kBcGeneric, kBcGeneric);
+    constexpr int SG = bc_single_generic(KC, T);  // the log-based set taking wexpm1_small, or 0
     constexpr int NS = nstats(MM);
     constexpr int NP = MM >= 4 ? 4 : MM;  // partial moments kept
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -588,9 +647,15 @@ __device__ __forceinline__ void accm_tile(const AccArgs a, int bx, int by) {
         if (MM >= 5) fg = nfac((float)(n0 + j + 1));
         const float dl = T > 0 && GEN ? log_ratio(v, L) : 0.f;
         const BcPow pw = bc_pow<T, KC>(v, x1, ix1);
+        float tg = 0.f;  // set SG's shifted value, as in the sequential mode
+        if (SG) {
+            const float z = a.lam[SG] * dl;  // dl is of |v|: a negative v takes the general form's NaN
+            tg = fabsf(z) <= kExpm1SmallZ && v >= 0.f ? wexpm1_small(z, Wt[SG])
+                                                      : bc_shifted<KC>(SG, a.lam[SG], dl, pw, Wt[SG], v, a.lam_par[SG]);
+        }
 #pragma unroll
         for (int s = 0; s <= T; ++s) {
-            const float t = s == 0 ? v : bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[s], v, a.lam_par[s]);
+            const float t = s == 0 ? v : s == SG ? tg : bc_shifted<KC>(s, a.lam[s], dl, pw, Wt[s], v, a.lam_par[s]);
             meng_factored<NP>(pm[s], t, fr, 0.f, 0.f);
             if (MM >= 5) {
                 const float dg = t - mg[s];
