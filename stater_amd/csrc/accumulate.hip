@@ -150,7 +150,8 @@ __device__ __forceinline__ float wexpm1_small(float z, float Wt) {
 //     rho^(1/2) - 1 = u / (rho^(1/2) + 1)                        = h2,
 //                     (the root of rho = x / x1 itself, precise for small rho too)
 //     rho^(1/4) - 1 = h2 / (rho^(1/4) + 1)                       = h4,
-//     rho^(3/4) - 1 = h2 + h4 + h2 h4,
+//     rho^(3/4) - 1 = h2 rho^(1/4) + h4   (= h2 + h4 + h2 h4; both terms
+//                                          have the sign of u: no cancellation),
 //     rho^3 - 1     = u (u (u + 3) + 3)   (Wt u taken first, so the
 //                                          product overflows only where x^3 does),
 // with the hardware square root and reciprocal (the square roots only in
@@ -183,22 +184,22 @@ __host__ __device__ constexpr int bc_single_generic(int KC, int T) {
 }
 
 struct BcPow {
-    float u, h2, h4;  // rho - 1, rho^(1/2) - 1, rho^(1/4) - 1
+    float u, h2, h4, s4;  // rho - 1, rho^(1/2) - 1, rho^(1/4) - 1, rho^(1/4)
 };
 
 template <int T, int KC>
 __device__ __forceinline__ BcPow bc_pow(float v, float x1, float ix1) {
-    BcPow p{0.f, 0.f, 0.f};
+    BcPow p{0.f, 0.f, 0.f, 0.f};
     if (KC == 0) return p;
     p.u = (v - x1) * ix1;
     if (bc_has(KC, T, kBcHalf, kBcThreeQuarter) || bc_has(KC, T, kBcQuarter, kBcQuarter)) {
         const float s2 = __builtin_amdgcn_sqrtf(v * ix1);  // rho itself: u + 1 loses a small rho's precision
         if (bc_has(KC, T, kBcQuarter, kBcThreeQuarter)) {
             // one reciprocal for both denominators: h4 = u / ((s2 + 1)(s4 + 1)), h2 = h4 (s4 + 1)
-            const float s4 = __builtin_amdgcn_sqrtf(s2);
-            const float t4 = s4 + 1.f;
+            p.s4 = __builtin_amdgcn_sqrtf(s2);
+            const float t4 = p.s4 + 1.f;
             p.h4 = p.u * __builtin_amdgcn_rcpf(__builtin_fmaf(s2, t4, t4));
-            p.h2 = __builtin_fmaf(p.h4, s4, p.h4);
+            p.h2 = __builtin_fmaf(p.h4, p.s4, p.h4);
         } else {
             p.h2 = p.u * __builtin_amdgcn_rcpf(s2 + 1.f);
         }
@@ -221,7 +222,7 @@ __device__ __forceinline__ float bc_shifted(int s, float lam, float dl, const Bc
         return par == 0 ? __builtin_nanf("") : par == 1 ? e : -e - 2.f * Wt;
     }
     if (kd == kBcCube) return (Wt * p.u) * __builtin_fmaf(p.u, p.u + 3.f, 3.f);
-    const float h = kd == kBcQuarter ? p.h4 : kd == kBcHalf ? p.h2 : __builtin_fmaf(p.h2, p.h4, p.h2 + p.h4);
+    const float h = kd == kBcQuarter ? p.h4 : kd == kBcHalf ? p.h2 : __builtin_fmaf(p.h2, p.s4, p.h4);
     return Wt * h;
 }
 
@@ -433,6 +434,7 @@ __device__ __forceinline__ void accp_tile(const AccArgs a, int x, int y) {
             // kept here, ahead of set 0's branches (the compiler would sink them past them)
             if (T > 0 && GEN && !SG) asm volatile("" : "+v"(dl[c]));
             if (T > 0 && KC != 0) asm volatile("" : "+v"(pw[c].u), "+v"(pw[c].h2), "+v"(pw[c].h4));
+            if (T > 0 && bc_has(KC, T, kBcThreeQuarter, kBcThreeQuarter)) asm volatile("" : "+v"(pw[c].s4));
         }
         // Set SG's shifted values: the small-z form, or the general one on a
         // lane with a channel outside it.
