@@ -1330,17 +1330,27 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
         for (int k = 0; k <= kTCritDegShort; ++k) asm volatile("" : "+v"(tp.cs[k]));
         asm volatile("" : "+v"(tp.umax), "+v"(tp.neg_hi), "+v"(tp.neg_lo), "+v"(tp.band_abs), "+v"(tp.band_abs_short));
     }
-    for (int dx = dx0; dx <= a.r; ++dx) {
-        const int col = tx + a.r + dx;
-        const float2 *pq = (const float2 *)(slot + 2 * HWC) + col * (3 * S);
+    // The ring as an LDS pointer: 32-bit address arithmetic per column, where
+    // the generic pointer took two 64-bit multiply-adds.
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) const float lds_float;
+    typedef __attribute__((address_space(3))) const v2f lds_float2;
+    typedef __attribute__((address_space(3))) const v4f lds_float4;
+    // byte addresses in LDS of the first column's ok flag and operands, stepped per column
+    const uint32_t col0 = (uint32_t)(tx + a.r + dx0);
+    uint32_t aok = (uint32_t)(uintptr_t)(lds_float *)slot + col0 * 4u;
+    uint32_t aop = aok + (2u * HWC + col0 * (6u * S - 1u)) * 4u;
+    for (int dx = dx0; dx <= a.r; ++dx, aok += 4u, aop += 24u * S) {
+        lds_float2 *pq = (lds_float2 *)(uintptr_t)aop;
         float2 mv[S][3];
-        const float okq = slot[col];
-        const float nm1q = slot[HWC + col];
+        const float okq = *(lds_float *)(uintptr_t)aok;
+        const float nm1q = *(lds_float *)(uintptr_t)(aok + 4u * HWC);
         if ((3 * S) % 2 == 0) {  // whole 16-byte units per column (slot, 2 HWC floats and the column 16-byte aligned)
-            const float4 *p4 = (const float4 *)__builtin_assume_aligned(pq, 16);
+            lds_float4 *p4 = (lds_float4 *)pq;
 #pragma unroll
             for (int k = 0; k < 3 * S / 2; ++k) {
-                const float4 t = p4[k];
+                const v4f t = p4[k];
                 mv[(2 * k) / 3][(2 * k) % 3] = make_float2(t.x, t.y);
                 mv[(2 * k + 1) / 3][(2 * k + 1) % 3] = make_float2(t.z, t.w);
             }
@@ -1348,10 +1358,15 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
 #pragma unroll
             for (int s = 0; s < S; ++s)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) mv[s][c] = pq[s * 3 + c];
+                for (int c = 0; c < 3; ++c) {
+                    const v2f t = pq[s * 3 + c];
+                    mv[s][c] = make_float2(t.x, t.y);
+                }
         }
         __builtin_amdgcn_sched_barrier(0);
-        const PairR pr = pair_r(rp, __builtin_amdgcn_rcpf(nm1q));
+        // level 3's gaps need no 1/(n_q - 1): there only a near-threshold test reads it
+        PairR pr{};
+        if (LEVEL != 3) pr = pair_r(rp, __builtin_amdgcn_rcpf(nm1q));
         float R[3];
         bool nc[3];
         if (LEVEL == 0) {
@@ -1394,12 +1409,17 @@ __device__ __forceinline__ void decide_row(const WelchArgs &a, const float *slot
             }
         }
         if ((nc[0] || nc[1] || nc[2]) && okq < 0.f && okp) {
+            if (LEVEL == 3) {
+                float nq = nm1q;
+                asm volatile("" : "+v"(nq));  // keeps the reciprocal in this branch (the compiler hoisted it)
+                pr = pair_r(rp, __builtin_amdgcn_rcpf(nq));
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 if (!nc[c]) continue;
                 bool pass = true;
                 for (int s = 0; s < S; ++s) {
-                    const float2 m = pq[s * 3 + c];
+                    const v2f m = pq[s * 3 + c];
                     float e, f;
                     welch_margin<LEVEL == 0>(a.tp, Mp[s][c], Vp[s][c], m.x, m.y, pr, e, f);
                     if (e > 0.f) { pass = false; break; }
